@@ -1,0 +1,286 @@
+"""Seeded synthetic inputs in the reference's data layouts.
+
+The reference reads integrals from a Columbus SIFS file into two packed
+arrays (PsociIntegrals.C:426-540, 623-762):
+
+* ``valone[NNDAF(nbf)]``: one-electron h_pq at the 1-based lower-triangle index
+  ``max(p,q)*(max(p,q)-1)/2 + min(p,q)`` (p, q 1-based), stored 0-based;
+* ``valtwo[NNDAF(NNDAF(nbf))]``: two-electron (pq|rs) at the canonical 8-fold
+  index (``PsociIntegrals::canonicalIndex``, PsociIntegrals.C:892-901).
+
+and configurations from a text file (PsociConfigs.C:143-245): a title line, a
+line ``nbf nelec ksym ncfg nopen`` and one ``jsym occstring`` line per
+configuration, whose file order defines the SEF numbering
+(``readConfigsToGA``, PsociConfigs.C:336-495).
+
+This module makes such inputs from a seed, with the distributions of the
+benchmark workloads: h_pp sorted uniform[-2, 0.5] Eh (external h_pp
+uniform[0.5, 3]); h_pq ~ N(0, 0.05^2); (pp|qq) uniform[0.3, 0.8] (including
+(pp|pp)); exchange (pq|qp) uniform[0.01, 0.1]; other (pq|rs) ~ N(0, 0.01^2) with
+8-fold symmetry.  The spin-orbit coupling is drawn separately (N(0, 0.005^2))
+and returned but not applied: in a C1 double group every configuration has
+the same symmetry, so the reference never reaches its spin-orbit branch
+(PsociHamiltonian.C:223-238).
+"""
+from __future__ import annotations
+
+import itertools
+from dataclasses import dataclass, field
+from math import comb
+
+import numpy as np
+
+
+def nndaf(i: int) -> int:
+    return i * (i + 1) // 2
+
+
+def pair_index(p: int, q: int) -> int:
+    """0-based packed index of (p, q), p, q 0-based (TwoEIndex - 1)."""
+    a, b = (p, q) if p >= q else (q, p)
+    return a * (a + 1) // 2 + b
+
+
+def canonical_index(p: int, q: int, r: int, s: int) -> int:
+    """0-based canonical index of (pq|rs) (canonicalIndex - 1)."""
+    ij = pair_index(p, q) + 1
+    kl = pair_index(r, s) + 1
+    a, b = (ij, kl) if ij >= kl else (kl, ij)
+    return (a - 1) * a // 2 + b - 1
+
+
+@dataclass
+class Integrals:
+    nbf: int
+    valone: np.ndarray  # float64 [NNDAF(nbf)]
+    valtwo: np.ndarray  # float64 [NNDAF(NNDAF(nbf))]
+    ecore: float = 0.0
+    so: np.ndarray | None = None  # float64 [NNDAF(nbf)], not applied (C1)
+
+    def h(self, p, q):
+        return self.valone[pair_index(p, q)]
+
+    def g(self, p, q, r, s):
+        return self.valtwo[canonical_index(p, q, r, s)]
+
+    def dense(self):
+        """h[p,q] and (pq|rs) as dense arrays (for brute-force checks)."""
+        n = self.nbf
+        h = np.zeros((n, n))
+        for p in range(n):
+            for q in range(n):
+                h[p, q] = self.h(p, q)
+        pairs = np.array([[pair_index(p, q) for q in range(n)] for p in range(n)])
+        a = pairs[:, :, None, None] + 1
+        b = pairs[None, None, :, :] + 1
+        hi = np.maximum(a, b)
+        lo = np.minimum(a, b)
+        g = self.valtwo[(hi - 1) * hi // 2 + lo - 1]
+        return h, g
+
+
+def make_integrals(nbf: int, seed: int, n_internal: int | None = None) -> Integrals:
+    """Seeded integrals.  Orbitals >= n_internal are external (h_pp in [0.5, 3])."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nint = nbf if n_internal is None else n_internal
+    diag_int = np.sort(rng.uniform(-2.0, 0.5, size=nint))
+    diag_ext = np.sort(rng.uniform(0.5, 3.0, size=nbf - nint))
+    hdiag = np.concatenate([diag_int, diag_ext])
+    n1 = nndaf(nbf)
+    valone = np.empty(n1)
+    off = rng.normal(0.0, 0.05, size=n1)
+    for p in range(nbf):
+        for q in range(p + 1):
+            k = pair_index(p, q)
+            valone[k] = hdiag[p] if p == q else off[k]
+    # two-electron: classify every canonical (ij|kl), ij >= kl
+    n2 = nndaf(n1)
+    pi, pj = np.tril_indices(nbf)  # pair k -> (i >= j), k = i(i+1)/2 + j
+    order = np.argsort(pi * (pi + 1) // 2 + pj)
+    pi, pj = pi[order], pj[order]
+    IJ, KL = np.tril_indices(n1)
+    idx = IJ * (IJ + 1) // 2 + KL
+    same_pair = IJ == KL
+    i_diag = pi[IJ] == pj[IJ]
+    k_diag = pi[KL] == pj[KL]
+    coul = rng.uniform(0.3, 0.8, size=n2)
+    exch = rng.uniform(0.01, 0.1, size=n2)
+    other = rng.normal(0.0, 0.01, size=n2)
+    vals = np.where(i_diag & k_diag, coul[idx], np.where(same_pair, exch[idx], other[idx]))
+    valtwo = np.empty(n2)
+    valtwo[idx] = vals
+    so = rng.normal(0.0, 0.005, size=n1)
+    return Integrals(nbf=nbf, valone=valone, valtwo=valtwo, ecore=0.0, so=so)
+
+
+@dataclass
+class Configs:
+    nbf: int
+    nelec: int
+    ksym: int
+    occ: np.ndarray  # uint8 [ncfg][nbf], values 0/1/2, file order
+    jsym: np.ndarray  # int32 [ncfg]
+    title: str = "synthetic"
+    nopen_max: int = field(default=0)
+
+    def __post_init__(self):
+        self.occ = np.ascontiguousarray(self.occ, dtype=np.uint8)
+        self.jsym = np.ascontiguousarray(self.jsym, dtype=np.int32)
+        if self.nopen_max == 0 and len(self.occ):
+            self.nopen_max = int((self.occ == 1).sum(axis=1).max())
+
+    @property
+    def ncfg(self) -> int:
+        return int(self.occ.shape[0])
+
+    def nopen(self) -> np.ndarray:
+        return (self.occ == 1).sum(axis=1).astype(np.int32)
+
+    def nsef(self) -> np.ndarray:
+        s = self.nopen()
+        h = np.maximum(s - 1, 0) // 2
+        return (1 << h) * (1 << h)
+
+    def ndet(self) -> np.ndarray:
+        return 1 << np.maximum(self.nopen() - 1, 0)
+
+
+def _occupations(norb: int, nel: int):
+    """All 0/1/2 occupation tuples over norb orbitals with nel electrons, in
+    lexicographic order of the tuple read as digits (highest first)."""
+    out = []
+    for t in itertools.product((2, 1, 0), repeat=norb):
+        if sum(t) == nel:
+            out.append(t)
+    return out
+
+
+def fci_configs(nbf: int, nelec: int, jsym: int = 1, ksym: int = 1) -> Configs:
+    occ = np.array(_occupations(nbf, nelec), dtype=np.uint8).reshape(-1, nbf)
+    return Configs(nbf=nbf, nelec=nelec, ksym=ksym, occ=occ, jsym=np.full(len(occ), jsym, np.int32),
+                   title=f"FCI {nbf} orbitals {nelec} electrons")
+
+
+def _ext_patterns(next_: int, nel: int):
+    """External occupation patterns with nel (0, 1 or 2) electrons."""
+    if nel == 0:
+        return [()]
+    pats = []
+    if nel == 1:
+        for a in range(next_):
+            pats.append(((a, 1),))
+    elif nel == 2:
+        for a in range(next_):
+            pats.append(((a, 2),))
+        for b in range(next_):
+            for a in range(b):
+                pats.append(((a, 1), (b, 1)))
+    return pats
+
+
+def mrcisd_configs(n_internal: int, n_external: int, nelec: int, max_external: int = 2,
+                   jsym: int = 1, ksym: int = 1) -> Configs:
+    """Internal full CI plus every configuration with at most max_external
+    electrons in the external orbitals.  Order: by number of external
+    electrons, then external pattern, then internal occupation."""
+    nbf = n_internal + n_external
+    rows = []
+    for ne in range(0, max_external + 1):
+        internal = _occupations(n_internal, nelec - ne)
+        for pat in _ext_patterns(n_external, ne):
+            ext = np.zeros(n_external, dtype=np.uint8)
+            for a, v in pat:
+                ext[a] = v
+            for t in internal:
+                rows.append(np.concatenate([np.array(t, dtype=np.uint8), ext]))
+    occ = np.array(rows, dtype=np.uint8)
+    return Configs(nbf=nbf, nelec=nelec, ksym=ksym, occ=occ, jsym=np.full(len(occ), jsym, np.int32),
+                   title=f"MR-CISD {n_internal}+{n_external} orbitals {nelec} electrons")
+
+
+def write_configs(path: str, cfg: Configs) -> None:
+    """Write the reference's configuration file format (PsociConfigs.C:143-245)."""
+    with open(path, "w") as f:
+        f.write(cfg.title + "\n")
+        f.write(f"{cfg.nbf} {cfg.nelec} {cfg.ksym} {cfg.ncfg} {cfg.nopen_max}\n")
+        for js, row in zip(cfg.jsym, cfg.occ):
+            f.write(f"{int(js)} {''.join(str(int(v)) for v in row)}\n")
+
+
+class ConfigFileError(ValueError):
+    def __init__(self, msg: str, line: int):
+        super().__init__(f"line {line}: {msg}")
+        self.line = line
+
+
+def read_configs(path: str) -> Configs:
+    """Read a configuration file in file order, validating each line as the
+    reference reader does (PsociConfigs.C:143-245): the reported line number
+    counts the title and header lines."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    if not lines:
+        raise ConfigFileError("missing title", 1)
+    title = lines[0]
+    try:
+        nbf, nelec, ksym, ncfg, nopen = (int(x) for x in lines[1].split()[:5])
+    except (IndexError, ValueError):
+        raise ConfigFileError("expected 5 integers: nbf nelec ksym ncfgs nopen", 2) from None
+    occ, js = [], []
+    for n, line in enumerate(lines[2:], start=3):
+        if not line.strip():
+            continue
+        parts = line.split()
+        if len(parts) < 2:
+            raise ConfigFileError("expected 'jsym occupation'", n)
+        try:
+            sym = int(parts[0])
+        except ValueError:
+            raise ConfigFileError("bad symmetry", n) from None
+        s = parts[1]
+        if len(s) != nbf:
+            raise ConfigFileError(f"configuration length {len(s)} != nbf {nbf}", n)
+        if any(ch not in "012" for ch in s):
+            raise ConfigFileError("occupations must be 0, 1 or 2", n)
+        row = [int(ch) for ch in s]
+        if row.count(1) > nopen:
+            raise ConfigFileError("more open shells than the header allows", n)
+        if sum(row) != nelec:
+            raise ConfigFileError(f"electron count {sum(row)} != {nelec}", n)
+        occ.append(row)
+        js.append(sym)
+    if len(occ) != ncfg:
+        raise ConfigFileError(f"found {len(occ)} configurations, header says {ncfg}", len(lines))
+    return Configs(nbf=nbf, nelec=nelec, ksym=ksym, occ=np.array(occ, dtype=np.uint8).reshape(-1, nbf),
+                   jsym=np.array(js, dtype=np.int32), title=title, nopen_max=nopen)
+
+
+# The five benchmark workloads.  Seeds are fixed per workload.
+WORKLOADS = {
+    "fci6": dict(kind="fci", nbf=6, nelec=6, roots=4, seed=20261203),
+    "fci10": dict(kind="fci", nbf=10, nelec=8, roots=4, seed=20261204),
+    "mrci8x24": dict(kind="mrci", nint=8, next=24, nelec=6, roots=8, seed=20261205),
+    "mrci10x24": dict(kind="mrci", nint=10, next=24, nelec=8, roots=16, seed=20261207),
+}
+
+
+def workload(name: str):
+    w = WORKLOADS[name]
+    if w["kind"] == "fci":
+        cfg = fci_configs(w["nbf"], w["nelec"])
+        ints = make_integrals(w["nbf"], w["seed"])
+    else:
+        cfg = mrcisd_configs(w["nint"], w["next"], w["nelec"])
+        ints = make_integrals(w["nint"] + w["next"], w["seed"], n_internal=w["nint"])
+    return cfg, ints, w["roots"]
+
+
+def count_space(cfg: Configs):
+    """(ncfg, sum ndet, sum nsef, sum over configs of 2^nopen)."""
+    s = cfg.nopen()
+    return cfg.ncfg, int(cfg.ndet().sum()), int(cfg.nsef().sum()), int((1 << s).sum())
+
+
+__all__ = ["Integrals", "Configs", "make_integrals", "fci_configs", "mrcisd_configs", "write_configs",
+           "read_configs", "ConfigFileError", "workload", "WORKLOADS", "count_space", "nndaf", "pair_index",
+           "canonical_index", "comb"]
