@@ -1,0 +1,164 @@
+/*
+ * psoci.h -- C ABI of the MI355X-native PSOCI Hamiltonian / Davidson-sigma path.
+ *
+ * One context owns one GPU's copy of a determinant/SEF space, its integrals,
+ * an optional stored Hamiltonian and the Davidson work vectors.  Every entry
+ * point returns an int status (PSOCI_OK or a negative PSOCI_ERR_* code) and
+ * never aborts; psoci_last_error() gives the message of the last failure on
+ * that context.  Orbital and configuration indices are 0-based; packed
+ * integral arrays keep the reference's 1-based packing formulas (stored
+ * 0-based, see psoci_set_integrals).  Host arrays passed in are copied;
+ * output buffers are caller-allocated with sizes from psoci_space_info.
+ *
+ * Each entry point names the reference interface it replaces
+ * (psociRestart-Cori/<file>:<line> in the reference tree).
+ */
+#ifndef PSOCI_H
+#define PSOCI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct psoci_ctx psoci_ctx;
+
+enum {
+  PSOCI_OK = 0,
+  PSOCI_ERR_ARG = -1,         /* bad argument or size */
+  PSOCI_ERR_UNSUPPORTED = -2, /* input outside the implemented model */
+  PSOCI_ERR_STATE = -3,       /* call order: e.g. sigma before integrals */
+  PSOCI_ERR_HIP = -4,         /* HIP runtime failure */
+  PSOCI_ERR_NOMEM = -5,       /* device allocation failed */
+  PSOCI_ERR_CONVERGE = -6,    /* Davidson hit its iteration limit */
+  PSOCI_ERR_NUMERIC = -7      /* breakdown: linear dependence, non-SPD overlap */
+};
+
+enum { PSOCI_FP64 = 0, PSOCI_FP32 = 1, PSOCI_MIXED = 2 };
+enum { PSOCI_SIGMA_STORED = 0, PSOCI_SIGMA_DIRECT = 1 };
+
+/* Create a context on HIP device `device`.  No reference counterpart: the
+ * reference's state lives in GA-backed class instances created in the
+ * drivers (driver.PSOCI.splittest.C:280-760). */
+int psoci_ctx_create(int device, psoci_ctx** out);
+int psoci_ctx_destroy(psoci_ctx* ctx);
+const char* psoci_last_error(const psoci_ctx* ctx);
+
+/* Integrals after the SIFS read: replaces PsociIntegrals::fetchAndProcessIntegrals
+ * (PsociIntegrals.C:914-961) and its accessors fetch1eIntegralValue /
+ * fetch2eIntegralValue / fetchCoreEnergy (PsociIntegrals.C:853-873, 79-82).
+ * valone[NNDAF(nbf)]: h_pq at max(p,q)*(max(p,q)+1)/2 + min(p,q) (0-based p,q).
+ * valtwo[NNDAF(NNDAF(nbf))]: (pq|rs) at the canonical 8-fold index of
+ * PsociIntegrals::canonicalIndex (PsociIntegrals.C:892-901) minus one. */
+int psoci_set_integrals(psoci_ctx* ctx, int nbf, const double* valone, const double* valtwo, double ecore);
+
+/* Configuration list in file order: replaces PsociConfigs::readConfigsToGA
+ * (PsociConfigs.C:336-495) plus the determinant/SEF generation of
+ * PsociDeterminants::computeConfigs (PsociDeterminants.C:1083-1191) and
+ * computeNsefEntryPoints (PsociGAhamiltonian.C:4022-4048).
+ * occ[ncfg][nbf] holds 0/1/2 occupations, jsym[ncfg] the spatial symmetry.
+ * Implemented: even electron count, every configuration in the state's
+ * symmetry class, one common jsym (a C1 double group); others return
+ * PSOCI_ERR_UNSUPPORTED. nbf <= 64, at most 10 open shells. */
+int psoci_set_configs(psoci_ctx* ctx, int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ,
+                      const int32_t* jsym);
+
+/* Space sizes (findParams, PsociDeterminants.C:749-840): total SEFs (the H
+ * dimension), total determinants, and optionally per-configuration nsef and
+ * SEF offsets (arrays of length ncfg, may be NULL). */
+int psoci_space_info(psoci_ctx* ctx, int64_t* nsef_total, int64_t* ndet_total, int32_t* nsef, int64_t* nsef_offset);
+
+/* Determinants of configuration I in the reference's convention
+ * (JOUTFG after computePhaseAndSort, PsociDeterminants.hpp:60-87,
+ * PsociDeterminants.C:4355-4469): per determinant the alpha and beta
+ * occupation bit masks (bit p = orbital p), the Ms label (n_alpha-n_beta)/2
+ * and the phase bit.  Arrays of length ndet(I). */
+int psoci_dets(psoci_ctx* ctx, int64_t I, uint64_t* alpha, uint64_t* beta, int32_t* ms, int32_t* phase);
+
+/* One configuration-pair block on the GPU: PsociHamiltonian::generateSOCIblocks
+ * (PsociHamiltonian.C:401-558), no threshold.  sefsef is host memory,
+ * row-major nsef(I) x nsef(J); *status receives the reference's block status
+ * (0 = the reference skips the block). */
+int psoci_block(psoci_ctx* ctx, int64_t I, int64_t J, double* sefsef, int* status);
+
+/* Build the stored Hamiltonian on the GPU: constructGlobalHamiltonian-
+ * AltOrbMapDirectChunkJmap with packSefsef (PsociGAhamiltonian.C:2530-2857,
+ * 4077-4136).  The lower triangle + diagonal (diagonal last in each row,
+ * |h| >= 1e-8 or diagonal kept) is the reference's pattern; the context also
+ * keeps the symmetric full matrix and an FP32 copy for sigma.  *nnz_lower
+ * receives the reference's element count. */
+int psoci_build_h(psoci_ctx* ctx, int64_t* nnz_lower);
+
+/* Copy the stored lower triangle to host arrays: rowptr[n+1], col[nnz],
+ * val[nnz], diag[n] (any may be NULL).  Replaces the GA row fetch
+ * fetchLocalHblock (PsociGAhamiltonian.C:11791-11846) and fetchDIAG_SEF. */
+int psoci_get_h(psoci_ctx* ctx, int64_t* rowptr, int32_t* col, double* val, double* diag);
+
+/* Sigma Y = H X for k vectors: matrixVectorProductsSplitChunkMultipleJchunks
+ * (PsociGAhamiltonian.C:6921-7269; mode STORED) and the direct
+ * ...DirectFetch path (:7549-7823; mode DIRECT, elements regenerated on the
+ * fly).  X and Y are [n][k] row-major, FP64 for prec PSOCI_FP64 or FP32 for
+ * PSOCI_FP32 (FP32 H values, FP32 accumulation).  on_device != 0: X and Y
+ * are device pointers (no copies, launched on the context stream). */
+int psoci_sigma(psoci_ctx* ctx, int mode, int prec, int k, const void* X, void* Y, int on_device);
+
+/* Diagonal of H in the SEF basis: computeOnlyDiagElements
+ * (PsociGAhamiltonian.C:13171-13250).  diag[n] host. */
+int psoci_diag(psoci_ctx* ctx, double* diag);
+
+typedef struct {
+  int nroots;         /* k: lowest roots sought */
+  int prec;           /* PSOCI_FP64, PSOCI_FP32 (sigma only) or PSOCI_MIXED */
+  int sigma_mode;     /* PSOCI_SIGMA_STORED or PSOCI_SIGMA_DIRECT */
+  int max_iter;       /* block iterations */
+  int max_subspace;   /* basis size cap before collapse (0: 5*nroots) */
+  int guess;          /* 0: unit vectors on the k lowest diagonal elements;
+                         1: unit vectors on the first k SEFs (PsociGAvectors.C:687-726) */
+  double tol;         /* residual-norm tolerance for convergence */
+  double switch_tol;  /* PSOCI_MIXED: FP32 sigma until max residual < switch_tol */
+} psoci_dav_params;
+
+typedef struct {
+  int iterations;     /* block iterations */
+  int sigma_calls;    /* sigma launches (each covers a block of vectors) */
+  int sigma_vectors;  /* vectors multiplied */
+  int switch_iter;    /* iteration of the FP32->FP64 switch (-1: none) */
+  int converged;
+  double seconds;     /* wall time of the solve */
+  double sigma_seconds;
+  double max_rnorm;
+} psoci_dav_stats;
+
+/* Block Davidson on the GPU: replaces the production driver loop
+ * (driver.PSOCI.splittest.C:895-1380) with PsociGAbasis / PsociGAsubspace
+ * (subspace VtV/VtHV, dsygv, residuals, diagonal preconditioner with
+ * DAVIDSON_MIN = 1e-3, Ritz collapse).  evals[k] (without ecore), evecs
+ * [n][k] row-major host (may be NULL), stats may be NULL. */
+int psoci_davidson(psoci_ctx* ctx, const psoci_dav_params* params, double* evals, double* evecs,
+                   psoci_dav_stats* stats);
+
+/* Adjoint of E_k: dE/dtheta = c^T (dH/dtheta) c for every packed integral,
+ * accumulated over k vectors C[n][k] (row-major, host) with weights w[k]
+ * (NULL: all 1).  grad_one[NNDAF(nbf)], grad_two[NNDAF(NNDAF(nbf))];
+ * grad_so[NNDAF(nbf)] is the spin-orbit gradient (identically zero for one
+ * common jsym, where the reference never enters socig, PsociHamiltonian.C:223-238).
+ * No reference counterpart for the 2-RDM; the 1-RDM part relates to
+ * PsociNOpopulations::constructDensityMatrix (PsociNOpopulations.C:858-989)
+ * as grad_one[pp] = D_pp and grad_one[pq] = 2 D_pq (p != q). */
+int psoci_rdm(psoci_ctx* ctx, int k, const double* C, const double* w, double* grad_one, double* grad_two,
+              double* grad_so);
+
+/* Device handles of the context (for zero-copy callers and for timing with
+ * HIP events on the stream the library launches on): which = 0: diag
+ * (double[n]); which = 1: the context's hipStream_t. */
+int psoci_device_ptr(psoci_ctx* ctx, int which, void** ptr);
+
+/* Synchronise the context stream. */
+int psoci_sync(psoci_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSOCI_H */

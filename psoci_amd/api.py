@@ -1,0 +1,236 @@
+"""Python mirror of the C ABI (include/psoci.h) over ctypes.
+
+The class names follow the reference's objects for this path:
+``Hamiltonian`` stands for PsociGAhamiltonian (build, stored/direct sigma,
+diagonal), ``davidson`` for the driver loop over PsociGAbasis/PsociGAsubspace
+(driver.PSOCI.splittest.C:895-1380), ``rdm`` for the CI density
+(PsociNOpopulations.C:858-989) and its two-electron extension.  Every call
+goes to libpsoci_amd.so on the GPU; there is no CPU fallback: without the
+library or a HIP device the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpsoci_amd.so")
+HEADER = os.path.join(HERE, "..", "include", "psoci.h")
+
+OK, ERR_ARG, ERR_UNSUPPORTED, ERR_STATE, ERR_HIP, ERR_NOMEM, ERR_CONVERGE, ERR_NUMERIC = 0, -1, -2, -3, -4, -5, -6, -7
+FP64, FP32, MIXED = 0, 1, 2
+STORED, DIRECT = 0, 1
+
+_lib = None
+
+
+class PsociError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"psoci error {code}: {msg}")
+        self.code = code
+
+
+class DavParams(ctypes.Structure):
+    _fields_ = [("nroots", ctypes.c_int), ("prec", ctypes.c_int), ("sigma_mode", ctypes.c_int),
+                ("max_iter", ctypes.c_int), ("max_subspace", ctypes.c_int), ("guess", ctypes.c_int),
+                ("tol", ctypes.c_double), ("switch_tol", ctypes.c_double)]
+
+
+class DavStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("sigma_calls", ctypes.c_int), ("sigma_vectors", ctypes.c_int),
+                ("switch_iter", ctypes.c_int), ("converged", ctypes.c_int), ("seconds", ctypes.c_double),
+                ("sigma_seconds", ctypes.c_double), ("max_rnorm", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def header_functions() -> list[str]:
+    """Names of the entry points declared in include/psoci.h."""
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(psoci_\w+)\s*\(", txt, re.M)))
+
+
+def load(path: str | None = None):
+    """Load libpsoci_amd.so (raises if it is missing: no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(f"{p} is not built: run python -m psoci_amd.build")
+    L = ctypes.CDLL(p)
+    V, I, I64, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+    sig = {
+        "psoci_ctx_create": (I, [I, ctypes.POINTER(V)]),
+        "psoci_ctx_destroy": (I, [V]),
+        "psoci_last_error": (ctypes.c_char_p, [V]),
+        "psoci_set_integrals": (I, [V, I, V, V, D]),
+        "psoci_set_configs": (I, [V, I, I, I, I64, V, V]),
+        "psoci_space_info": (I, [V, V, V, V, V]),
+        "psoci_dets": (I, [V, I64, V, V, V, V]),
+        "psoci_block": (I, [V, I64, I64, V, V]),
+        "psoci_build_h": (I, [V, V]),
+        "psoci_get_h": (I, [V, V, V, V, V]),
+        "psoci_sigma": (I, [V, I, I, I, V, V, I]),
+        "psoci_diag": (I, [V, V]),
+        "psoci_davidson": (I, [V, ctypes.POINTER(DavParams), V, V, ctypes.POINTER(DavStats)]),
+        "psoci_rdm": (I, [V, I, V, V, V, V, V]),
+        "psoci_device_ptr": (I, [V, I, ctypes.POINTER(V)]),
+        "psoci_sync": (I, [V]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+@dataclass
+class SpaceInfo:
+    nsef_total: int
+    ndet_total: int
+    nsef: np.ndarray
+    sef_offset: np.ndarray
+
+
+class Hamiltonian:
+    """One GPU context holding a determinant/SEF space, its integrals and
+    (optionally) the stored Hamiltonian."""
+
+    def __init__(self, cfg, ints, device: int = 0):
+        self.L = load()
+        h = ctypes.c_void_p()
+        rc = self.L.psoci_ctx_create(device, ctypes.byref(h))
+        if rc != OK:
+            raise PsociError(rc, "psoci_ctx_create failed (no HIP device?)")
+        self._h = h
+        self.cfg = cfg
+        self.ints = ints
+        self._check(self.L.psoci_set_configs(self._h, cfg.nbf, cfg.nelec, cfg.ksym, cfg.ncfg, _p(cfg.occ),
+                                             _p(cfg.jsym)))
+        self.set_integrals(ints)
+        n, nd = ctypes.c_int64(), ctypes.c_int64()
+        nsef = np.zeros(cfg.ncfg, np.int32)
+        off = np.zeros(cfg.ncfg, np.int64)
+        self._check(self.L.psoci_space_info(self._h, ctypes.byref(n), ctypes.byref(nd), _p(nsef), _p(off)))
+        self.info = SpaceInfo(n.value, nd.value, nsef, off)
+        self.n = n.value
+        self.nnz_lower = None
+
+    def _check(self, rc):
+        if rc != OK:
+            raise PsociError(rc, self.L.psoci_last_error(self._h).decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self.L.psoci_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_integrals(self, ints):
+        one = np.ascontiguousarray(ints.valone, np.float64)
+        two = np.ascontiguousarray(ints.valtwo, np.float64)
+        self._check(self.L.psoci_set_integrals(self._h, ints.nbf, _p(one), _p(two), float(ints.ecore)))
+        self.ints = ints
+        self.nnz_lower = None
+
+    def dets(self, I: int):
+        nd = 1 << max(int((self.cfg.occ[I] == 1).sum()) - 1, 0)
+        a = np.zeros(nd, np.uint64)
+        b = np.zeros(nd, np.uint64)
+        ms = np.zeros(nd, np.int32)
+        ph = np.zeros(nd, np.int32)
+        self._check(self.L.psoci_dets(self._h, I, _p(a), _p(b), _p(ms), _p(ph)))
+        return dict(alpha=a, beta=b, ms=ms, phase=ph)
+
+    def block(self, I: int, J: int):
+        out = np.zeros((int(self.info.nsef[I]), int(self.info.nsef[J])), np.float64)
+        st = ctypes.c_int()
+        self._check(self.L.psoci_block(self._h, I, J, _p(out), ctypes.byref(st)))
+        return st.value, out
+
+    def build(self) -> int:
+        nnz = ctypes.c_int64()
+        self._check(self.L.psoci_build_h(self._h, ctypes.byref(nnz)))
+        self.nnz_lower = nnz.value
+        return nnz.value
+
+    def get_h(self):
+        n, nnz = self.n, self.nnz_lower
+        rowptr = np.zeros(n + 1, np.int64)
+        col = np.zeros(nnz, np.int32)
+        val = np.zeros(nnz, np.float64)
+        diag = np.zeros(n, np.float64)
+        self._check(self.L.psoci_get_h(self._h, _p(rowptr), _p(col), _p(val), _p(diag)))
+        return rowptr, col, val, diag
+
+    def diag(self):
+        d = np.zeros(self.n, np.float64)
+        self._check(self.L.psoci_diag(self._h, _p(d)))
+        return d
+
+    def sigma(self, X: np.ndarray, mode: int = STORED, prec: int = FP64):
+        dt = np.float32 if prec == FP32 else np.float64
+        X = np.ascontiguousarray(X, dt)
+        squeeze = X.ndim == 1
+        if squeeze:
+            X = X[:, None]
+        Y = np.zeros_like(X)
+        self._check(self.L.psoci_sigma(self._h, mode, prec, X.shape[1], _p(X), _p(Y), 0))
+        return Y[:, 0] if squeeze else Y
+
+    def sigma_device(self, X, Y, mode: int = STORED, prec: int = FP64):
+        """X, Y: torch tensors on the GPU, [n][k] contiguous."""
+        k = 1 if X.dim() == 1 else X.shape[1]
+        self._check(self.L.psoci_sigma(self._h, mode, prec, k, ctypes.c_void_p(X.data_ptr()),
+                                       ctypes.c_void_p(Y.data_ptr()), 1))
+        return Y
+
+    def davidson(self, nroots: int, prec: int = FP64, mode: int = STORED, tol: float = 1e-6,
+                 switch_tol: float = 1e-4, max_iter: int = 200, max_subspace: int = 0, guess: int = 0,
+                 want_vectors: bool = True):
+        p = DavParams(nroots, prec, mode, max_iter, max_subspace, guess, tol, switch_tol)
+        ev = np.zeros(nroots)
+        vec = np.zeros((self.n, nroots)) if want_vectors else None
+        st = DavStats()
+        rc = self.L.psoci_davidson(self._h, ctypes.byref(p), _p(ev), _p(vec), ctypes.byref(st))
+        if rc not in (OK, ERR_CONVERGE):
+            self._check(rc)
+        return dict(evals=ev, evecs=vec, stats=st.as_dict(), converged=rc == OK)
+
+    def stream_handle(self) -> int:
+        """The hipStream_t the library launches on (as an integer)."""
+        ptr = ctypes.c_void_p()
+        self._check(self.L.psoci_device_ptr(self._h, 1, ctypes.byref(ptr)))
+        return ptr.value or 0
+
+    def rdm(self, C: np.ndarray, weights=None):
+        """Gradients of sum_k w_k E_k with respect to the packed integrals."""
+        C = np.ascontiguousarray(C, np.float64)
+        if C.ndim == 1:
+            C = C[:, None]
+        k = C.shape[1]
+        w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+        nbf = self.cfg.nbf
+        n1 = nbf * (nbf + 1) // 2
+        g1 = np.zeros(n1)
+        g2 = np.zeros(n1 * (n1 + 1) // 2)
+        gso = np.zeros(n1)
+        self._check(self.L.psoci_rdm(self._h, k, _p(C), _p(w), _p(g1), _p(g2), _p(gso)))
+        return g1, g2, gso

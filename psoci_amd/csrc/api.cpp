@@ -1,0 +1,247 @@
+// extern "C" boundary of the PSOCI MI355X library (declared in include/psoci.h).
+// Every entry point catches psoci::Error and HIP failures and returns a
+// status code; the message is kept on the context for psoci_last_error().
+#include <cstring>
+
+#include "ctx.hpp"
+
+using psoci::Ctx;
+using psoci::Error;
+
+struct psoci_ctx {
+  Ctx c;
+};
+
+namespace {
+
+template <class F>
+int guarded(psoci_ctx* ctx, F&& f) {
+  if (!ctx) return PSOCI_ERR_ARG;
+  try {
+    PSOCI_HIP(hipSetDevice(ctx->c.device));
+    ctx->c.err.clear();
+    return f(ctx->c);
+  } catch (const Error& e) {
+    ctx->c.err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    ctx->c.err = e.what();
+    return PSOCI_ERR_ARG;
+  }
+}
+
+int64_t nndaf(int64_t i) { return i * (i + 1) / 2; }
+
+}  // namespace
+
+extern "C" {
+
+int psoci_ctx_create(int device, psoci_ctx** out) {
+  if (!out) return PSOCI_ERR_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return PSOCI_ERR_HIP;
+  if (device < 0 || device >= count) return PSOCI_ERR_ARG;
+  psoci_ctx* ctx = new psoci_ctx;
+  ctx->c.device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return PSOCI_ERR_HIP;
+  }
+  *out = ctx;
+  return PSOCI_OK;
+}
+
+int psoci_ctx_destroy(psoci_ctx* ctx) {
+  if (!ctx) return PSOCI_ERR_ARG;
+  (void)hipSetDevice(ctx->c.device);
+  (void)hipStreamSynchronize(ctx->c.stream);
+  delete ctx;
+  return PSOCI_OK;
+}
+
+const char* psoci_last_error(const psoci_ctx* ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+int psoci_set_integrals(psoci_ctx* ctx, int nbf, const double* valone, const double* valtwo, double ecore) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(nbf > 0 && nbf <= 64 && valone && valtwo, PSOCI_ERR_ARG, "bad integral arguments");
+    if (c.have_space) PSOCI_CHECK(nbf == c.sp.nbf, PSOCI_ERR_ARG, "nbf differs from the configuration list");
+    c.d_one.upload(valone, nndaf(nbf), c.stream);
+    c.d_two.upload(valtwo, nndaf(nndaf(nbf)), c.stream);
+    c.sync();
+    c.nbf_ints = nbf;
+    c.ecore = ecore;
+    c.have_ints = true;
+    c.have_h = c.have_diag = c.have_pairs = false;
+    return PSOCI_OK;
+  });
+}
+
+int psoci_set_configs(psoci_ctx* ctx, int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ,
+                      const int32_t* jsym) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(occ && jsym, PSOCI_ERR_ARG, "null configuration arrays");
+    if (c.have_ints) PSOCI_CHECK(nbf == c.nbf_ints, PSOCI_ERR_ARG, "nbf differs from the integrals");
+    c.have_space = false;
+    psoci::build_space(c.sp, nbf, nelec, ksym, ncfg, occ, jsym);
+    psoci::upload_space(c);
+    c.have_space = true;
+    c.have_h = c.have_diag = c.have_pairs = false;
+    return PSOCI_OK;
+  });
+}
+
+int psoci_space_info(psoci_ctx* ctx, int64_t* nsef_total, int64_t* ndet_total, int32_t* nsef, int64_t* nsef_offset) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_space, PSOCI_ERR_STATE, "no configuration list");
+    if (nsef_total) *nsef_total = c.sp.nsef_total;
+    if (ndet_total) *ndet_total = c.sp.ndet_total;
+    if (nsef) std::memcpy(nsef, c.sp.nsef.data(), c.sp.ncfg * sizeof(int32_t));
+    if (nsef_offset) std::memcpy(nsef_offset, c.sp.sef_off.data(), c.sp.ncfg * sizeof(int64_t));
+    return PSOCI_OK;
+  });
+}
+
+int psoci_dets(psoci_ctx* ctx, int64_t I, uint64_t* alpha, uint64_t* beta, int32_t* ms, int32_t* phase) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_space, PSOCI_ERR_STATE, "no configuration list");
+    PSOCI_CHECK(I >= 0 && I < c.sp.ncfg, PSOCI_ERR_ARG, "configuration index out of range");
+    const psoci::Space& sp = c.sp;
+    int s = sp.nopen[I];
+    const uint16_t* pat = sp.pat.data() + sp.pat_off[s];
+    for (int d = 0; d < sp.ndet[I]; ++d) {
+      uint64_t ao = 0, rest = sp.sm[I];
+      uint32_t pbits = pat[d];
+      while (rest) {
+        uint64_t low = rest & (~rest + 1);
+        if (pbits & 1u) ao |= low;
+        pbits >>= 1;
+        rest ^= low;
+      }
+      uint64_t a = sp.dm[I] | ao, b = sp.dm[I] | (sp.sm[I] & ~ao);
+      if (alpha) alpha[d] = a;
+      if (beta) beta[d] = b;
+      if (ms) ms[d] = (__builtin_popcountll(a) - __builtin_popcountll(b)) / 2;
+      if (phase) phase[d] = sp.cphase[I];
+    }
+    return PSOCI_OK;
+  });
+}
+
+int psoci_block(psoci_ctx* ctx, int64_t I, int64_t J, double* sefsef, int* status) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
+    PSOCI_CHECK(I >= 0 && I < c.sp.ncfg && J >= 0 && J < c.sp.ncfg && sefsef, PSOCI_ERR_ARG, "bad block arguments");
+    psoci::block_on_device(c, I, J, sefsef, status);
+    return PSOCI_OK;
+  });
+}
+
+int psoci_build_h(psoci_ctx* ctx, int64_t* nnz_lower) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
+    psoci::build_h(c);
+    if (nnz_lower) *nnz_lower = c.nnz_lower;
+    return PSOCI_OK;
+  });
+}
+
+int psoci_get_h(psoci_ctx* ctx, int64_t* rowptr, int32_t* col, double* val, double* diag) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_h, PSOCI_ERR_STATE, "no stored Hamiltonian");
+    psoci::get_h_lower(c, rowptr, col, val, diag);
+    return PSOCI_OK;
+  });
+}
+
+int psoci_diag(psoci_ctx* ctx, double* diag) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
+    PSOCI_CHECK(diag, PSOCI_ERR_ARG, "null output");
+    if (!c.have_diag) psoci::compute_diag(c);
+    c.d_diag.download(diag, c.sp.nsef_total, c.stream);
+    c.sync();
+    return PSOCI_OK;
+  });
+}
+
+int psoci_sigma(psoci_ctx* ctx, int mode, int prec, int k, const void* X, void* Y, int on_device) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
+    PSOCI_CHECK(k >= 1 && X && Y, PSOCI_ERR_ARG, "bad sigma arguments");
+    PSOCI_CHECK(prec == PSOCI_FP64 || prec == PSOCI_FP32, PSOCI_ERR_ARG, "sigma precision must be FP64 or FP32");
+    PSOCI_CHECK(mode == PSOCI_SIGMA_STORED || mode == PSOCI_SIGMA_DIRECT, PSOCI_ERR_ARG, "bad sigma mode");
+    if (mode == PSOCI_SIGMA_STORED) PSOCI_CHECK(c.have_h, PSOCI_ERR_STATE, "stored sigma needs psoci_build_h first");
+    if (mode == PSOCI_SIGMA_DIRECT && !c.have_pairs) psoci::build_pairs(c);
+    const int64_t n = c.sp.nsef_total;
+    const size_t es = prec == PSOCI_FP32 ? 4 : 8;
+    const void* dx = X;
+    void* dy = Y;
+    psoci::DevBuf<char> bx, by;
+    if (!on_device) {
+      bx.upload(static_cast<const char*>(X), (size_t)n * k * es, c.stream);
+      by.alloc((size_t)n * k * es);
+      dx = bx.p;
+      dy = by.p;
+    }
+    if (mode == PSOCI_SIGMA_STORED) psoci::sigma_stored(c, prec, k, dx, k, dy, k);
+    else psoci::sigma_direct(c, prec, k, dx, k, dy, k);
+    if (!on_device) by.download(static_cast<char*>(Y), (size_t)n * k * es, c.stream);
+    c.sync();
+    return PSOCI_OK;
+  });
+}
+
+int psoci_davidson(psoci_ctx* ctx, const psoci_dav_params* params, double* evals, double* evecs,
+                   psoci_dav_stats* stats) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
+    PSOCI_CHECK(params && evals, PSOCI_ERR_ARG, "null Davidson arguments");
+    return psoci::davidson(c, *params, evals, evecs, stats);
+  });
+}
+
+int psoci_rdm(psoci_ctx* ctx, int k, const double* C, const double* w, double* grad_one, double* grad_two,
+              double* grad_so) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
+    PSOCI_CHECK(k >= 1 && C, PSOCI_ERR_ARG, "bad RDM arguments");
+    const int64_t n = c.sp.nsef_total;
+    const int64_t n1 = nndaf(c.sp.nbf), n2 = nndaf(n1);
+    if (!c.have_pairs) psoci::build_pairs(c);
+    psoci::DevBuf<double> dC, g1, g2;
+    dC.upload(C, (size_t)n * k, c.stream);
+    g1.alloc(n1);
+    g2.alloc(n2);
+    psoci::rdm_grad(c, k, dC.p, w, g1.p, g2.p);
+    if (grad_one) g1.download(grad_one, n1, c.stream);
+    if (grad_two) g2.download(grad_two, n2, c.stream);
+    c.sync();
+    if (grad_so) std::memset(grad_so, 0, n1 * sizeof(double));
+    return PSOCI_OK;
+  });
+}
+
+int psoci_device_ptr(psoci_ctx* ctx, int which, void** ptr) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(ptr, PSOCI_ERR_ARG, "null output");
+    PSOCI_CHECK(which == 0 || which == 1, PSOCI_ERR_ARG, "unknown handle");
+    if (which == 1) {
+      *ptr = (void*)c.stream;
+      return PSOCI_OK;
+    }
+    PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
+    if (!c.have_diag) psoci::compute_diag(c);
+    *ptr = c.d_diag.p;
+    return PSOCI_OK;
+  });
+}
+
+int psoci_sync(psoci_ctx* ctx) {
+  return guarded(ctx, [&](Ctx& c) {
+    c.sync();
+    return PSOCI_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+// Shared host-side infrastructure of the PSOCI MI355X library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/psoci.h"
+
+namespace psoci {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define PSOCI_HIP(call)                                                                              \
+  do {                                                                                               \
+    hipError_t e_ = (call);                                                                          \
+    if (e_ != hipSuccess)                                                                            \
+      throw ::psoci::Error(PSOCI_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+#define PSOCI_CHECK(cond, code, msg)           \
+  do {                                         \
+    if (!(cond)) throw ::psoci::Error(code, msg); \
+  } while (0)
+
+// Device buffer owned by a context.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count == 0) return;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      throw Error(PSOCI_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+    }
+    n = count;
+  }
+  // grow only (keeps the allocation when large enough)
+  void reserve(size_t count) {
+    if (count > n) alloc(count);
+  }
+  void upload(const T* h, size_t count, hipStream_t s) {
+    alloc(count);
+    if (count) PSOCI_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+  void download(T* h, size_t count, hipStream_t s) const {
+    if (count) PSOCI_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+  }
+};
+
+// Host description of the determinant/SEF space.
+struct Space {
+  int nbf = 0, nelec = 0, ksym = 1, jsym0 = 1;
+  int64_t ncfg = 0;
+  double mult = 1.0;  // SEF combination sign: -1 when jsym and ksym parities differ
+  std::vector<uint64_t> dm, sm;   // doubly / singly occupied orbital masks
+  std::vector<int8_t> nopen;
+  std::vector<int32_t> nsef, ndet;
+  std::vector<int64_t> sef_off, det_off;
+  std::vector<uint8_t> cphase;
+  int64_t nsef_total = 0, ndet_total = 0;
+  std::vector<uint16_t> pat;  // determinant patterns, all open-shell counts
+  int pat_off[12] = {0};      // pat_off[s]: first pattern of s open shells
+  int max_open = 0;
+};
+
+// Builds Space from a configuration list; throws Error on unsupported input.
+void build_space(Space& sp, int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, const int32_t* jsym);
+
+// Library-wide kernel launch helpers (defined in the .hip files).
+void exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t n, hipStream_t s, void*& tmp, size_t& tmp_bytes);
+
+}  // namespace psoci

@@ -1,0 +1,82 @@
+// Context state of the PSOCI MI355X library (one GPU).
+#pragma once
+#include "common.hpp"
+#include "elem.hpp"
+
+namespace psoci {
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  Space sp;
+  bool have_space = false;
+  int nbf_ints = 0;
+  double ecore = 0.0;
+  bool have_ints = false;
+
+  // space on the device
+  DevBuf<uint64_t> d_dm, d_sm;
+  DevBuf<int8_t> d_nopen;
+  DevBuf<int32_t> d_nsef;
+  DevBuf<int64_t> d_sefoff;
+  DevBuf<uint8_t> d_cph;
+  DevBuf<uint16_t> d_pat;
+  DevBuf<int32_t> d_patoff;
+  DevBuf<double> d_one, d_two;
+
+  // stored Hamiltonian: symmetric full CSR (lower part first in each row)
+  bool have_h = false;
+  int64_t nnz_lower = 0, nnz_full = 0;
+  DevBuf<int64_t> d_rowptr;   // full, n+1
+  DevBuf<int32_t> d_col;      // full
+  DevBuf<double> d_val;       // full FP64
+  DevBuf<float> d_val32;      // full FP32
+  DevBuf<int32_t> d_lowcnt;   // per row: entries of the lower part (incl. diagonal)
+  DevBuf<double> d_diag;
+  bool have_diag = false;
+
+  // direct sigma: connected configuration pairs
+  bool have_pairs = false;
+  int64_t npairs = 0;
+  DevBuf<PairInfo> d_pairs;   // lower pairs J <= I, grouped by I ascending
+  DevBuf<int64_t> d_pairoff;  // per I: first pair (ncfg+1)
+
+  // scratch
+  void* scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+
+  ~Ctx() {
+    if (scan_tmp) (void)hipFree(scan_tmp);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void scan(const int64_t* in, int64_t* out, int64_t n) { exclusive_scan_i64(in, out, n, stream, scan_tmp, scan_tmp_bytes); }
+  void sync() { PSOCI_HIP(hipStreamSynchronize(stream)); }
+};
+
+// kernels-side entry points (hbuild.hip, sigma.hip, ...)
+void upload_space(Ctx& c);
+void build_pairs(Ctx& c);               // lower pair list + PairInfo (direct sigma / build)
+void build_h(Ctx& c);                   // stored Hamiltonian
+void compute_diag(Ctx& c);              // diagonal only
+void block_on_device(Ctx& c, int64_t I, int64_t J, double* host_out, int* status);
+void sigma_stored(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy);
+void sigma_direct(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy);
+void get_h_lower(Ctx& c, int64_t* rowptr, int32_t* col, double* val, double* diag);
+void rdm_grad(Ctx& c, int k, const double* dC, const double* w, double* d_g1, double* d_g2);
+
+struct DavParams;
+int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, psoci_dav_stats* st);
+
+inline ConfView conf_view_host(const Space& sp, int64_t c, const uint16_t* pat_dev) {
+  ConfView v;
+  v.d = sp.dm[c];
+  v.s = sp.sm[c];
+  v.nopen = sp.nopen[c];
+  v.ph = sp.cphase[c];
+  v.pat = pat_dev + sp.pat_off[v.nopen];
+  return v;
+}
+
+}  // namespace psoci

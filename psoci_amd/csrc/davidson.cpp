@@ -1,0 +1,306 @@
+// Block Davidson on one GPU with an FP32 -> FP64 sigma switch.
+//
+// Replaces the production driver loop (driver.PSOCI.splittest.C:895-1380)
+// built on PsociGAbasis / PsociGAsubspace.  The reference corrects one root
+// per iteration (the lowest unconverged, curroot) with a single new vector,
+// modified Gram-Schmidt and a GA gop per dot product; here every
+// unconverged root of the block gets its correction in the same iteration,
+// so one sigma launch multiplies the whole block, and all products over the
+// CI dimension are single GPU launches (FP64 MFMA for V^T W).  The numerics
+// kept from the reference: Rayleigh-Ritz on V^T H V, residual
+// r = sum_j y_j (HV_j - lambda V_j) (PsociGAsubspace.C:874-989), the diagonal
+// preconditioner r_i / (D_ii - lambda) applied where |D_ii - lambda| > 1e-3
+// (DAVIDSON_MIN, PsociGAbasis.C:1860-1920), and Ritz collapse that
+// transforms HV instead of recomputing it (expandAndReplaceVectorsNew,
+// PsociGAbasis.C:483-579).
+//
+// Mixed precision: sigma runs on the FP32 copy of H with FP32 vectors and
+// FP32 accumulation until every residual norm is below switch_tol; then the
+// basis collapses to the current Ritz vectors, their sigma is recomputed in
+// FP64, and FP64 sigma continues to convergence.  Subspace products,
+// orthogonalisation, residuals and Rayleigh quotients are FP64 throughout.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+
+#include "ctx.hpp"
+#include "linalg.hpp"
+
+namespace psoci {
+
+bool sym_eig(int m, double* a, double* w) {
+  // cyclic Jacobi with threshold; a (m x m row-major) -> eigenvectors in columns
+  std::vector<double> v((size_t)m * m, 0.0), s(a, a + (size_t)m * m);
+  for (int i = 0; i < m; ++i) v[(size_t)i * m + i] = 1.0;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int p = 0; p < m; ++p)
+      for (int q = 0; q < m; ++q) {
+        double x = s[(size_t)p * m + q] * s[(size_t)p * m + q];
+        tot += x;
+        if (p != q) off += x;
+      }
+    if (off <= 1e-32 * (tot > 0 ? tot : 1.0)) break;
+    for (int p = 0; p < m - 1; ++p)
+      for (int q = p + 1; q < m; ++q) {
+        double apq = s[(size_t)p * m + q];
+        if (apq == 0.0) continue;
+        double app = s[(size_t)p * m + p], aqq = s[(size_t)q * m + q];
+        double tau = (aqq - app) / (2.0 * apq);
+        double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+        double cs = 1.0 / std::sqrt(1.0 + t * t), sn = t * cs;
+        for (int r = 0; r < m; ++r) {
+          double srp = s[(size_t)r * m + p], srq = s[(size_t)r * m + q];
+          s[(size_t)r * m + p] = cs * srp - sn * srq;
+          s[(size_t)r * m + q] = sn * srp + cs * srq;
+        }
+        for (int r = 0; r < m; ++r) {
+          double spr = s[(size_t)p * m + r], sqr = s[(size_t)q * m + r];
+          s[(size_t)p * m + r] = cs * spr - sn * sqr;
+          s[(size_t)q * m + r] = sn * spr + cs * sqr;
+        }
+        for (int r = 0; r < m; ++r) {
+          double vrp = v[(size_t)r * m + p], vrq = v[(size_t)r * m + q];
+          v[(size_t)r * m + p] = cs * vrp - sn * vrq;
+          v[(size_t)r * m + q] = sn * vrp + cs * vrq;
+        }
+      }
+  }
+  std::vector<int> ord(m);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::sort(ord.begin(), ord.end(), [&](int x, int y) { return s[(size_t)x * m + x] < s[(size_t)y * m + y]; });
+  for (int c = 0; c < m; ++c) {
+    w[c] = s[(size_t)ord[c] * m + ord[c]];
+    for (int r = 0; r < m; ++r) a[(size_t)r * m + c] = v[(size_t)r * m + ord[c]];
+  }
+  for (int c = 0; c < m; ++c)
+    if (!std::isfinite(w[c])) return false;
+  return true;
+}
+
+namespace {
+
+struct Timer {
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  double sec() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+}  // namespace
+
+int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, psoci_dav_stats* st) {
+  Timer total;
+  const int64_t n = c.sp.nsef_total;
+  const int k = p.nroots;
+  PSOCI_CHECK(k >= 1 && k <= 64 && k <= n, PSOCI_ERR_ARG, "nroots must be in 1..min(64, n)");
+  PSOCI_CHECK(p.prec == PSOCI_FP64 || p.prec == PSOCI_FP32 || p.prec == PSOCI_MIXED, PSOCI_ERR_ARG, "bad precision");
+  PSOCI_CHECK(p.sigma_mode == PSOCI_SIGMA_STORED || p.sigma_mode == PSOCI_SIGMA_DIRECT, PSOCI_ERR_ARG, "bad sigma mode");
+  PSOCI_CHECK(p.tol > 0, PSOCI_ERR_ARG, "tol must be positive");
+  int mmax = p.max_subspace > 0 ? p.max_subspace : 5 * k;
+  mmax = std::max(mmax, 3 * k);
+  if (mmax > n) mmax = (int)n;
+  const int kc = std::min(2 * k, mmax - k > k ? mmax - k : k);
+  const int maxit = p.max_iter > 0 ? p.max_iter : 200;
+  if (p.sigma_mode == PSOCI_SIGMA_STORED)
+    PSOCI_CHECK(c.have_h, PSOCI_ERR_STATE, "stored sigma needs psoci_build_h first");
+  if (p.sigma_mode == PSOCI_SIGMA_DIRECT && !c.have_pairs) build_pairs(c);
+  if (!c.have_diag) compute_diag(c);
+
+  Linalg L;
+  L.stream = c.stream;
+  DevBuf<double> V, HV, R, T;
+  V.alloc((size_t)n * mmax);
+  HV.alloc((size_t)n * mmax);
+  R.alloc((size_t)n * k);
+  T.alloc((size_t)n * std::max(kc, k));
+  DevBuf<float> X32, Y32;
+  const bool want32 = p.prec != PSOCI_FP64;
+  if (want32) {
+    X32.alloc((size_t)n * k);
+    Y32.alloc((size_t)n * k);
+  }
+  psoci_dav_stats S{};
+  S.switch_iter = -1;
+
+  // guess vectors
+  std::vector<int64_t> idx(k);
+  if (p.guess == 1) {
+    std::iota(idx.begin(), idx.end(), 0);
+  } else {
+    std::vector<double> D(n);
+    c.d_diag.download(D.data(), n, c.stream);
+    c.sync();
+    std::vector<int64_t> ord(n);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::partial_sort(ord.begin(), ord.begin() + k, ord.end(),
+                      [&](int64_t a, int64_t b) { return D[a] < D[b] || (D[a] == D[b] && a < b); });
+    std::copy(ord.begin(), ord.begin() + k, idx.begin());
+  }
+  L.unit_cols(n, k, idx.data(), V.p, mmax);
+
+  bool phase32 = (p.prec == PSOCI_FP32 || p.prec == PSOCI_MIXED);
+  const double switch_tol = p.switch_tol > 0 ? p.switch_tol : 1e-4;
+  auto do_sigma = [&](int c0, int nb) {
+    Timer ts;
+    if (phase32) {
+      L.to_f32(n, nb, V.p + c0, mmax, X32.p, nb);
+      if (p.sigma_mode == PSOCI_SIGMA_STORED) sigma_stored(c, PSOCI_FP32, nb, X32.p, nb, Y32.p, nb);
+      else sigma_direct(c, PSOCI_FP32, nb, X32.p, nb, Y32.p, nb);
+      L.to_f64(n, nb, Y32.p, nb, HV.p + c0, mmax);
+    } else {
+      if (p.sigma_mode == PSOCI_SIGMA_STORED) sigma_stored(c, PSOCI_FP64, nb, V.p + c0, mmax, HV.p + c0, mmax);
+      else sigma_direct(c, PSOCI_FP64, nb, V.p + c0, mmax, HV.p + c0, mmax);
+    }
+    c.sync();
+    S.sigma_seconds += ts.sec();
+    S.sigma_calls += 1;
+    S.sigma_vectors += nb;
+  };
+
+  std::vector<double> A((size_t)mmax * mmax, 0.0), tmp;
+  auto update_A = [&](int c0, int nb) {
+    int m = c0 + nb;
+    tmp.resize((size_t)m * nb);
+    L.gemm_tn_host(n, m, nb, V.p, mmax, HV.p + c0, mmax, tmp.data());
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < nb; ++j) A[(size_t)i * mmax + c0 + j] = tmp[(size_t)i * nb + j];
+    for (int i = c0; i < m; ++i)
+      for (int j = c0; j < i; ++j) {
+        double s = 0.5 * (A[(size_t)i * mmax + j] + A[(size_t)j * mmax + i]);
+        A[(size_t)i * mmax + j] = A[(size_t)j * mmax + i] = s;
+      }
+    for (int i = 0; i < c0; ++i)
+      for (int j = c0; j < m; ++j) A[(size_t)j * mmax + i] = A[(size_t)i * mmax + j];
+  };
+
+  int m = k;
+  do_sigma(0, k);
+  update_A(0, k);
+
+  std::vector<double> Y, w, Yk, Yk2, G, P;
+  auto solve = [&](int mm) {
+    Y.assign((size_t)mm * mm, 0.0);
+    w.assign(mm, 0.0);
+    for (int i = 0; i < mm; ++i)
+      for (int j = 0; j < mm; ++j) Y[(size_t)i * mm + j] = A[(size_t)i * mmax + j];
+    PSOCI_CHECK(sym_eig(mm, Y.data(), w.data()), PSOCI_ERR_NUMERIC, "subspace eigensolver failed");
+  };
+  auto cols_of = [&](int mm, int first, int cnt, std::vector<double>& out) {
+    out.assign((size_t)mm * cnt, 0.0);
+    for (int i = 0; i < mm; ++i)
+      for (int j = 0; j < cnt; ++j) out[(size_t)i * cnt + j] = Y[(size_t)i * mm + first + j];
+  };
+  // collapse V, HV onto Ritz vectors 0..nc-1 of the current solve (Y, w)
+  auto collapse = [&](int nc) {
+    std::vector<double> Yc;
+    cols_of(m, 0, nc, Yc);
+    L.gemm_nn(n, m, nc, V.p, mmax, Yc.data(), 0.0, T.p, nc);
+    L.copy_cols(n, nc, T.p, nc, V.p, mmax);
+    L.gemm_nn(n, m, nc, HV.p, mmax, Yc.data(), 0.0, T.p, nc);
+    L.copy_cols(n, nc, T.p, nc, HV.p, mmax);
+    std::fill(A.begin(), A.end(), 0.0);
+    for (int i = 0; i < nc; ++i) A[(size_t)i * mmax + i] = w[i];
+    m = nc;
+  };
+  // symmetric orthonormalisation of columns [c0, c0+nb) of V in place (via T)
+  auto sym_orth = [&](int c0, int nb) -> int {
+    G.resize((size_t)nb * nb);
+    L.gemm_tn_host(n, nb, nb, V.p + c0, mmax, V.p + c0, mmax, G.data());
+    std::vector<double> lam(nb);
+    if (!sym_eig(nb, G.data(), lam.data())) return 0;
+    double lmax = lam[nb - 1];
+    std::vector<int> keep;
+    for (int j = nb - 1; j >= 0; --j)
+      if (lam[j] > 1e-14 * lmax && lam[j] > 1e-28) keep.push_back(j);
+    int nk = (int)keep.size();
+    if (nk == 0) return 0;
+    std::vector<double> M((size_t)nb * nk);
+    for (int i = 0; i < nb; ++i)
+      for (int j = 0; j < nk; ++j) M[(size_t)i * nk + j] = G[(size_t)i * nb + keep[j]] / std::sqrt(lam[keep[j]]);
+    L.copy_cols(n, nb, V.p + c0, mmax, T.p, nb);
+    L.gemm_nn(n, nb, nk, T.p, nb, M.data(), 0.0, V.p + c0, mmax);
+    return nk;
+  };
+  auto project_out = [&](int c0, int nb) {
+    if (c0 == 0) return;
+    P.resize((size_t)c0 * nb);
+    L.gemm_tn_host(n, c0, nb, V.p, mmax, V.p + c0, mmax, P.data());
+    for (auto& x : P) x = -x;
+    L.gemm_nn(n, c0, nb, V.p, mmax, P.data(), 1.0, V.p + c0, mmax);
+  };
+
+  std::vector<double> rn(k, 0.0);
+  bool converged = false;
+  int it = 0;
+  for (it = 1; it <= maxit; ++it) {
+    solve(m);
+    cols_of(m, 0, k, Yk);
+    Yk2 = Yk;
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < k; ++j) Yk2[(size_t)i * k + j] *= w[j];
+    L.gemm_nn2(n, m, k, HV.p, mmax, Yk.data(), V.p, mmax, Yk2.data(), R.p, k);
+    G.resize((size_t)k * k);
+    L.gemm_tn_host(n, k, k, R.p, k, R.p, k, G.data());
+    double maxr = 0.0;
+    for (int j = 0; j < k; ++j) {
+      rn[j] = std::sqrt(std::max(G[(size_t)j * k + j], 0.0));
+      maxr = std::max(maxr, rn[j]);
+    }
+    S.max_rnorm = maxr;
+    if (phase32 && p.prec == PSOCI_MIXED && maxr < switch_tol) {
+      // switch: collapse to the Ritz vectors and recompute their sigma in FP64
+      collapse(k);
+      phase32 = false;
+      S.switch_iter = it;
+      do_sigma(0, k);
+      update_A(0, k);
+      continue;
+    }
+    if (!phase32 && maxr < p.tol) {
+      converged = true;
+      break;
+    }
+    if (phase32 && p.prec == PSOCI_FP32 && maxr < p.tol) {
+      converged = true;
+      break;
+    }
+    const double sel_tol = (phase32 && p.prec == PSOCI_MIXED) ? switch_tol : p.tol;
+    std::vector<int> sel;
+    for (int j = 0; j < k; ++j)
+      if (rn[j] >= sel_tol) sel.push_back(j);
+    int nsel = (int)sel.size();
+    if (m + nsel > mmax) {
+      collapse(kc);
+      // the retained Ritz values and vectors are unchanged: w[0..k) still valid
+    }
+    L.precondition(n, k, c.d_diag.p, w.data(), 1.0e-3, R.p, k);
+    // new block = selected preconditioned residuals, placed at V[:, m:m+nsel)
+    std::vector<double> Ssel((size_t)k * nsel, 0.0);
+    for (int j = 0; j < nsel; ++j) Ssel[(size_t)sel[j] * nsel + j] = 1.0;
+    L.gemm_nn(n, k, nsel, R.p, k, Ssel.data(), 0.0, V.p + m, mmax);
+    int nb = nsel;
+    for (int pass = 0; pass < 2 && nb > 0; ++pass) {
+      project_out(m, nb);
+      nb = sym_orth(m, nb);
+    }
+    if (nb == 0) break;  // no new directions: stagnation
+    do_sigma(m, nb);
+    update_A(m, nb);
+    m += nb;
+  }
+  S.iterations = std::min(it, maxit);
+  S.converged = converged ? 1 : 0;
+  solve(m);
+  for (int j = 0; j < k; ++j) evals[j] = w[j];
+  if (evecs) {
+    cols_of(m, 0, k, Yk);
+    L.gemm_nn(n, m, k, V.p, mmax, Yk.data(), 0.0, R.p, k);
+    R.download(evecs, (size_t)n * k, c.stream);
+    c.sync();
+  }
+  S.seconds = total.sec();
+  if (st) *st = S;
+  return converged ? PSOCI_OK : PSOCI_ERR_CONVERGE;
+}
+
+}  // namespace psoci

@@ -1,0 +1,275 @@
+// Configuration-pair matrix elements on CDNA4, in bit-mask form.
+//
+// A configuration is two 64-bit words: d (doubly occupied orbitals) and s
+// (singly occupied).  A determinant of a configuration with n open shells is
+// an n-bit pattern p over the open shells (bit t set = the t-th open shell,
+// in ascending orbital order, carries alpha spin); its alpha and beta
+// orbital sets are
+//     A = d | deposit(p, s),   B = d | (s & ~deposit(p, s)).
+// The reference keeps the same determinants as sorted orbital lists
+// (JOUTFG, PsociDeterminants.hpp:60-87); everything it computes from those
+// lists -- excitation counts, the orbitals that differ, and the sign windows
+// "count orbitals strictly between a and b" -- is a popcount or a
+// count-trailing-zeros of these words.
+//
+// The arithmetic of every element follows the reference's operation order
+// exactly (PsociHamiltonian.C:139-388 for the configuration-level sums,
+// :860-1118 for determinant pairs, :1350-1454 for the SEF transform); the
+// library is compiled with -ffp-contract=off so no multiply-add is fused.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#define PSOCI_HD __host__ __device__ inline
+
+namespace psoci {
+
+// 1/sqrt(2) rounded to double: sqrt(0.5) as the reference computes it.
+constexpr double kRthlf = 0x1.6a09e667f3bcdp-1;
+
+struct PairInfo {
+  int32_t I, J;
+  int8_t kind;  // 0 skipped by the reference, 1 diagonal, 2 single, 3 double
+  int8_t e0, e1, f0, f1;  // iex[0..1], jex[0..1] (orbitals leaving I / entering J)
+  int8_t pad[3];
+  double v0, v1;  // vnt[0], vnt[1]
+};
+
+struct ConfView {
+  uint64_t d, s;
+  int nopen;
+  int ph;              // configuration phase bit (see space.cpp)
+  const uint16_t* pat; // determinant patterns of this open-shell count
+};
+
+PSOCI_HD int pair_idx(int p, int q) {
+  int a = p > q ? p : q, b = p > q ? q : p;
+  return a * (a + 1) / 2 + b;
+}
+
+// (pq|rs) at the canonical 8-fold slot (PsociIntegrals::canonicalIndex - 1).
+PSOCI_HD double g2(const double* __restrict__ two, int p, int q, int r, int s) {
+  int ij = pair_idx(p, q), kl = pair_idx(r, s);
+  int a = ij > kl ? ij : kl, b = ij > kl ? kl : ij;
+  return two[(int64_t)a * (a + 1) / 2 + b];
+}
+PSOCI_HD int g2_slot(int p, int q, int r, int s) {
+  int ij = pair_idx(p, q), kl = pair_idx(r, s);
+  int a = ij > kl ? ij : kl, b = ij > kl ? kl : ij;
+  return a * (a + 1) / 2 + b;
+}
+
+PSOCI_HD double h1(const double* __restrict__ one, int p, int q) { return one[pair_idx(p, q)]; }
+
+// deposit the low bits of p into the set bits of mask, lowest first
+PSOCI_HD uint64_t deposit(uint32_t p, uint64_t mask) {
+  uint64_t r = 0;
+  while (mask) {
+    uint64_t low = mask & (~mask + 1);
+    if (p & 1u) r |= low;
+    p >>= 1;
+    mask ^= low;
+  }
+  return r;
+}
+
+// orbitals strictly between a and b
+PSOCI_HD uint64_t between(int a, int b) {
+  int lo = a < b ? a : b, hi = a < b ? b : a;
+  if (hi - lo <= 1) return 0ull;
+  return ((1ull << hi) - 1ull) & ~((2ull << lo) - 1ull);
+}
+
+PSOCI_HD int popc(uint64_t x) { return __builtin_popcountll(x); }
+PSOCI_HD int ctz(uint64_t x) { return __builtin_ctzll(x); }
+
+// Electrons that must move to turn configuration I into J (the reference's
+// running iex/jex count, compareOrbMap, PsociGADeterminants.C:2249-2275).
+PSOCI_HD int moved(uint64_t dI, uint64_t sI, uint64_t dJ, uint64_t sJ) {
+  uint64_t eJ = ~(sJ | dJ);
+  return 2 * popc(dI & eJ) + popc(dI & sJ) + popc(sI & eJ);
+}
+
+// Configuration-pair setup: generateSOCIblocks' early exits and iex/jex lists
+// (PsociHamiltonian.C:436-512) and the configuration-level integral sums
+// processDiagExcitation / processSingleExcitation / processDoubleExcitation
+// (:306-388, :190-261, :139-186), each in the reference's loop order.
+PSOCI_HD void pair_setup(PairInfo& P, int I, int J, uint64_t dI, uint64_t sI, int nI, uint64_t dJ, uint64_t sJ,
+                                  int nJ, int nbf, const double* __restrict__ one, const double* __restrict__ two) {
+  P.I = I;
+  P.J = J;
+  P.kind = 0;
+  P.e0 = P.e1 = P.f0 = P.f1 = -1;
+  P.v0 = P.v1 = 0.0;
+  int dn = nI - nJ;
+  if (dn > 4 || dn < -4) return;
+  // iex: orbitals where occ_I > occ_J (repeated for a difference of two), ascending
+  int ie[2] = {-1, -1}, je[2] = {-1, -1}, ni = 0, nj = 0;
+  uint64_t occI = dI | sI, occJ = dJ | sJ;
+  uint64_t diff = (dI ^ dJ) | (sI ^ sJ);
+  while (diff) {
+    int o = ctz(diff);
+    diff &= diff - 1;
+    int oi = (int)((dI >> o) & 1) * 2 + (int)((sI >> o) & 1);
+    int oj = (int)((dJ >> o) & 1) * 2 + (int)((sJ >> o) & 1);
+    int m = oi - oj;
+    for (; m > 0; --m) {
+      if (ni >= 2) return;
+      ie[ni++] = o;
+    }
+    for (; m < 0; ++m) {
+      if (nj >= 2) return;
+      je[nj++] = o;
+    }
+  }
+  (void)occI;
+  (void)occJ;
+  P.e0 = (int8_t)ie[0];
+  P.e1 = (int8_t)ie[1];
+  P.f0 = (int8_t)je[0];
+  P.f1 = (int8_t)je[1];
+  if (I == J) {
+    // processDiagExcitation: vnt[0]; the open-shell exchanges are taken per
+    // determinant from the integral table in detval().
+    double v = 0.0;
+    for (int ib = 0; ib < nbf; ++ib) {
+      int oi = (int)((dI >> ib) & 1) * 2 + (int)((sI >> ib) & 1);
+      if (!oi) continue;
+      if (oi == 1) {
+        v += h1(one, ib, ib);
+      } else {
+        v += 2.0 * h1(one, ib, ib) + g2(two, ib, ib, ib, ib);
+      }
+      for (int jb = 0; jb < ib; ++jb) {
+        int oj = (int)((dI >> jb) & 1) * 2 + (int)((sI >> jb) & 1);
+        if (!oj) continue;
+        int ninj = oi * oj;
+        if (ninj == 1) {
+          v += g2(two, ib, ib, jb, jb);
+        } else {
+          double buf = 2.0 * g2(two, ib, ib, jb, jb);
+          buf -= g2(two, ib, jb, ib, jb);
+          if (ninj == 2) v += buf;
+          else v += 2.0 * buf;
+        }
+      }
+    }
+    P.v0 = v;
+    P.kind = 1;
+    return;
+  }
+  if (ni == 2) {
+    double a = g2(two, ie[0], je[0], ie[1], je[1]);
+    double b = g2(two, ie[0], je[1], ie[1], je[0]);
+    P.v0 = a;
+    P.v1 = b;
+    P.kind = (a == 0.0 && b == 0.0) ? 0 : 3;
+    return;
+  }
+  if (ni == 1) {
+    int ex = ie[0], jx = je[0];
+    double v = h1(one, ex, jx) - g2(two, ex, jx, ex, ex);
+    for (int i = 0; i < nbf; ++i) {
+      int oi = (int)((dI >> i) & 1) * 2 + (int)((sI >> i) & 1);
+      if (!oi) continue;
+      double buf = g2(two, ex, jx, i, i);
+      v += buf;
+      if (oi == 2) v += buf;
+      if (oi == 2 && ((dJ >> i) & 1)) v -= g2(two, ex, i, i, jx);
+    }
+    P.v0 = v;
+    P.kind = (v == 0.0) ? 0 : 2;
+    return;
+  }
+  // identical occupations under two indices: the reference computes no block
+}
+
+// One determinant-pair element (generateSEFCIG, PsociHamiltonian.C:860-1118).
+// pa / pb: patterns of determinant ii of the row configuration and jj of the
+// column configuration.
+PSOCI_HD double detval(const PairInfo& P, const ConfView& A, const ConfView& B, uint32_t pa, uint32_t pb,
+                                         const double* __restrict__ two) {
+  uint64_t ao = deposit(pa, A.s), bo = deposit(pb, B.s);
+  uint64_t aI = A.d | ao, bI = A.d | (A.s & ~ao);
+  uint64_t aJ = B.d | bo, bJ = B.d | (B.s & ~bo);
+  if (popc(aI) != popc(aJ)) return 0.0;  // spin projections differ
+  uint64_t xa = aJ & ~aI, xb = bJ & ~bI;
+  int ma = popc(xa), mb = popc(xb);
+  int tot = ma + mb;
+  if (tot > 2) return 0.0;
+  if (tot == 0) {
+    // same determinant: vnt[0] minus same-spin open-shell exchanges, pairs
+    // (singly[i], singly[j]) with j < i in ascending order
+    double v = P.v0;
+    uint64_t si = A.s;
+    uint64_t seen = 0;
+    while (si) {
+      int io = ctz(si);
+      si &= si - 1;
+      uint64_t sj = seen;
+      int ia = (int)((aI >> io) & 1);
+      while (sj) {
+        int jo = ctz(sj);
+        sj &= sj - 1;
+        if (ia == (int)((aI >> jo) & 1)) v -= g2(two, io, jo, jo, io);
+      }
+      seen |= 1ull << io;
+    }
+    return v;
+  }
+  int cnt = A.ph + B.ph;
+  double v;
+  if (tot == 1) {
+    uint64_t S = ma ? aI : bI;
+    cnt += popc(S & between(P.e0, P.f0));
+    v = P.v0;
+    uint64_t T = S & ~A.d & ~(1ull << P.e0) & ~(1ull << P.f0);
+    while (T) {
+      int ia = ctz(T);
+      T &= T - 1;
+      v -= g2(two, P.e0, ia, ia, P.f0);
+    }
+  } else if (ma == 1) {
+    int iex2 = ctz(xa), jex2 = ctz(xb);
+    int iex3 = ctz(aI & ~aJ), jex3 = ctz(bI & ~bJ);
+    cnt += popc(aI & between(iex2, iex3)) + popc(bI & between(jex2, jex3));
+    v = g2(two, jex2, jex3, iex2, iex3);
+  } else {
+    uint64_t SI = ma ? aI : bI, SJ = ma ? aJ : bJ;
+    cnt += popc(SJ & between(P.f0, P.f1)) + popc(SI & between(P.e0, P.e1));
+    v = P.v0 - P.v1;
+  }
+  return (cnt & 1) ? -v : v;
+}
+
+// SEF element (i, j) of the block: transformCDCt (PsociHamiltonian.C:1350-1454)
+// applied to the two 2x2 determinant sub-blocks it reads; SEF k pairs
+// determinants 2k and 2k+1 (determinantToDoublegroupMaps, :1304-1322);
+// mult = -1 when the configuration's symmetry parity differs from the
+// state's (ciout, :1487-1495).
+PSOCI_HD double sef_elem(const PairInfo& P, const ConfView& A, const ConfView& B, int i, int j,
+                                           double multi, double multj, const double* __restrict__ two) {
+  double ds0, ds1 = 0.0;
+  if (B.nopen == 0) {
+    ds0 = detval(P, A, B, A.pat[A.nopen ? 2 * i : 0], B.pat[0], two);
+    if (A.nopen) ds1 = detval(P, A, B, A.pat[2 * i + 1], B.pat[0], two);
+  } else {
+    uint32_t b0 = B.pat[2 * j], b1 = B.pat[2 * j + 1];
+    uint32_t a0 = A.pat[A.nopen ? 2 * i : 0];
+    double t = detval(P, A, B, a0, b0, two);
+    t += multj * detval(P, A, B, a0, b1, two);
+    ds0 = t * kRthlf;
+    if (A.nopen) {
+      uint32_t a1 = A.pat[2 * i + 1];
+      double u = detval(P, A, B, a1, b0, two);
+      u += multj * detval(P, A, B, a1, b1, two);
+      ds1 = u * kRthlf;
+    }
+  }
+  if (A.nopen == 0) return ds0;
+  double t = kRthlf * ds0;
+  t += (multi * kRthlf) * ds1;
+  return t;
+}
+
+}  // namespace psoci

@@ -1,0 +1,34 @@
+// Dense GPU kernels used by the block Davidson (linalg.hip).
+#pragma once
+#include "common.hpp"
+
+namespace psoci {
+
+struct Linalg {
+  hipStream_t stream = nullptr;
+  DevBuf<double> part, small, ysmall, theta;
+
+  // out (device, m x b row-major) = A[:, 0:m]^T B[:, 0:b] over n rows (FP64 MFMA)
+  void gemm_tn(int64_t n, int m, int b, const double* A, int lda, const double* B, int ldb, double* out_dev);
+  // same, result copied to host (synchronises)
+  void gemm_tn_host(int64_t n, int m, int b, const double* A, int lda, const double* B, int ldb, double* out_host);
+  // Out[:, 0:k] = alpha * Out + A[:, 0:m] Y1 (Y1 host, m x k row-major)
+  void gemm_nn(int64_t n, int m, int k, const double* A, int lda, const double* Y1_host, double alpha, double* Out,
+               int ldo);
+  // Out[:, 0:k] = A Y1 - B Y2 (Y1, Y2 host, m x k)
+  void gemm_nn2(int64_t n, int m, int k, const double* A, int lda, const double* Y1_host, const double* B, int ldb,
+                const double* Y2_host, double* Out, int ldo);
+  // R[:, c] /= (D - theta_c) where |D - theta_c| > dmin
+  void precondition(int64_t n, int k, const double* D, const double* theta_host, double dmin, double* R, int ldr);
+  void to_f32(int64_t n, int k, const double* A, int lda, float* B, int ldb);
+  void to_f64(int64_t n, int k, const float* A, int lda, double* B, int ldb);
+  void copy_cols(int64_t n, int k, const double* A, int lda, double* B, int ldb);
+  void unit_cols(int64_t n, int k, const int64_t* idx_host, double* V, int ldv);
+};
+
+// Symmetric eigen-decomposition of a small dense matrix (host, FP64):
+// Householder tridiagonalisation + implicit QL.  a is m x m row-major and is
+// overwritten by the eigenvectors (column c = eigenvector c); w ascending.
+bool sym_eig(int m, double* a, double* w);
+
+}  // namespace psoci

@@ -1,0 +1,205 @@
+"""Pins the oracle (the CPU restatement of the reference arithmetic) against
+what fixes it independently: the reference's own known-answer tables,
+closed forms, second-quantised brute force, dense LAPACK and the FCI spectrum.
+A plausible slip in the restatement (a dropped term, a wrong window in a phase
+count, a transposed block) fails at least one of these."""
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import bruteforce as bf
+from psoci_amd import inputs
+
+HERE = os.path.dirname(__file__)
+
+
+def test_packed_indices_bruteforce():
+    L = oracle.lib()
+    for n in range(1, 6):
+        seen = {}
+        for p, q, r, s in itertools.product(range(n), repeat=4):
+            idx = L.orc_canonical_index(p, q, r, s)
+            # all eight permutations of a real (pq|rs) share one slot
+            for perm in [(q, p, r, s), (p, q, s, r), (r, s, p, q), (s, r, q, p)]:
+                assert L.orc_canonical_index(*perm) == idx
+            key = (frozenset([frozenset([p, q]), frozenset([r, s])]))
+            seen.setdefault(idx, set()).add(key)
+            assert inputs.canonical_index(p, q, r, s) == idx - 1
+        # a bijection onto 1..NNDAF(NNDAF(n))
+        m = n * (n + 1) // 2
+        assert sorted(seen) == list(range(1, m * (m + 1) // 2 + 1))
+        assert all(len(v) == 1 for v in seen.values())
+        for p, q in itertools.product(range(n), repeat=2):
+            assert L.orc_two_e_index(p, q) == max(p, q) * (max(p, q) + 1) // 2 + min(p, q) + 1
+
+
+def _single_config_space(nopen, ndoub=2, nbf=None):
+    nbf = nbf or (nopen + 2 * ndoub + 1)
+    occ = np.zeros((1, nbf), np.uint8)
+    # interleave doubly and singly orbitals so phases are non-trivial
+    order = list(range(nbf))
+    dbl = order[1:1 + 2 * ndoub:2][:ndoub]
+    rest = [o for o in order if o not in dbl][:nopen]
+    occ[0, dbl] = 2
+    occ[0, rest] = 1
+    cfg = inputs.Configs(nbf=nbf, nelec=2 * ndoub + nopen, ksym=1, occ=occ, jsym=np.ones(1, np.int32))
+    return cfg, oracle.OracleSpace(cfg), sorted(rest), sorted(dbl)
+
+
+@pytest.mark.parametrize("nopen", [2, 4, 6])
+def test_det_tables_match_reference_comments(nopen):
+    tab = json.load(open(os.path.join(HERE, "golden", "det_tables.json")))
+    rows = tab[str(nopen)]
+    if nopen == 6:
+        rows = [list(r) for r in rows]
+        rows[27] = [[3], [1, 2, 4, 5, 6]]  # documented erratum of the comment table
+    cfg, sp, singly, dbl = _single_config_space(nopen)
+    d = sp.dets(0, raw=True)
+    assert len(d["alpha"]) == len(rows)
+    for (ta, tb), a, b in zip(rows, d["alpha"], d["beta"]):
+        # generator order: doubly occupied first, then the open shells ascending
+        assert a == dbl + [singly[t - 1] for t in ta]
+        assert b == dbl + [singly[t - 1] for t in tb]
+
+
+@pytest.mark.parametrize("nopen", [0, 2, 4, 6, 8, 10])
+def test_det_invariants(nopen):
+    cfg, sp, singly, dbl = _single_config_space(nopen, ndoub=2)
+    d = sp.dets(0)
+    nd = 1 << max(nopen - 1, 0)
+    assert len(d["alpha"]) == nd
+    seen = set()
+    for k, (a, b) in enumerate(zip(d["alpha"], d["beta"])):
+        assert a == sorted(a) and b == sorted(b)
+        # alpha U beta reproduces the occupations; doubly in both
+        assert sorted(set(a) & set(b)) == dbl
+        assert sorted((set(a) | set(b)) - set(dbl)) == singly
+        assert d["ms"][k] == (len(a) - len(b)) // 2
+        assert d["suma"][k] == sum(o + 1 for o in a)
+        seen.add((tuple(a), tuple(b)))
+    assert len(seen) == nd
+    # Kramers pairing assumed by the SEF transform: det 2j+1 is the spin flip of det 2j
+    if nopen:
+        for j in range(nd // 2):
+            assert d["alpha"][2 * j + 1] == d["beta"][2 * j] and d["beta"][2 * j + 1] == d["alpha"][2 * j]
+
+
+@pytest.mark.parametrize("nopen", [2, 4, 6, 8])
+def test_phases_by_explicit_inversion_count(nopen):
+    cfg, sp, singly, dbl = _single_config_space(nopen, ndoub=3, nbf=nopen + 5)
+    raw = sp.dets(0, raw=True)
+    d = sp.dets(0)
+    for k in range(len(raw["alpha"])):
+        inv = 0
+        for lst in (raw["alpha"][k], raw["beta"][k]):
+            # sign of the permutation that sorts the generator-order list
+            perm = sorted(range(len(lst)), key=lambda t: lst[t])
+            seen = [False] * len(perm)
+            for i in range(len(perm)):
+                if not seen[i]:
+                    j, cyc = i, 0
+                    while not seen[j]:
+                        seen[j] = True
+                        j = perm[j]
+                        cyc += 1
+                    inv += cyc - 1
+        assert d["phase"][k] == inv % 2
+
+
+def test_blocks_match_bruteforce_fci6(fci6):
+    cfg, ints, _ = fci6
+    sp = oracle.OracleSpace(cfg, ints)
+    h, g = ints.dense()
+    B = bf.Brute(h, g)
+    sefs = [bf.sef_vectors(sp.dets(c), cfg.nbf, int(sp.nopen[c])) for c in range(cfg.ncfg)]
+    nblocks = 0
+    for I in range(cfg.ncfg):
+        for J in range(I + 1):
+            st, blk = sp.block(I, J)
+            ref = bf.block(B, sefs[I], sefs[J])
+            if st:
+                nblocks += 1
+                np.testing.assert_allclose(blk, ref, rtol=0, atol=1e-12)
+            else:
+                assert np.abs(ref).max() < 1e-12
+    assert nblocks == 4656
+
+
+def test_blocks_match_bruteforce_fci10_sample(fci10):
+    """Sampled blocks of the 31,650-SEF space, including 8-open-shell ones."""
+    cfg, ints, _ = fci10
+    sp = oracle.OracleSpace(cfg, ints)
+    h, g = ints.dense()
+    B = bf.Brute(h, g)
+    rng = np.random.default_rng(7)
+    nopen = sp.nopen
+    picks = list(np.where(nopen == 8)[0][:2]) + list(rng.choice(cfg.ncfg, 6, replace=False))
+    checked = 0
+    for I in picks:
+        js = sp.jlist(int(I))
+        for J in rng.choice(js, min(6, len(js)), replace=False):
+            st, blk = sp.block(int(I), int(J))
+            if not st:
+                continue
+            si = bf.sef_vectors(sp.dets(int(I)), cfg.nbf, int(nopen[I]))
+            sj = bf.sef_vectors(sp.dets(int(J)), cfg.nbf, int(nopen[J]))
+            np.testing.assert_allclose(blk, bf.block(B, si, sj), rtol=0, atol=1e-12)
+            checked += 1
+    assert checked >= 20
+
+
+def test_spectrum_in_fci_spectrum(fci6, oracle_fci6):
+    cfg, ints, _ = fci6
+    h, g = ints.dense()
+    Hf, _ = bf.fci_matrix(bf.Brute(h, g), cfg.nelec)
+    wf = np.linalg.eigvalsh(Hf)
+    w = np.linalg.eigvalsh(oracle_fci6.dense_h())
+    assert len(w) == 246
+    assert max(np.abs(wf - x).min() for x in w) < 1e-10
+
+
+def test_stored_matrix_properties(oracle_fci6):
+    sp = oracle_fci6
+    rp, col, val = sp.rowptr, sp.col, sp.val
+    for r in range(sp.n):
+        c = col[rp[r]:rp[r + 1]]
+        assert c[-1] == r  # diagonal stored last
+        assert np.all(np.diff(c) > 0) and np.all(c <= r)
+        v = val[rp[r]:rp[r + 1] - 1]
+        assert np.all(np.abs(v) >= 1e-8)
+    np.testing.assert_array_equal(sp.diag, val[rp[1:] - 1])
+
+
+def test_sigma_matches_dense(oracle_fci6, rng):
+    sp = oracle_fci6
+    X = rng.standard_normal((sp.n, 3))
+    np.testing.assert_allclose(sp.sigma(X), sp.dense_h() @ X, rtol=0, atol=1e-12)
+
+
+def test_davidson_converges_to_lapack(oracle_fci6):
+    sp = oracle_fci6
+    out = sp.davidson(4, rtol=1e-6)
+    assert out["iterations"] > 0
+    w = np.linalg.eigvalsh(sp.dense_h())[:4]
+    np.testing.assert_allclose(out["evals"], w, atol=1e-9)
+    assert np.all(out["rnorms"] < 1e-6)
+
+
+def test_rdm1_matches_bruteforce(fci6, oracle_fci6):
+    cfg, ints, _ = fci6
+    sp = oracle_fci6
+    w, V = np.linalg.eigh(sp.dense_h())
+    c = V[:, 0]
+    D = sp.rdm1(c)
+    sefs = []
+    for I in range(cfg.ncfg):
+        sefs += bf.sef_vectors(sp.dets(I), cfg.nbf, int(sp.nopen[I]))
+    gam = bf.one_rdm(c, sefs, cfg.nbf)
+    for p in range(cfg.nbf):
+        for q in range(p + 1):
+            assert abs(D[inputs.pair_index(p, q)] - gam[p, q]) < 1e-12
+    assert abs(sum(D[inputs.pair_index(p, p)] for p in range(cfg.nbf)) - cfg.nelec) < 1e-10
