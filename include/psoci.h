@@ -138,6 +138,15 @@ typedef struct {
 int psoci_davidson(psoci_ctx* ctx, const psoci_dav_params* params, double* evals, double* evecs,
                    psoci_dav_stats* stats);
 
+/* Subspace projection out = A^T B over n rows on FP64 MFMA: the VtV / VtHV
+ * dot products of PsociGAsubspace::generateIncrementalSubspace
+ * (PsociGAsubspace.C:273-362) and the MGS overlaps of
+ * PsociGAbasis::orthonormalizeAppendedVectors (PsociGAbasis.C:1371-1470) as
+ * one product.  A [n][lda] and B [n][ldb] are device pointers (row-major,
+ * first m resp. b columns used); out is host memory, m x b row-major. */
+int psoci_project(psoci_ctx* ctx, int64_t n, int m, int b, const double* A, int lda, const double* B, int ldb,
+                  double* out);
+
 /* Adjoint of E_k: dE/dtheta = c^T (dH/dtheta) c for every packed integral,
  * accumulated over k vectors C[n][k] (row-major, host) with weights w[k]
  * (NULL: all 1).  grad_one[NNDAF(nbf)], grad_two[NNDAF(NNDAF(nbf))];

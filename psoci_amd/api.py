@@ -63,6 +63,13 @@ def load(path: str | None = None):
     p = path or LIB_PATH
     if not os.path.exists(p):
         raise FileNotFoundError(f"{p} is not built: run python -m psoci_amd.build")
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7.  When
+    # torch is importable it is loaded first, so the library binds to the
+    # same runtime (same soname) instead of bringing in a second one.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(p)
     V, I, I64, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
     sig = {
@@ -81,6 +88,7 @@ def load(path: str | None = None):
         "psoci_davidson": (I, [V, ctypes.POINTER(DavParams), V, V, ctypes.POINTER(DavStats)]),
         "psoci_rdm": (I, [V, I, V, V, V, V, V]),
         "psoci_device_ptr": (I, [V, I, ctypes.POINTER(V)]),
+        "psoci_project": (I, [V, I64, I, I, V, I, V, I, V]),
         "psoci_sync": (I, [V]),
     }
     for name, (res, args) in sig.items():
@@ -213,6 +221,15 @@ class Hamiltonian:
         if rc not in (OK, ERR_CONVERGE):
             self._check(rc)
         return dict(evals=ev, evecs=vec, stats=st.as_dict(), converged=rc == OK)
+
+    def project(self, A, B):
+        """A^T B over the rows of two GPU tensors (FP64 MFMA)."""
+        n, m = A.shape
+        b = B.shape[1]
+        out = np.zeros((m, b))
+        self._check(self.L.psoci_project(self._h, n, m, b, ctypes.c_void_p(A.data_ptr()), A.stride(0),
+                                         ctypes.c_void_p(B.data_ptr()), B.stride(0), _p(out)))
+        return out
 
     def stream_handle(self) -> int:
         """The hipStream_t the library launches on (as an integer)."""

@@ -4,6 +4,7 @@
 #include <cstring>
 
 #include "ctx.hpp"
+#include "linalg.hpp"
 
 using psoci::Ctx;
 using psoci::Error;
@@ -218,6 +219,18 @@ int psoci_rdm(psoci_ctx* ctx, int k, const double* C, const double* w, double* g
     if (grad_two) g2.download(grad_two, n2, c.stream);
     c.sync();
     if (grad_so) std::memset(grad_so, 0, n1 * sizeof(double));
+    return PSOCI_OK;
+  });
+}
+
+int psoci_project(psoci_ctx* ctx, int64_t n, int m, int b, const double* A, int lda, const double* B, int ldb,
+                  double* out) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(n >= 1 && m >= 1 && b >= 1 && m <= lda && b <= ldb && A && B && out, PSOCI_ERR_ARG,
+                "bad projection arguments");
+    psoci::Linalg L;
+    L.stream = c.stream;
+    L.gemm_tn_host(n, m, b, A, lda, B, ldb, out);
     return PSOCI_OK;
   });
 }

@@ -22,6 +22,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <numeric>
 
 #include "ctx.hpp"
@@ -230,6 +232,8 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
   };
 
   std::vector<double> rn(k, 0.0);
+  const char* tr = std::getenv("PSOCI_TRACE");
+  const bool trace = tr && tr[0] == '1';
   bool converged = false;
   int it = 0;
   for (it = 1; it <= maxit; ++it) {
@@ -247,6 +251,9 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
       maxr = std::max(maxr, rn[j]);
     }
     S.max_rnorm = maxr;
+    if (trace)
+      std::fprintf(stderr, "[psoci davidson] it %d m %d %s max|r| %.3e  E0 %.12f E%d %.12f\n", it, m,
+                   phase32 ? "fp32" : "fp64", maxr, w[0], k - 1, w[k - 1]);
     if (phase32 && p.prec == PSOCI_MIXED && maxr < switch_tol) {
       // switch: collapse to the Ritz vectors and recompute their sigma in FP64
       collapse(k);

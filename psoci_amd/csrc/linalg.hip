@@ -18,9 +18,9 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 
 // C_partial[chunk][m][b] = sum over rows of chunk of A[row][a0 + i] * B[row][b0 + j]
 // for one 16x16 output tile per wavefront, on v_mfma_f64_16x16x4f64.
-// Lane layout of the instruction: A operand lane l = (row k = l/16, col i = l%16),
-// B operand lane l = (row k = l/16, col j = l%16), accumulator register t of
-// lane l = (i = 4*(l/16) + t, j = l%16).
+// Lane layout of the instruction (measured on gfx950): A operand lane l =
+// (row k = l/16, col i = l%16), B operand lane l = (row k = l/16,
+// col j = l%16), accumulator register t of lane l = (i = 4*t + l/16, j = l%16).
 __global__ __launch_bounds__(256) void k_gemm_tn_mfma(int64_t n, int m, int b, const double* __restrict__ A, int lda,
                                                       const double* __restrict__ B, int ldb, int64_t rows_per_chunk,
                                                       double* __restrict__ partial) {
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void k_gemm_tn_mfma(int64_t n, int m, int b, c
   double* out = partial + chunk * (int64_t)m * b;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    const int i = tm * 16 + 4 * (lane >> 4) + t;
+    const int i = tm * 16 + 4 * t + (lane >> 4);
     const int j = tb * 16 + (lane & 15);
     if (i < m && j < b) out[(int64_t)i * b + j] = acc[t];
   }
