@@ -73,7 +73,7 @@ def lib():
         L.orc_sigma.argtypes = [P, ctypes.c_int, P, P]
         L.orc_sigma_csr.argtypes = [ctypes.c_long, P, P, P, ctypes.c_int, P, P]
         L.orc_davidson.restype = ctypes.c_int
-        L.orc_davidson.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_int, P, P, P, P]
+        L.orc_davidson.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, P, P, P, P]
         L.orc_rdm1.argtypes = [P, P, P]
         L.orc_two_e_index.restype = ctypes.c_int
         L.orc_two_e_index.argtypes = [ctypes.c_int] * 2
@@ -171,12 +171,14 @@ class OracleSpace:
         lib().orc_sigma(self._h, k, _p(X), _p(Y))
         return Y
 
-    def davidson(self, k: int, rtol: float = 1e-6, maxit: int = 400):
+    def davidson(self, k: int, rtol: float = 1e-6, maxit: int = 400, guess: int = 0):
+        """guess 0: the reference's unit guesses on the first k SEFs;
+        guess 1: unit guesses on the k lowest diagonal elements."""
         ev = np.zeros(k)
         vecs = np.zeros((self.n, k))
         rn = np.zeros(k)
         ns = ctypes.c_int(0)
-        it = lib().orc_davidson(self._h, k, rtol, maxit, _p(ev), _p(vecs), _p(rn), ctypes.byref(ns))
+        it = lib().orc_davidson(self._h, k, rtol, maxit, guess, _p(ev), _p(vecs), _p(rn), ctypes.byref(ns))
         return dict(evals=ev, evecs=vecs, rnorms=rn, iterations=it, nsigma=ns.value)
 
     def rdm1(self, c: np.ndarray):

@@ -1045,7 +1045,7 @@ void orc_sigma(void* h, int k, const double* X, double* Y) {
 // coarse 1e-3 then final rtol, collapse to 2k when above 4k vectors or at the
 // coarse->final switch (expandAndReplaceVectorsNew, :483-579).
 // Returns total iterations; evals[k], evecs [n][k] row-major (may be null).
-int orc_davidson(void* h, int k, double rtol, int maxit, double* evals, double* evecs, double* rnorms_out,
+int orc_davidson(void* h, int k, double rtol, int maxit, int guess, double* evals, double* evecs, double* rnorms_out,
                  int* nsigma_out) {
   Space* sp = static_cast<Space*>(h);
   long n = sp->nsef_total;
@@ -1077,9 +1077,20 @@ int orc_davidson(void* h, int k, double rtol, int maxit, double* evals, double* 
     return true;
   };
   int nsigma = 0;
+  // guess 0: unit vectors on the first k SEFs (the reference's guess);
+  // guess 1: on the k lowest diagonal elements (same roots as the GPU runs)
+  std::vector<long> start(k);
+  for (int r = 0; r < k; ++r) start[r] = r;
+  if (guess == 1) {
+    std::vector<long> ord(n);
+    for (long i = 0; i < n; ++i) ord[i] = i;
+    std::partial_sort(ord.begin(), ord.begin() + k, ord.end(),
+                      [&](long a, long b) { return D[a] < D[b] || (D[a] == D[b] && a < b); });
+    for (int r = 0; r < k; ++r) start[r] = ord[r];
+  }
   for (int r = 0; r < k; ++r) {
     std::vector<double> v(n, 0.0);
-    v[r] = 1.0;
+    v[start[r]] = 1.0;
     if (!mgs_append(v)) return -1;
   }
   for (int r = 0; r < k; ++r) {

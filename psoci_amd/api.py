@@ -168,6 +168,11 @@ class Hamiltonian:
         return dict(alpha=a, beta=b, ms=ms, phase=ph)
 
     def block(self, I: int, J: int):
+        ncfg = self.cfg.ncfg
+        if not (0 <= I < ncfg and 0 <= J < ncfg):
+            # let the library reject the indices (PSOCI_ERR_ARG)
+            dummy = np.zeros(1)
+            self._check(self.L.psoci_block(self._h, I, J, _p(dummy), None))
         out = np.zeros((int(self.info.nsef[I]), int(self.info.nsef[J])), np.float64)
         st = ctypes.c_int()
         self._check(self.L.psoci_block(self._h, I, J, _p(out), ctypes.byref(st)))

@@ -6,6 +6,8 @@
 // :874-989) and PsociGAbasis (MGS append :1371-1470; collapse :483-579;
 // precondition :1860-1920).  Vectors are HBM-resident [n][ld] row-major
 // blocks; every product over n is one launch.
+#include <algorithm>
+
 #include "linalg.hpp"
 
 namespace psoci {
@@ -37,16 +39,24 @@ __global__ __launch_bounds__(256) void k_gemm_tn_mfma(int64_t n, int m, int b, c
   const int cj = tb * 16 + (lane & 15);
   const int kk = lane >> 4;
   const bool vi = ci < m, vj = cj < b;
-  double4_t acc = {0.0, 0.0, 0.0, 0.0};
-  for (int64_t r = r0; r < r1; r += 4) {
-    const int64_t row = r + kk;
-    double a = 0.0, bb = 0.0;
-    if (row < r1) {
-      if (vi) a = A[row * lda + ci];
-      if (vj) bb = B[row * ldb + cj];
+  // four independent accumulation chains, 16 rows per step: the loads of a
+  // step are all issued before its MFMAs
+  double4_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0, acc2 = acc0, acc3 = acc0;
+  for (int64_t r = r0; r < r1; r += 16) {
+    double a[4], bb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t row = r + 4 * u + kk;
+      const bool in = row < r1;
+      a[u] = (in && vi) ? A[row * lda + ci] : 0.0;
+      bb[u] = (in && vj) ? B[row * ldb + cj] : 0.0;
     }
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
+    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], bb[0], acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], bb[1], acc1, 0, 0, 0);
+    acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2], bb[2], acc2, 0, 0, 0);
+    acc3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[3], bb[3], acc3, 0, 0, 0);
   }
+  const double4_t acc = (acc0 + acc1) + (acc2 + acc3);
   double* out = partial + chunk * (int64_t)m * b;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
@@ -149,15 +159,14 @@ inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per
 
 void Linalg::gemm_tn(int64_t n, int m, int b, const double* A, int lda, const double* B, int ldb, double* out_dev) {
   if (m == 0 || b == 0) return;
-  int64_t rows_per_chunk = 2048;
-  int64_t nchunks = (n + rows_per_chunk - 1) / rows_per_chunk;
-  if (nchunks > 65535) {
-    nchunks = 65535;
-    rows_per_chunk = ((n + nchunks - 1) / nchunks + 3) / 4 * 4;
-    nchunks = (n + rows_per_chunk - 1) / rows_per_chunk;
-  }
-  part.reserve((size_t)nchunks * m * b);
+  // about 4096 wavefronts in flight: chunks of a multiple of 16 rows
   const int tiles = ((m + 15) / 16) * ((b + 15) / 16);
+  int64_t want = std::max<int64_t>(1, 4096 / tiles);
+  want = std::min<int64_t>(want, 65535);
+  int64_t rows_per_chunk = (n + want - 1) / want;
+  rows_per_chunk = std::max<int64_t>(64, (rows_per_chunk + 15) / 16 * 16);
+  int64_t nchunks = (n + rows_per_chunk - 1) / rows_per_chunk;
+  part.reserve((size_t)nchunks * m * b);
   dim3 grid(nblk(tiles, 4), (unsigned)nchunks);
   hipLaunchKernelGGL(k_gemm_tn_mfma, grid, dim3(256), 0, stream, n, m, b, A, lda, B, ldb, rows_per_chunk, part.p);
   PSOCI_HIP(hipGetLastError());
