@@ -37,11 +37,15 @@ struct Ctx {
   DevBuf<double> d_diag;
   bool have_diag = false;
 
-  // direct sigma: connected configuration pairs
+  // direct sigma: every connected configuration (both directions, J = I
+  // included) per configuration I, ascending, and per-determinant alpha
+  // open-shell masks
   bool have_pairs = false;
-  int64_t npairs = 0;
-  DevBuf<PairInfo> d_pairs;   // lower pairs J <= I, grouped by I ascending
-  DevBuf<int64_t> d_pairoff;  // per I: first pair (ncfg+1)
+  int64_t nadj = 0;
+  DevBuf<int64_t> d_adjoff;   // ncfg + 1
+  DevBuf<int32_t> d_adj;
+  DevBuf<int64_t> d_detoff;   // ncfg
+  DevBuf<uint64_t> d_detao;   // ndet_total
 
   // scratch
   void* scan_tmp = nullptr;
@@ -57,7 +61,7 @@ struct Ctx {
 
 // kernels-side entry points (hbuild.hip, sigma.hip, ...)
 void upload_space(Ctx& c);
-void build_pairs(Ctx& c);               // lower pair list + PairInfo (direct sigma / build)
+void build_pairs(Ctx& c);               // adjacency + determinant masks (direct sigma, adjoint)
 void build_h(Ctx& c);                   // stored Hamiltonian
 void compute_diag(Ctx& c);              // diagonal only
 void block_on_device(Ctx& c, int64_t I, int64_t J, double* host_out, int* status);

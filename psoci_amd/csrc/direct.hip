@@ -1,15 +1,311 @@
-// Direct sigma and the adjoint (RDM) kernels: elements regenerated on the fly.
+// Direct sigma: Y = H X with the Hamiltonian regenerated on the fly.
+//
+// Replaces the direct path of the reference
+// (matrixVectorProductsScatterAccumulateLowLevelAccumulateDirectFetch,
+// PsociGAhamiltonian.C:7549-7823, with matrixVectorUpdate :13036-13115),
+// which per configuration I screens every J <= I (compareAllOrbMap...),
+// rebuilds the block with generateSOCIblocks and scatter-accumulates both
+// y_I and y_J into a global array.
+//
+// Here one wavefront owns configuration I and walks every connected J, in
+// both directions, so y_I is reduced in registers and written once: no
+// atomics, a deterministic result, and X rows are only gathered.  Each block
+// is evaluated in the reference's own orientation (row = the later
+// configuration), so every element is bit-identical to the stored
+// Hamiltonian's.  Lanes are laid out as (SEF row i, pair slice q) with
+// nsef_I * g = 64; SEF pairs whose |Ms| classes differ are skipped (their
+// four determinant pairs all have different spin projections, so the
+// element is exactly zero).
 #include "ctx.hpp"
 
 namespace psoci {
 
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kTile = 1024;    // configurations per LDS tile in the screen
+constexpr int kIPerWave = 16;  // row configurations per wavefront in the screen
+
+inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+// alpha open-shell mask of every determinant: deposit(pattern, singly mask)
+__global__ void k_detao(int64_t ncfg, const int64_t* __restrict__ detoff, const uint64_t* __restrict__ sm,
+                        const int8_t* __restrict__ nopen, const uint16_t* __restrict__ pat,
+                        const int32_t* __restrict__ patoff, uint64_t* __restrict__ detao) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (c >= ncfg) return;
+  const int s = nopen[c];
+  const int nd = 1 << (s > 0 ? s - 1 : 0);
+  const uint16_t* p = pat + patoff[s];
+  const uint64_t mask = sm[c];
+  for (int d = lane; d < nd; d += kWave) detao[detoff[c] + d] = deposit(p[d], mask);
+}
+
+// Connected configurations of every I (all J, ascending), screened through
+// LDS tiles of the configuration words; each wavefront handles kIPerWave rows.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_adj(int64_t ncfg, const uint64_t* __restrict__ dm,
+                                             const uint64_t* __restrict__ sm, const int8_t* __restrict__ nopen,
+                                             int32_t* __restrict__ cnt, const int64_t* __restrict__ off,
+                                             int32_t* __restrict__ out) {
+  __shared__ uint64_t tdm[kTile], tsm[kTile];
+  __shared__ int8_t tno[kTile];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int64_t I0 = ((int64_t)blockIdx.x * (blockDim.x / kWave) + wave) * kIPerWave;
+  uint64_t dI[kIPerWave], sI[kIPerWave];
+  int nI[kIPerWave];
+  int64_t pos[kIPerWave];
+#pragma unroll
+  for (int u = 0; u < kIPerWave; ++u) {
+    const int64_t I = I0 + u;
+    const bool v = I < ncfg;
+    dI[u] = v ? dm[I] : 0;
+    sI[u] = v ? sm[I] : 0;
+    nI[u] = v ? nopen[I] : 0;
+    pos[u] = (FILL && v) ? off[I] : 0;
+  }
+  const uint64_t below = (1ull << lane) - 1ull;
+  for (int64_t T0 = 0; T0 < ncfg; T0 += kTile) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < kTile; t += blockDim.x) {
+      const int64_t J = T0 + t;
+      if (J < ncfg) {
+        tdm[t] = dm[J];
+        tsm[t] = sm[J];
+        tno[t] = nopen[J];
+      } else {
+        tdm[t] = 0;
+        tsm[t] = 0;
+        tno[t] = 100;  // never within 4 open shells of a real configuration
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kIPerWave; ++u) {
+      if (I0 + u >= ncfg) break;  // uniform per wavefront
+      for (int s = 0; s < kTile; s += kWave) {
+        const int t = s + lane;
+        const int dn = nI[u] - tno[t];
+        const bool ok = dn <= 4 && dn >= -4 && moved(dI[u], sI[u], tdm[t], tsm[t]) <= 2;
+        const uint64_t m = __ballot(ok);
+        if (FILL && ok) out[pos[u] + __popcll(m & below)] = (int32_t)(T0 + t);
+        pos[u] += __popcll(m);
+      }
+    }
+  }
+  if (!FILL && lane == 0) {
+#pragma unroll
+    for (int u = 0; u < kIPerWave; ++u)
+      if (I0 + u < ncfg) cnt[I0 + u] = (int32_t)pos[u];
+  }
+}
+
+__global__ void k_cnt64(int64_t n, const int32_t* __restrict__ a, int64_t* __restrict__ b) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = a[t];
+}
+
+template <typename T>
+__device__ __forceinline__ T fmadd(T a, T b, T c);
+template <>
+__device__ __forceinline__ double fmadd<double>(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+template <>
+__device__ __forceinline__ float fmadd<float>(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+
+struct DirectArgs {
+  int64_t ncfg;
+  const int64_t* adjoff;
+  const int32_t* adj;
+  const uint64_t* dm;
+  const uint64_t* sm;
+  const int8_t* nopen;
+  const uint8_t* cph;
+  const int32_t* nsef;
+  const int64_t* sefoff;
+  const int64_t* detoff;
+  const uint64_t* detao;
+  int nbf;
+  double mult;
+  const double* one;
+  const double* two;
+};
+
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restrict__ X, int ldx, T* __restrict__ Y,
+                                                int ldy) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (I >= a.ncfg) return;  // uniform per wavefront
+  const int nI = a.nsef[I];
+  const int g = kWave / nI;
+  const int i = lane / g, q = lane - i * g;
+  ConfView A;
+  A.d = a.dm[I];
+  A.s = a.sm[I];
+  A.nopen = a.nopen[I];
+  A.ph = a.cph[I];
+  A.pat = nullptr;
+  const int64_t dI0 = a.detoff[I];
+  const uint64_t aI0 = a.detao[dI0 + (A.nopen ? 2 * i : 0)];
+  const uint64_t aI1 = A.nopen ? a.detao[dI0 + 2 * i + 1] : 0ull;
+  const int msI = abs(2 * popc(aI0) - A.nopen);
+  T acc[K];
+#pragma unroll
+  for (int v = 0; v < K; ++v) acc[v] = (T)0;
+  const int64_t t1 = a.adjoff[I + 1];
+  for (int64_t t = a.adjoff[I] + q; t < t1; t += g) {
+    const int J = a.adj[t];
+    ConfView B;
+    B.d = a.dm[J];
+    B.s = a.sm[J];
+    B.nopen = a.nopen[J];
+    B.ph = a.cph[J];
+    B.pat = nullptr;
+    // the reference's orientation: row = the later configuration
+    const bool rowI = J <= I;
+    PairInfo P;
+    if (rowI)
+      pair_setup(P, (int)I, J, A.d, A.s, A.nopen, B.d, B.s, B.nopen, a.nbf, a.one, a.two);
+    else
+      pair_setup(P, J, (int)I, B.d, B.s, B.nopen, A.d, A.s, A.nopen, a.nbf, a.one, a.two);
+    if (P.kind == 0) continue;
+    const int nJ = a.nsef[J];
+    const int64_t dJ0 = a.detoff[J];
+    const T* xJ = X + a.sefoff[J] * (int64_t)ldx;
+    for (int j = 0; j < nJ; ++j) {
+      const uint64_t aJ0 = a.detao[dJ0 + (B.nopen ? 2 * j : 0)];
+      if (abs(2 * popc(aJ0) - B.nopen) != msI) continue;
+      const uint64_t aJ1 = B.nopen ? a.detao[dJ0 + 2 * j + 1] : 0ull;
+      const double h = rowI ? sef_elem_ao(P, A, B, aI0, aI1, aJ0, aJ1, a.mult, a.mult, a.two)
+                            : sef_elem_ao(P, B, A, aJ0, aJ1, aI0, aI1, a.mult, a.mult, a.two);
+      if (h == 0.0) continue;
+      const T hv = (T)h;
+      const T* __restrict__ x = xJ + (int64_t)j * ldx;
+#pragma unroll
+      for (int v = 0; v < K; ++v) acc[v] = fmadd(hv, x[v], acc[v]);
+    }
+  }
+  for (int off = g >> 1; off > 0; off >>= 1) {
+#pragma unroll
+    for (int v = 0; v < K; ++v) acc[v] += __shfl_xor(acc[v], off, kWave);
+  }
+  if (q == 0) {
+    T* y = Y + (a.sefoff[I] + i) * (int64_t)ldy;
+#pragma unroll
+    for (int v = 0; v < K; ++v) y[v] = acc[v];
+  }
+}
+
+DirectArgs direct_args(Ctx& c) {
+  DirectArgs a;
+  a.ncfg = c.sp.ncfg;
+  a.adjoff = c.d_adjoff.p;
+  a.adj = c.d_adj.p;
+  a.dm = c.d_dm.p;
+  a.sm = c.d_sm.p;
+  a.nopen = c.d_nopen.p;
+  a.cph = c.d_cph.p;
+  a.nsef = c.d_nsef.p;
+  a.sefoff = c.d_sefoff.p;
+  a.detoff = c.d_detoff.p;
+  a.detao = c.d_detao.p;
+  a.nbf = c.sp.nbf;
+  a.mult = c.sp.mult;
+  a.one = c.d_one.p;
+  a.two = c.d_two.p;
+  return a;
+}
+
+template <typename T, int K>
+void launch_direct(Ctx& c, const T* X, int ldx, T* Y, int ldy) {
+  DirectArgs a = direct_args(c);
+  hipLaunchKernelGGL((k_direct<T, K>), dim3(nblk(a.ncfg, 4)), dim3(256), 0, c.stream, a, X, ldx, Y, ldy);
+  PSOCI_HIP(hipGetLastError());
+}
+
+template <typename T>
+void direct_any_k(Ctx& c, int k, const T* X, int ldx, T* Y, int ldy) {
+  int done = 0;
+  while (done < k) {
+    const int rem = k - done;
+    const T* x = X + done;
+    T* y = Y + done;
+    if (rem >= 16) {
+      launch_direct<T, 16>(c, x, ldx, y, ldy);
+      done += 16;
+    } else if (rem >= 8) {
+      launch_direct<T, 8>(c, x, ldx, y, ldy);
+      done += 8;
+    } else if (rem >= 4) {
+      launch_direct<T, 4>(c, x, ldx, y, ldy);
+      done += 4;
+    } else if (rem >= 2) {
+      launch_direct<T, 2>(c, x, ldx, y, ldy);
+      done += 2;
+    } else {
+      launch_direct<T, 1>(c, x, ldx, y, ldy);
+      done += 1;
+    }
+  }
+}
+
+}  // namespace
+
+void build_pairs(Ctx& c) {
+  const Space& sp = c.sp;
+  PSOCI_CHECK(sp.max_open <= 8, PSOCI_ERR_UNSUPPORTED,
+              "direct sigma supports at most 8 open shells (64 SEFs per configuration)");
+  const int64_t ncfg = sp.ncfg;
+  c.d_detoff.upload(sp.det_off, c.stream);
+  c.d_detao.alloc(sp.ndet_total);
+  hipLaunchKernelGGL(k_detao, dim3(nblk(ncfg, 4)), dim3(256), 0, c.stream, ncfg, c.d_detoff.p, c.d_sm.p, c.d_nopen.p,
+                     c.d_pat.p, c.d_patoff.p, c.d_detao.p);
+  PSOCI_HIP(hipGetLastError());
+  DevBuf<int32_t> cnt;
+  DevBuf<int64_t> cnt64;
+  cnt.alloc(ncfg);
+  cnt64.alloc(ncfg + 1);
+  const unsigned grid = nblk(ncfg, 4 * kIPerWave);
+  hipLaunchKernelGGL(k_adj<false>, dim3(grid), dim3(256), 0, c.stream, ncfg, c.d_dm.p, c.d_sm.p, c.d_nopen.p, cnt.p,
+                     nullptr, nullptr);
+  PSOCI_HIP(hipGetLastError());
+  PSOCI_HIP(hipMemsetAsync(cnt64.p + ncfg, 0, sizeof(int64_t), c.stream));
+  hipLaunchKernelGGL(k_cnt64, dim3(nblk(ncfg, 256)), dim3(256), 0, c.stream, ncfg, cnt.p, cnt64.p);
+  c.d_adjoff.alloc(ncfg + 1);
+  c.scan(cnt64.p, c.d_adjoff.p, ncfg + 1);
+  int64_t total = 0;
+  PSOCI_HIP(hipMemcpyAsync(&total, c.d_adjoff.p + ncfg, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  c.d_adj.alloc(total);
+  hipLaunchKernelGGL(k_adj<true>, dim3(grid), dim3(256), 0, c.stream, ncfg, c.d_dm.p, c.d_sm.p, c.d_nopen.p, nullptr,
+                     c.d_adjoff.p, c.d_adj.p);
+  PSOCI_HIP(hipGetLastError());
+  c.sync();
+  c.nadj = total;
+  c.have_pairs = true;
+}
+
 void sigma_direct(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy) {
-  (void)c; (void)prec; (void)k; (void)X; (void)ldx; (void)Y; (void)ldy;
-  throw Error(PSOCI_ERR_UNSUPPORTED, "direct sigma is not available in this build");
+  if (!c.have_pairs) build_pairs(c);
+  if (prec == PSOCI_FP32)
+    direct_any_k<float>(c, k, static_cast<const float*>(X), ldx, static_cast<float*>(Y), ldy);
+  else
+    direct_any_k<double>(c, k, static_cast<const double*>(X), ldx, static_cast<double*>(Y), ldy);
 }
 
 void rdm_grad(Ctx& c, int k, const double* dC, const double* w, double* d_g1, double* d_g2) {
-  (void)c; (void)k; (void)dC; (void)w; (void)d_g1; (void)d_g2;
+  (void)c;
+  (void)k;
+  (void)dC;
+  (void)w;
+  (void)d_g1;
+  (void)d_g2;
   throw Error(PSOCI_ERR_UNSUPPORTED, "the adjoint is not available in this build");
 }
 

@@ -185,11 +185,10 @@ PSOCI_HD void pair_setup(PairInfo& P, int I, int J, uint64_t dI, uint64_t sI, in
 }
 
 // One determinant-pair element (generateSEFCIG, PsociHamiltonian.C:860-1118).
-// pa / pb: patterns of determinant ii of the row configuration and jj of the
-// column configuration.
-PSOCI_HD double detval(const PairInfo& P, const ConfView& A, const ConfView& B, uint32_t pa, uint32_t pb,
-                                         const double* __restrict__ two) {
-  uint64_t ao = deposit(pa, A.s), bo = deposit(pb, B.s);
+// ao / bo: the alpha open shells (orbital masks) of determinant ii of the row
+// configuration and jj of the column configuration.
+PSOCI_HD double detval_ao(const PairInfo& P, const ConfView& A, const ConfView& B, uint64_t ao, uint64_t bo,
+                          const double* __restrict__ two) {
   uint64_t aI = A.d | ao, bI = A.d | (A.s & ~ao);
   uint64_t aJ = B.d | bo, bJ = B.d | (B.s & ~bo);
   if (popc(aI) != popc(aJ)) return 0.0;  // spin projections differ
@@ -242,27 +241,31 @@ PSOCI_HD double detval(const PairInfo& P, const ConfView& A, const ConfView& B, 
   return (cnt & 1) ? -v : v;
 }
 
+PSOCI_HD double detval(const PairInfo& P, const ConfView& A, const ConfView& B, uint32_t pa, uint32_t pb,
+                       const double* __restrict__ two) {
+  return detval_ao(P, A, B, deposit(pa, A.s), deposit(pb, B.s), two);
+}
+
 // SEF element (i, j) of the block: transformCDCt (PsociHamiltonian.C:1350-1454)
 // applied to the two 2x2 determinant sub-blocks it reads; SEF k pairs
 // determinants 2k and 2k+1 (determinantToDoublegroupMaps, :1304-1322);
 // mult = -1 when the configuration's symmetry parity differs from the
-// state's (ciout, :1487-1495).
-PSOCI_HD double sef_elem(const PairInfo& P, const ConfView& A, const ConfView& B, int i, int j,
-                                           double multi, double multj, const double* __restrict__ two) {
+// state's (ciout, :1487-1495).  aI0/aI1: alpha open shells of determinants
+// 2i and 2i+1 of the row configuration (aI1 unused without open shells),
+// aJ0/aJ1 likewise for 2j, 2j+1 of the column configuration.
+PSOCI_HD double sef_elem_ao(const PairInfo& P, const ConfView& A, const ConfView& B, uint64_t aI0, uint64_t aI1,
+                            uint64_t aJ0, uint64_t aJ1, double multi, double multj, const double* __restrict__ two) {
   double ds0, ds1 = 0.0;
   if (B.nopen == 0) {
-    ds0 = detval(P, A, B, A.pat[A.nopen ? 2 * i : 0], B.pat[0], two);
-    if (A.nopen) ds1 = detval(P, A, B, A.pat[2 * i + 1], B.pat[0], two);
+    ds0 = detval_ao(P, A, B, aI0, aJ0, two);
+    if (A.nopen) ds1 = detval_ao(P, A, B, aI1, aJ0, two);
   } else {
-    uint32_t b0 = B.pat[2 * j], b1 = B.pat[2 * j + 1];
-    uint32_t a0 = A.pat[A.nopen ? 2 * i : 0];
-    double t = detval(P, A, B, a0, b0, two);
-    t += multj * detval(P, A, B, a0, b1, two);
+    double t = detval_ao(P, A, B, aI0, aJ0, two);
+    t += multj * detval_ao(P, A, B, aI0, aJ1, two);
     ds0 = t * kRthlf;
     if (A.nopen) {
-      uint32_t a1 = A.pat[2 * i + 1];
-      double u = detval(P, A, B, a1, b0, two);
-      u += multj * detval(P, A, B, a1, b1, two);
+      double u = detval_ao(P, A, B, aI1, aJ0, two);
+      u += multj * detval_ao(P, A, B, aI1, aJ1, two);
       ds1 = u * kRthlf;
     }
   }
@@ -270,6 +273,15 @@ PSOCI_HD double sef_elem(const PairInfo& P, const ConfView& A, const ConfView& B
   double t = kRthlf * ds0;
   t += (multi * kRthlf) * ds1;
   return t;
+}
+
+PSOCI_HD double sef_elem(const PairInfo& P, const ConfView& A, const ConfView& B, int i, int j, double multi,
+                         double multj, const double* __restrict__ two) {
+  const uint64_t aI0 = deposit(A.pat[A.nopen ? 2 * i : 0], A.s);
+  const uint64_t aI1 = A.nopen ? deposit(A.pat[2 * i + 1], A.s) : 0;
+  const uint64_t aJ0 = deposit(B.pat[B.nopen ? 2 * j : 0], B.s);
+  const uint64_t aJ1 = B.nopen ? deposit(B.pat[2 * j + 1], B.s) : 0;
+  return sef_elem_ao(P, A, B, aI0, aI1, aJ0, aJ1, multi, multj, two);
 }
 
 }  // namespace psoci

@@ -343,6 +343,7 @@ struct PairLists {
   DevBuf<int32_t> nl, nu;
   DevBuf<int64_t> offL, offU;
   DevBuf<int32_t> colL, rowU, rowcfg;
+  DevBuf<PairInfo> info;
   int64_t nL = 0, nU = 0;
 };
 
@@ -373,23 +374,13 @@ void lower_pairs(Ctx& c, PairLists& L) {
   L.rowcfg.alloc(L.nL);
   hipLaunchKernelGGL(k_expand_rows, dim3(blocks_for(ncfg, 4)), dim3(256), 0, c.stream, ncfg, L.offL.p, L.rowcfg.p);
   PSOCI_HIP(hipGetLastError());
-  c.d_pairs.alloc(L.nL);
+  L.info.alloc(L.nL);
   hipLaunchKernelGGL(k_pair_setup, dim3(blocks_for(L.nL, 256)), dim3(256), 0, c.stream, L.nL, L.rowcfg.p, L.colL.p,
-                     c.d_dm.p, c.d_sm.p, c.d_nopen.p, c.sp.nbf, c.d_one.p, c.d_two.p, c.d_pairs.p);
+                     c.d_dm.p, c.d_sm.p, c.d_nopen.p, c.sp.nbf, c.d_one.p, c.d_two.p, L.info.p);
   PSOCI_HIP(hipGetLastError());
-  c.d_pairoff.alloc(ncfg + 1);
-  PSOCI_HIP(hipMemcpyAsync(c.d_pairoff.p, L.offL.p, (ncfg + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, c.stream));
-  c.npairs = L.nL;
-  c.have_pairs = true;
 }
 
 }  // namespace
-
-void build_pairs(Ctx& c) {
-  PairLists L;
-  lower_pairs(c, L);
-  c.sync();
-}
 
 void build_h(Ctx& c) {
   const Space& sp = c.sp;
@@ -403,7 +394,7 @@ void build_h(Ctx& c) {
   bsz.alloc(np + 1);
   boff.alloc(np + 1);
   PSOCI_HIP(hipMemsetAsync(bsz.p + np, 0, sizeof(int64_t), c.stream));
-  hipLaunchKernelGGL(k_block_sizes, dim3(blocks_for(np, 256)), dim3(256), 0, c.stream, np, c.d_pairs.p, c.d_nsef.p,
+  hipLaunchKernelGGL(k_block_sizes, dim3(blocks_for(np, 256)), dim3(256), 0, c.stream, np, L.info.p, c.d_nsef.p,
                      bsz.p);
   PSOCI_HIP(hipGetLastError());
   c.scan(bsz.p, boff.p, np + 1);
@@ -413,7 +404,7 @@ void build_h(Ctx& c) {
   DevBuf<double> staging;
   staging.alloc(nstage);
   hipLaunchKernelGGL(k_block_elems, dim3(blocks_for(nstage, 256)), dim3(256), 0, c.stream, nstage, np, boff.p,
-                     c.d_pairs.p, c.d_dm.p, c.d_sm.p, c.d_nopen.p, c.d_cph.p, c.d_pat.p, c.d_patoff.p, c.d_nsef.p,
+                     L.info.p, c.d_dm.p, c.d_sm.p, c.d_nopen.p, c.d_cph.p, c.d_pat.p, c.d_patoff.p, c.d_nsef.p,
                      sp.mult, c.d_two.p, staging.p);
   PSOCI_HIP(hipGetLastError());
   // slots: per row of J, nl(J) lower segments, the diagonal block's upper
@@ -435,7 +426,7 @@ void build_h(Ctx& c) {
   pos.alloc(ns + 1);
   PSOCI_HIP(hipMemsetAsync(cnt.p + ns, 0, sizeof(int64_t), c.stream));
   hipLaunchKernelGGL(k_segments<false>, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, ncfg, d_slotbase.p,
-                     L.nl.p, L.nu.p, L.offL.p, L.colL.p, L.offU.p, L.rowU.p, c.d_pairs.p, boff.p, c.d_nsef.p,
+                     L.nl.p, L.nu.p, L.offL.p, L.colL.p, L.offU.p, L.rowU.p, L.info.p, boff.p, c.d_nsef.p,
                      c.d_sefoff.p, staging.p, cnt.p, nullptr, nullptr, nullptr, nullptr, nullptr);
   PSOCI_HIP(hipGetLastError());
   c.scan(cnt.p, pos.p, ns + 1);
@@ -448,7 +439,7 @@ void build_h(Ctx& c) {
   c.d_lowcnt.alloc(n);
   PSOCI_HIP(hipMemcpyAsync(c.d_rowptr.p + n, &nnz, sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
   hipLaunchKernelGGL(k_segments<true>, dim3(blocks_for(ns, 256)), dim3(256), 0, c.stream, ns, ncfg, d_slotbase.p,
-                     L.nl.p, L.nu.p, L.offL.p, L.colL.p, L.offU.p, L.rowU.p, c.d_pairs.p, boff.p, c.d_nsef.p,
+                     L.nl.p, L.nu.p, L.offL.p, L.colL.p, L.offU.p, L.rowU.p, L.info.p, boff.p, c.d_nsef.p,
                      c.d_sefoff.p, staging.p, nullptr, pos.p, c.d_col.p, c.d_val.p, c.d_rowptr.p, c.d_lowcnt.p);
   PSOCI_HIP(hipGetLastError());
   c.d_diag.alloc(n);

@@ -184,16 +184,17 @@ def mrcisd_configs(n_internal: int, n_external: int, nelec: int, max_external: i
     electrons in the external orbitals.  Order: by number of external
     electrons, then external pattern, then internal occupation."""
     nbf = n_internal + n_external
-    rows = []
+    blocks = []
     for ne in range(0, max_external + 1):
-        internal = _occupations(n_internal, nelec - ne)
+        internal = np.array(_occupations(n_internal, nelec - ne), dtype=np.uint8).reshape(-1, n_internal)
+        if len(internal) == 0:
+            continue
         for pat in _ext_patterns(n_external, ne):
             ext = np.zeros(n_external, dtype=np.uint8)
             for a, v in pat:
                 ext[a] = v
-            for t in internal:
-                rows.append(np.concatenate([np.array(t, dtype=np.uint8), ext]))
-    occ = np.array(rows, dtype=np.uint8)
+            blocks.append(np.hstack([internal, np.broadcast_to(ext, (len(internal), n_external))]))
+    occ = np.ascontiguousarray(np.vstack(blocks), dtype=np.uint8)
     return Configs(nbf=nbf, nelec=nelec, ksym=ksym, occ=occ, jsym=np.full(len(occ), jsym, np.int32),
                    title=f"MR-CISD {n_internal}+{n_external} orbitals {nelec} electrons")
 
