@@ -103,6 +103,13 @@ int psoci_get_h(psoci_ctx* ctx, int64_t* rowptr, int32_t* col, double* val, doub
  * are device pointers (no copies, launched on the context stream). */
 int psoci_sigma(psoci_ctx* ctx, int mode, int prec, int k, const void* X, void* Y, int on_device);
 
+/* Work counters of the direct sigma (built on first use): connected
+ * configuration pairs visited per sigma (both orders, J = I included) and
+ * structural SEF elements generated per sigma (sum of nsef_I * nsef_J over
+ * them).  Replaces the reference's allcount / didblock counters
+ * (PsociGAhamiltonian.C:2822-2839). */
+int psoci_direct_info(psoci_ctx* ctx, int64_t* npairs, int64_t* nelems);
+
 /* Diagonal of H in the SEF basis: computeOnlyDiagElements
  * (PsociGAhamiltonian.C:13171-13250).  diag[n] host. */
 int psoci_diag(psoci_ctx* ctx, double* diag);

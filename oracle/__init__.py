@@ -75,6 +75,8 @@ def lib():
         L.orc_davidson.restype = ctypes.c_int
         L.orc_davidson.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, P, P, P, P]
         L.orc_rdm1.argtypes = [P, P, P]
+        L.orc_direct_rows.restype = ctypes.c_int
+        L.orc_direct_rows.argtypes = [P, ctypes.c_long, P, ctypes.c_int, P, P, ctypes.c_int]
         L.orc_two_e_index.restype = ctypes.c_int
         L.orc_two_e_index.argtypes = [ctypes.c_int] * 2
         L.orc_canonical_index.restype = ctypes.c_int
@@ -180,6 +182,18 @@ class OracleSpace:
         ns = ctypes.c_int(0)
         it = lib().orc_davidson(self._h, k, rtol, maxit, guess, _p(ev), _p(vecs), _p(rn), ctypes.byref(ns))
         return dict(evals=ev, evecs=vecs, rnorms=rn, iterations=it, nsigma=ns.value)
+
+    def direct_rows(self, rows, X: np.ndarray, nthreads: int = 0):
+        """Rows of the direct sigma for the listed configurations, packed in
+        list order ([sum nsef][k])."""
+        rows = np.ascontiguousarray(rows, np.int64)
+        X = np.ascontiguousarray(X, np.float64)
+        k = X.shape[1]
+        Y = np.zeros((int(self.nsef[rows].sum()), k))
+        rc = lib().orc_direct_rows(self._h, len(rows), _p(rows), k, _p(X), _p(Y), nthreads)
+        if rc != 0:
+            raise RuntimeError(f"oracle direct rows failed ({rc})")
+        return Y
 
     def rdm1(self, c: np.ndarray):
         c = np.ascontiguousarray(c, np.float64)

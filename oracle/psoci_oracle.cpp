@@ -53,7 +53,9 @@ struct Space {
   std::vector<int> nopen, ndet, nsef;
   std::vector<long> sef_off;  // computeNsefEntryPoints
   long nsef_total = 0;
-  std::vector<Joutfg> conf;  // generated once; the reference regenerates per use
+  std::vector<Joutfg> conf;  // generated once for small spaces; large spaces
+                             // regenerate per use, as the reference does
+  bool prebuilt = true;
   std::vector<double> valone, valtwo;
   bool have_ints = false;
   // stored Hamiltonian (lower triangle + diagonal, diagonal last in each row)
@@ -650,6 +652,32 @@ struct Ham {
   }
 };
 
+// computeConfigs(orbmap, JOUTFG&) (PsociDeterminants.C:1083-1191):
+// determinants of one configuration, generated on demand.
+void make_conf(const Space& sp, long c, Joutfg& j) {
+  j = Joutfg();
+  j.index = (int)c + 1;
+  j.jsym = sp.jsym[c];
+  for (int o = 0; o < sp.nbf; ++o) {
+    if (sp.occ[c][o] == 1) j.singly.push_back(o);
+    if (sp.occ[c][o] == 2) j.doubly.push_back(o);
+  }
+  j.num_singly = (int)j.singly.size();
+  j.num_doubly = (int)j.doubly.size();
+  int h = std::max(j.num_singly - 1, 0) / 2;
+  j.nsefi = (1 << h) * (1 << h);
+  j.ndeti = 1 << std::max(j.num_singly - 1, 0);
+  std::string e;
+  even_determinants(j, sp.mxopen / 2, e);
+  phase_and_sort(j, sp.nbf);
+}
+
+const Joutfg& conf_of(const Space& sp, long c, Joutfg& tmp) {
+  if (sp.prebuilt) return sp.conf[c];
+  make_conf(sp, c, tmp);
+  return tmp;
+}
+
 Ham make_ham(const Space& sp) {
   Ham h;
   h.nbf = sp.nbf;
@@ -821,8 +849,11 @@ void* orc_space_new(int nbf, int nelec, int ksym, long ncfg, const unsigned char
   int msbas = mxopen / 2;
   std::string e2;
   long off = 0;
+  sp->prebuilt = ncfg <= 300000;
+  if (!sp->prebuilt) sp->conf.clear();
   for (long c = 0; c < ncfg; ++c) {
-    Joutfg& j = sp->conf[c];
+    Joutfg jtmp;
+    Joutfg& j = sp->prebuilt ? sp->conf[c] : jtmp;
     j.index = (int)c + 1;
     j.jsym = sp->jsym[c];
     for (int o = 0; o < nbf; ++o) {
@@ -839,12 +870,19 @@ void* orc_space_new(int nbf, int nelec, int ksym, long ncfg, const unsigned char
       delete sp;
       return nullptr;
     }
-    if (!even_determinants(j, msbas, e2)) {
-      set_err(err, errlen, e2);
+    if (j.num_singly > 10) {
+      set_err(err, errlen, "EVEN class supports 0,2,4,6,8,10 open shells");
       delete sp;
       return nullptr;
     }
-    phase_and_sort(j, nbf);
+    if (sp->prebuilt) {
+      if (!even_determinants(j, msbas, e2)) {
+        set_err(err, errlen, e2);
+        delete sp;
+        return nullptr;
+      }
+      phase_and_sort(j, nbf);
+    }
     sp->nopen[c] = j.num_singly;
     sp->ndet[c] = j.ndeti;
     sp->nsef[c] = j.nsefi;
@@ -876,7 +914,8 @@ void orc_space_info(void* h, int* nopen, int* ndet, int* nsef, long* sef_off) {
 int orc_dets(void* h, long I, int raw, int* alpha, int* beta, int* ms, int* phase, int* suma, int* sumb) {
   Space* sp = static_cast<Space*>(h);
   if (I < 0 || I >= sp->ncfg) return -1;
-  Joutfg c = sp->conf[I];
+  Joutfg ctmp;
+  Joutfg c = conf_of(*sp, I, ctmp);
   if (raw) {
     int msbas = sp->mxopen / 2;
     std::string e;
@@ -921,7 +960,8 @@ int orc_block(void* h, long I, long J, double* sefsef) {
   Space* sp = static_cast<Space*>(h);
   Ham hm = make_ham(*sp);
   std::vector<double> blk;
-  int st = hm.block(sp->conf[I], sp->conf[J], blk, sp->err);
+  Joutfg ti, tj;
+  int st = hm.block(conf_of(*sp, I, ti), conf_of(*sp, J, tj), blk, sp->err);
   std::copy(blk.begin(), blk.end(), sefsef);
   return st;
 }
@@ -952,7 +992,8 @@ long orc_build_h(void* h, int nthreads) {
 #pragma omp parallel for schedule(dynamic, 4) reduction(| : bad)
 #endif
   for (long I = 0; I < ncfg; ++I) {
-    const Joutfg& ic = sp->conf[I];
+    Joutfg ti;
+    const Joutfg& ic = conf_of(*sp, I, ti);
     std::vector<long> list;
     jlist_for(*sp, I, list);
     auto& cimat = cim[I];
@@ -962,7 +1003,8 @@ long orc_build_h(void* h, int nthreads) {
     std::vector<double> sefsef;
     std::string err;
     for (long J : list) {
-      const Joutfg& jc = sp->conf[J];
+      Joutfg tj;
+      const Joutfg& jc = conf_of(*sp, J, tj);
       int st = hm.block(ic, jc, sefsef, err);
       if (st < 0) { bad = 1; continue; }
       if (!st) continue;
@@ -1213,10 +1255,12 @@ void orc_rdm1(void* h, const double* c, double* D) {
   std::fill(D, D + nd, 0.0);
   std::vector<double> blk;
   for (long I = 0; I < sp->ncfg; ++I) {
-    const Joutfg& ic = sp->conf[I];
+    Joutfg ti;
+    const Joutfg& ic = conf_of(*sp, I, ti);
     for (long J = 0; J <= I; ++J) {
       if (!orbmap_connected(sp->occ[I], sp->occ[J], 1)) continue;
-      const Joutfg& jc = sp->conf[J];
+      Joutfg tj;
+      const Joutfg& jc = conf_of(*sp, J, tj);
       int st = hm.block_ciden(ic, jc, blk);
       if (!st) continue;
       int nsefi = ic.nsefi, nsefj = jc.nsefi;
@@ -1250,6 +1294,63 @@ void orc_rdm1(void* h, const double* c, double* D) {
       }
     }
   }
+}
+
+// Direct sigma for selected rows: y_I = sum_J H_IJ x_J with every block
+// regenerated (determinants of J recomputed per pair, as the reference's
+// direct fetch does: computeConfigsGA per pair, PsociGAhamiltonian.C:2763;
+// blocks by generateSOCIblocks in the reference orientation, J <= I, used
+// transposed otherwise; matrixVectorUpdate semantics, :13036-13115).
+// X is [n][k]; Y receives the rows of the listed configurations, packed in
+// list order ([sum nsef][k]).  OpenMP over the listed rows.
+int orc_direct_rows(void* h, long nrows, const long* rows, int k, const double* X, double* Y, int nthreads) {
+  Space* sp = static_cast<Space*>(h);
+  if (!sp->have_ints) return -1;
+  Ham hm = make_ham(*sp);
+  std::vector<long> yoff(nrows + 1, 0);
+  for (long r = 0; r < nrows; ++r) yoff[r + 1] = yoff[r] + sp->nsef[rows[r]];
+  int bad = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+#endif
+  for (long r = 0; r < nrows; ++r) {
+    const long I = rows[r];
+    double* y = Y + yoff[r] * k;
+    std::fill(y, y + (long)sp->nsef[I] * k, 0.0);
+    Joutfg ti;
+    const Joutfg& ic = conf_of(*sp, I, ti);
+    std::vector<double> blk;
+    std::string err;
+    for (long J = 0; J < sp->ncfg; ++J) {
+      if (std::abs(sp->nopen[I] - sp->nopen[J]) > 4) continue;
+      if (!orbmap_connected(sp->occ[I], sp->occ[J], 2)) continue;
+      Joutfg tj;
+      const Joutfg& jc = conf_of(*sp, J, tj);
+      int ni = ic.nsefi, nj = jc.nsefi;
+      const double* x = X + sp->sef_off[J] * k;
+      if (J <= I) {
+        int st = hm.block(ic, jc, blk, err);
+        if (st < 0) bad = 1;
+        if (st <= 0) continue;
+        for (int i = 0; i < ni; ++i)
+          for (int j = 0; j < nj; ++j) {
+            double hv = blk[i * nj + j];
+            for (int v = 0; v < k; ++v) y[i * k + v] += hv * x[j * k + v];
+          }
+      } else {
+        int st = hm.block(jc, ic, blk, err);
+        if (st < 0) bad = 1;
+        if (st <= 0) continue;
+        for (int j = 0; j < nj; ++j)
+          for (int i = 0; i < ni; ++i) {
+            double hv = blk[j * ni + i];
+            for (int v = 0; v < k; ++v) y[i * k + v] += hv * x[j * k + v];
+          }
+      }
+    }
+  }
+  return bad ? -2 : 0;
 }
 
 // Canonical packed indices, exported so tests can check them against brute

@@ -89,6 +89,7 @@ def load(path: str | None = None):
         "psoci_rdm": (I, [V, I, V, V, V, V, V]),
         "psoci_device_ptr": (I, [V, I, ctypes.POINTER(V)]),
         "psoci_project": (I, [V, I64, I, I, V, I, V, I, V]),
+        "psoci_direct_info": (I, [V, V, V]),
         "psoci_sync": (I, [V]),
     }
     for name, (res, args) in sig.items():
@@ -235,6 +236,12 @@ class Hamiltonian:
         self._check(self.L.psoci_project(self._h, n, m, b, ctypes.c_void_p(A.data_ptr()), A.stride(0),
                                          ctypes.c_void_p(B.data_ptr()), B.stride(0), _p(out)))
         return out
+
+    def direct_info(self):
+        """(connected pairs, structural SEF elements) per direct sigma."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.L.psoci_direct_info(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def stream_handle(self) -> int:
         """The hipStream_t the library launches on (as an integer)."""

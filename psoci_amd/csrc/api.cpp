@@ -155,6 +155,16 @@ int psoci_get_h(psoci_ctx* ctx, int64_t* rowptr, int32_t* col, double* val, doub
   });
 }
 
+int psoci_direct_info(psoci_ctx* ctx, int64_t* npairs, int64_t* nelems) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
+    if (!c.have_pairs) psoci::build_pairs(c);
+    if (npairs) *npairs = c.nadj;
+    if (nelems) *nelems = c.direct_elems;
+    return PSOCI_OK;
+  });
+}
+
 int psoci_diag(psoci_ctx* ctx, double* diag) {
   return guarded(ctx, [&](Ctx& c) {
     PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");

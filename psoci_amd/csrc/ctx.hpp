@@ -42,6 +42,7 @@ struct Ctx {
   // open-shell masks
   bool have_pairs = false;
   int64_t nadj = 0;
+  int64_t direct_elems = 0;  // structural SEF elements per direct sigma
   DevBuf<int64_t> d_adjoff;   // ncfg + 1
   DevBuf<int32_t> d_adj;
   DevBuf<int64_t> d_detoff;   // ncfg

@@ -102,6 +102,18 @@ __global__ __launch_bounds__(256) void k_adj(int64_t ncfg, const uint64_t* __res
   }
 }
 
+// structural elements per sigma: sum over I of nsef_I * sum over adj(I) of nsef_J
+__global__ void k_count_elems(int64_t ncfg, const int64_t* __restrict__ adjoff, const int32_t* __restrict__ adj,
+                              const int32_t* __restrict__ nsef, unsigned long long* __restrict__ total) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (I >= ncfg) return;
+  unsigned long long s = 0;
+  for (int64_t t = adjoff[I] + lane; t < adjoff[I + 1]; t += kWave) s += (unsigned long long)nsef[adj[t]];
+  for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
+  if (lane == 0) atomicAdd(total, s * (unsigned long long)nsef[I]);
+}
+
 __global__ void k_cnt64(int64_t n, const int32_t* __restrict__ a, int64_t* __restrict__ b) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) b[t] = a[t];
@@ -155,6 +167,9 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
   const uint64_t aI0 = a.detao[dI0 + (A.nopen ? 2 * i : 0)];
   const uint64_t aI1 = A.nopen ? a.detao[dI0 + 2 * i + 1] : 0ull;
   const int msI = abs(2 * popc(aI0) - A.nopen);
+  // full alpha / beta orbital sets of determinant 2i (or the only one)
+  const uint64_t fa = A.d | aI0, fb = A.d | (A.s & ~aI0);
+  const int na = popc(fa);
   T acc[K];
 #pragma unroll
   for (int v = 0; v < K; ++v) acc[v] = (T)0;
@@ -175,6 +190,8 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
     else
       pair_setup(P, J, (int)I, B.d, B.s, B.nopen, A.d, A.s, A.nopen, a.nbf, a.one, a.two);
     if (P.kind == 0) continue;
+    PairX px;
+    if (J != I) pair_x(P, rowI ? A : B, a.two, px);
     const int nJ = a.nsef[J];
     const int64_t dJ0 = a.detoff[J];
     const T* xJ = X + a.sefoff[J] * (int64_t)ldx;
@@ -182,8 +199,31 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
       const uint64_t aJ0 = a.detao[dJ0 + (B.nopen ? 2 * j : 0)];
       if (abs(2 * popc(aJ0) - B.nopen) != msI) continue;
       const uint64_t aJ1 = B.nopen ? a.detao[dJ0 + 2 * j + 1] : 0ull;
-      const double h = rowI ? sef_elem_ao(P, A, B, aI0, aI1, aJ0, aJ1, a.mult, a.mult, a.two)
-                            : sef_elem_ao(P, B, A, aJ0, aJ1, aI0, aI1, a.mult, a.mult, a.two);
+      // exact zero test: every determinant pair of the element is det 2i (or
+      // its spin flip) against det 2j or 2j+1 (or their flips); a pair is
+      // nonzero only with equal alpha counts and at most two moved
+      // spin-orbitals
+      {
+        const uint64_t ga = B.d | aJ0, gb = B.d | (B.s & ~aJ0);
+        bool near = popc(ga) == na && popc(ga & ~fa) + popc(gb & ~fb) <= 2;
+        if (!near && B.nopen) {
+          const uint64_t ha = B.d | aJ1, hb = B.d | (B.s & ~aJ1);
+          near = popc(ha) == na && popc(ha & ~fa) + popc(hb & ~fb) <= 2;
+        }
+        if (!near && !B.nopen && A.nopen) {
+          // closed-shell column: test det 2i+1 as well
+          const uint64_t ea = A.d | aI1, eb = A.d | (A.s & ~aI1);
+          near = popc(ga) == popc(ea) && popc(ga & ~ea) + popc(gb & ~eb) <= 2;
+        }
+        if (!near) continue;
+      }
+      double h;
+      if (J == I)
+        h = sef_elem_ao(P, A, B, aI0, aI1, aJ0, aJ1, a.mult, a.mult, a.two);
+      else if (rowI)
+        h = sef_elem_x(P, px, A, B, aI0, aI1, aJ0, aJ1, a.mult, a.mult, a.two);
+      else
+        h = sef_elem_x(P, px, B, A, aJ0, aJ1, aI0, aI1, a.mult, a.mult, a.two);
       if (h == 0.0) continue;
       const T hv = (T)h;
       const T* __restrict__ x = xJ + (int64_t)j * ldx;
@@ -286,7 +326,16 @@ void build_pairs(Ctx& c) {
   hipLaunchKernelGGL(k_adj<true>, dim3(grid), dim3(256), 0, c.stream, ncfg, c.d_dm.p, c.d_sm.p, c.d_nopen.p, nullptr,
                      c.d_adjoff.p, c.d_adj.p);
   PSOCI_HIP(hipGetLastError());
+  DevBuf<unsigned long long> ne;
+  ne.alloc(1);
+  PSOCI_HIP(hipMemsetAsync(ne.p, 0, sizeof(unsigned long long), c.stream));
+  hipLaunchKernelGGL(k_count_elems, dim3(nblk(ncfg, 4)), dim3(256), 0, c.stream, ncfg, c.d_adjoff.p, c.d_adj.p,
+                     c.d_nsef.p, ne.p);
+  PSOCI_HIP(hipGetLastError());
+  unsigned long long nelems = 0;
+  ne.download(&nelems, 1, c.stream);
   c.sync();
+  c.direct_elems = (int64_t)nelems;
   c.nadj = total;
   c.have_pairs = true;
 }

@@ -275,6 +275,116 @@ PSOCI_HD double sef_elem_ao(const PairInfo& P, const ConfView& A, const ConfView
   return t;
 }
 
+// ---------------------------------------------------------------------------
+// Load-free evaluation for the off-diagonal pairs of the direct sigma.
+//
+// For a single excitation e0 -> f0 every two-electron integral a determinant
+// pair needs is (e0 c|c f0) for an open shell c of the row configuration:
+// the exchange sum of generateSEFCIG's single branch runs over such c, and
+// its alpha-beta branch (the moved electron flips spin while c flips the
+// other way) reads (f0 c|c e0) or (c e0|f0 c), the same canonical slot.  For
+// a double excitation the alpha-beta branch reads vnt[0] or vnt[1].  So with
+// X[t] = (e0 o_t|o_t f0) for the row configuration's open shells o_t staged in
+// registers once per pair, the element loop touches no memory; the values
+// and their summation order are the reference's, so elements stay
+// bit-identical to detval_ao().
+struct PairX {
+  double X[8];
+};
+
+PSOCI_HD void pair_x(const PairInfo& P, const ConfView& A, const double* __restrict__ two, PairX& px) {
+  uint64_t rem = A.s;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    double v = 0.0;
+    if (rem) {
+      int o = ctz(rem);
+      rem &= rem - 1;
+      if (P.kind == 2) v = g2(two, P.e0, o, o, P.f0);
+    }
+    px.X[t] = v;
+  }
+}
+
+PSOCI_HD double pick8(const PairX& px, int t) {
+  double v = px.X[0];
+#pragma unroll
+  for (int u = 1; u < 8; ++u) v = (t == u) ? px.X[u] : v;
+  return v;
+}
+
+// detval_ao() for off-diagonal pairs (kind 2 or 3) without integral loads
+PSOCI_HD double detval_x(const PairInfo& P, const PairX& px, const ConfView& A, const ConfView& B, uint64_t ao,
+                         uint64_t bo, const double* __restrict__ two) {
+  uint64_t aI = A.d | ao, bI = A.d | (A.s & ~ao);
+  uint64_t aJ = B.d | bo, bJ = B.d | (B.s & ~bo);
+  if (popc(aI) != popc(aJ)) return 0.0;
+  uint64_t xa = aJ & ~aI, xb = bJ & ~bI;
+  int ma = popc(xa), mb = popc(xb);
+  int tot = ma + mb;
+  if (tot > 2 || tot == 0) return tot == 0 ? detval_ao(P, A, B, ao, bo, two) : 0.0;
+  int cnt = A.ph + B.ph;
+  double v;
+  if (tot == 1) {
+    uint64_t S = ma ? aI : bI;
+    cnt += popc(S & between(P.e0, P.f0));
+    v = P.v0;
+    const uint64_t T = S & ~A.d & ~(1ull << P.e0) & ~(1ull << P.f0);
+    uint64_t rem = A.s;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      if (rem) {
+        int o = ctz(rem);
+        rem &= rem - 1;
+        if ((T >> o) & 1ull) v -= px.X[t];
+      }
+    }
+  } else if (ma == 1) {
+    int iex2 = ctz(xa), jex2 = ctz(xb);
+    int iex3 = ctz(aI & ~aJ), jex3 = ctz(bI & ~bJ);
+    cnt += popc(aI & between(iex2, iex3)) + popc(bI & between(jex2, jex3));
+    if (P.kind == 3) {
+      // (jex2 jex3|iex2 iex3): alpha lost iex3, gained iex2
+      v = ((iex3 == P.e0) == (iex2 == P.f0)) ? P.v0 : P.v1;
+    } else {
+      int c = (iex2 == jex3) ? iex2 : ((iex3 == jex2) ? iex3 : -1);
+      if (c >= 0 && ((A.s >> c) & 1ull) && c != P.e0 && c != P.f0)
+        v = pick8(px, popc(A.s & ((1ull << c) - 1ull)));
+      else
+        v = g2(two, jex2, jex3, iex2, iex3);
+    }
+  } else {
+    uint64_t SI = ma ? aI : bI, SJ = ma ? aJ : bJ;
+    cnt += popc(SJ & between(P.f0, P.f1)) + popc(SI & between(P.e0, P.e1));
+    v = P.v0 - P.v1;
+  }
+  return (cnt & 1) ? -v : v;
+}
+
+// sef_elem_ao() for off-diagonal pairs, same operation order
+PSOCI_HD double sef_elem_x(const PairInfo& P, const PairX& px, const ConfView& A, const ConfView& B, uint64_t aI0,
+                           uint64_t aI1, uint64_t aJ0, uint64_t aJ1, double multi, double multj,
+                           const double* __restrict__ two) {
+  double ds0, ds1 = 0.0;
+  if (B.nopen == 0) {
+    ds0 = detval_x(P, px, A, B, aI0, aJ0, two);
+    if (A.nopen) ds1 = detval_x(P, px, A, B, aI1, aJ0, two);
+  } else {
+    double t = detval_x(P, px, A, B, aI0, aJ0, two);
+    t += multj * detval_x(P, px, A, B, aI0, aJ1, two);
+    ds0 = t * kRthlf;
+    if (A.nopen) {
+      double u = detval_x(P, px, A, B, aI1, aJ0, two);
+      u += multj * detval_x(P, px, A, B, aI1, aJ1, two);
+      ds1 = u * kRthlf;
+    }
+  }
+  if (A.nopen == 0) return ds0;
+  double t = kRthlf * ds0;
+  t += (multi * kRthlf) * ds1;
+  return t;
+}
+
 PSOCI_HD double sef_elem(const PairInfo& P, const ConfView& A, const ConfView& B, int i, int j, double multi,
                          double multj, const double* __restrict__ two) {
   const uint64_t aI0 = deposit(A.pat[A.nopen ? 2 * i : 0], A.s);

@@ -34,6 +34,7 @@ def hc():
     L.hc_space_new.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, P, P]
     L.hc_space_free.argtypes = [P]
     L.hc_block.argtypes = [P, P, P, ctypes.c_int64, ctypes.c_int64, P]
+    L.hc_block_x.argtypes = [P, P, P, ctypes.c_int64, ctypes.c_int64, P]
     L.hc_connected.argtypes = [P, ctypes.c_int64, ctypes.c_int64]
     return L
 
@@ -61,6 +62,12 @@ def _compare(L, cfg, ints, rows):
                 st2 = L.hc_block(h, _p(ints.valone), _p(ints.valtwo), I, J, _p(out))
                 assert st == st2, (I, J)
                 assert np.array_equal(ref, out), (I, J, np.abs(ref - out).max())
+                if I != J:
+                    # the load-free path used by the direct sigma
+                    out_x = np.zeros_like(ref)
+                    st3 = L.hc_block_x(h, _p(ints.valone), _p(ints.valtwo), I, J, _p(out_x))
+                    assert st3 == st, (I, J)
+                    assert np.array_equal(ref, out_x), (I, J, np.abs(ref - out_x).max())
                 nblocks += 1
     finally:
         L.hc_space_free(h)
