@@ -151,12 +151,19 @@ struct DirectArgs {
 template <typename T, int K>
 __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restrict__ X, int ldx, T* __restrict__ Y,
                                                 int ldy) {
+  // per wavefront: the current pair's column determinants and x rows
+  __shared__ uint64_t s_mask[4][128];
+  __shared__ T s_x[4][64 * K];
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-  if (I >= a.ncfg) return;  // uniform per wavefront
+  const int wave = threadIdx.x / kWave;
+  const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + wave;
+  if (I >= a.ncfg) return;  // uniform per wavefront; no block-wide barrier below
+  uint64_t* __restrict__ sm_ = s_mask[wave];
+  T* __restrict__ sx = s_x[wave];
   const int nI = a.nsef[I];
+  // lanes = (row i, column slice c): every lane works on the same pair
   const int g = kWave / nI;
-  const int i = lane / g, q = lane - i * g;
+  const int i = lane / g, c = lane - i * g;
   ConfView A;
   A.d = a.dm[I];
   A.s = a.sm[I];
@@ -174,7 +181,7 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
 #pragma unroll
   for (int v = 0; v < K; ++v) acc[v] = (T)0;
   const int64_t t1 = a.adjoff[I + 1];
-  for (int64_t t = a.adjoff[I] + q; t < t1; t += g) {
+  for (int64_t t = a.adjoff[I]; t < t1; ++t) {
     const int J = a.adj[t];
     ConfView B;
     B.d = a.dm[J];
@@ -193,12 +200,24 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
     PairX px;
     if (J != I) pair_x(P, rowI ? A : B, a.two, px);
     const int nJ = a.nsef[J];
+    const int ndJ = B.nopen ? 2 * nJ : 1;
     const int64_t dJ0 = a.detoff[J];
-    const T* xJ = X + a.sefoff[J] * (int64_t)ldx;
-    for (int j = 0; j < nJ; ++j) {
-      const uint64_t aJ0 = a.detao[dJ0 + (B.nopen ? 2 * j : 0)];
+    const T* __restrict__ xJ = X + a.sefoff[J] * (int64_t)ldx;
+    // stage J's determinant masks and x rows in LDS with coalesced loads; LDS
+    // operations of one wavefront complete in order, and the fence keeps the
+    // compiler from moving the reads above the writes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    for (int m = lane; m < ndJ; m += kWave) sm_[m] = a.detao[dJ0 + m];
+    for (int e = lane; e < nJ * K; e += kWave) {
+      const int r = e / K, v = e - r * K;
+      sx[e] = xJ[(int64_t)r * ldx + v];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int j = c; j < nJ; j += g) {
+      const uint64_t aJ0 = sm_[B.nopen ? 2 * j : 0];
       if (abs(2 * popc(aJ0) - B.nopen) != msI) continue;
-      const uint64_t aJ1 = B.nopen ? a.detao[dJ0 + 2 * j + 1] : 0ull;
+      const uint64_t aJ1 = B.nopen ? sm_[2 * j + 1] : 0ull;
       // exact zero test: every determinant pair of the element is det 2i (or
       // its spin flip) against det 2j or 2j+1 (or their flips); a pair is
       // nonzero only with equal alpha counts and at most two moved
@@ -226,16 +245,17 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
         h = sef_elem_x(P, px, B, A, aJ0, aJ1, aI0, aI1, a.mult, a.mult, a.two);
       if (h == 0.0) continue;
       const T hv = (T)h;
-      const T* __restrict__ x = xJ + (int64_t)j * ldx;
+      const T* x = sx + j * K;
 #pragma unroll
       for (int v = 0; v < K; ++v) acc[v] = fmadd(hv, x[v], acc[v]);
     }
+    __builtin_amdgcn_wave_barrier();
   }
   for (int off = g >> 1; off > 0; off >>= 1) {
 #pragma unroll
     for (int v = 0; v < K; ++v) acc[v] += __shfl_xor(acc[v], off, kWave);
   }
-  if (q == 0) {
+  if (c == 0) {
     T* y = Y + (a.sefoff[I] + i) * (int64_t)ldy;
 #pragma unroll
     for (int v = 0; v < K; ++v) y[v] = acc[v];

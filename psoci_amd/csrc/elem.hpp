@@ -169,9 +169,13 @@ PSOCI_HD void pair_setup(PairInfo& P, int I, int J, uint64_t dI, uint64_t sI, in
   if (ni == 1) {
     int ex = ie[0], jx = je[0];
     double v = h1(one, ex, jx) - g2(two, ex, jx, ex, ex);
-    for (int i = 0; i < nbf; ++i) {
+    // occupied orbitals of I in ascending order (the reference loops over all
+    // orbitals and skips the empty ones)
+    uint64_t occ = dI | sI;
+    while (occ) {
+      const int i = ctz(occ);
+      occ &= occ - 1;
       int oi = (int)((dI >> i) & 1) * 2 + (int)((sI >> i) & 1);
-      if (!oi) continue;
       double buf = g2(two, ex, jx, i, i);
       v += buf;
       if (oi == 2) v += buf;
