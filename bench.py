@@ -52,25 +52,68 @@ def cpu_model():
     return "unknown"
 
 
-def time_sigma(H, api, torch, k, prec, reps):
-    """Average duration of one stored-sigma launch set, timed with HIP events
-    recorded on the library's own stream."""
+def time_sigma(H, api, torch, k, prec, reps, mode, warm=3):
+    """Average duration of one sigma launch set (all kernels of one call),
+    timed with HIP events recorded on the library's own stream."""
     n = H.n
     dt = torch.float32 if prec == api.FP32 else torch.float64
     X = torch.randn(n, k, device="cuda", dtype=torch.float64).to(dt).contiguous()
     Y = torch.empty_like(X)
     stream = torch.cuda.ExternalStream(H.stream_handle())
-    for _ in range(3):
-        H.sigma_device(X, Y, api.STORED, prec)
+    for _ in range(warm):
+        H.sigma_device(X, Y, mode, prec)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(reps):
-        H.sigma_device(X, Y, api.STORED, prec)
+        H.sigma_device(X, Y, mode, prec)
     e1.record(stream)
     e1.synchronize()
     return e0.elapsed_time(e1) / reps  # ms
+
+
+def cpu_stored(args, cfg, ints, k, ref):
+    import oracle
+    threads = min(os.cpu_count() or 1, 16)
+    sp = oracle.OracleSpace(cfg, ints)
+    tc = time.time()
+    sp.build_h(threads)
+    tb = time.time() - tc
+    tc = time.time()
+    oo = sp.davidson(k, rtol=args.tol, maxit=4000, guess=1)
+    td = time.time() - tc
+    return {"value": round(tb + td, 3), "unit": "s", "cores": threads, "kind": "port",
+            "sample": (f"whole solve, not sampled: oracle (C++ restatement of the reference CPU path) on "
+                       f"{args.workload}: H build on {threads} OpenMP threads {tb:.2f} s + reference-schedule "
+                       f"single-vector Davidson (1 thread) for {k} roots to residual {args.tol:g} "
+                       f"({oo['iterations']} iterations) {td:.2f} s; host {cpu_model()}"),
+            "max_root_diff_vs_gpu_fp64": float(np.abs(oo["evals"] - ref["evals"]).max())}
+
+
+def cpu_direct(args, cfg, ints, k, sigma_calls, budget_s=15.0):
+    """Sampled rows of the reference direct sigma (every block regenerated),
+    timed until about budget_s seconds of wall time, scaled to the solve."""
+    import oracle
+    threads = min(os.cpu_count() or 1, 16)
+    sp = oracle.OracleSpace(cfg, ints)
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((sp.n, k))
+    done, tot = 0, 0.0
+    while tot < budget_s and done < cfg.ncfg:
+        rows = rng.choice(cfg.ncfg, threads * 2, replace=False)
+        tc = time.time()
+        sp.direct_rows(rows, X, threads)
+        tot += time.time() - tc
+        done += len(rows)
+    per_sigma = tot / done * cfg.ncfg
+    return {"value": round(per_sigma * sigma_calls, 1), "unit": "s", "cores": threads, "kind": "port",
+            "extrapolated": True,
+            "sample": (f"oracle direct sigma (every block regenerated, the reference's direct path) on {done} "
+                       f"random configuration rows of {args.workload}, {k} vectors, {threads} OpenMP threads: "
+                       f"{tot:.1f} s; scaled by ncfg/{done} to {per_sigma:.0f} s per sigma and by the "
+                       f"{sigma_calls} sigma calls of the GPU mixed solve (Davidson vector work not counted); "
+                       f"host {cpu_model()}")}
 
 
 def main():
@@ -87,67 +130,81 @@ def main():
         sys.exit(3)
 
     cfg, ints, _ = inputs.workload(args.workload)
+    direct = inputs.WORKLOADS[args.workload]["kind"] != "fci"
+    mode = api.DIRECT if direct else api.STORED
     k = args.roots
+    n_sef = int(cfg.nsef().sum())
+    print(f"[bench] {args.workload}: ncfg={cfg.ncfg} nsef={n_sef} mode={'direct' if direct else 'stored'} k={k}",
+          file=sys.stderr, flush=True)
     t0 = time.time()
     H = api.Hamiltonian(cfg, ints)
-    nnz = H.build()
+    if direct:
+        npairs, nelems = H.direct_info()
+        nnz = None
+    else:
+        nnz = H.build()
     torch.cuda.synchronize()
-    build_s = time.time() - t0
+    setup_s = time.time() - t0
+    print(f"[bench] setup {setup_s:.2f} s", file=sys.stderr, flush=True)
 
     # all-FP64 reference solve of the same space
-    ref = H.davidson(k, prec=api.FP64, tol=args.tol, want_vectors=False)
+    ref = H.davidson(k, prec=api.FP64, mode=mode, tol=args.tol, want_vectors=False)
     if not ref["converged"]:
         raise SystemExit("FP64 Davidson did not converge")
     fp64_s = ref["stats"]["seconds"]
+    print(f"[bench] fp64 solve {fp64_s:.2f} s, {ref['stats']['iterations']} iterations", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
-        H.davidson(k, prec=api.MIXED, tol=args.tol, switch_tol=1e-4, want_vectors=False)
+        H.davidson(k, prec=api.MIXED, mode=mode, tol=args.tol, switch_tol=1e-4, want_vectors=False)
     torch.cuda.synchronize()
     times, dmax, last = [], 0.0, None
     t_all = time.time()
-    for _ in range(args.steps):
+    for s in range(args.steps):
         torch.cuda.synchronize()
         ts = time.time()
-        out = H.davidson(k, prec=api.MIXED, tol=args.tol, switch_tol=1e-4, want_vectors=False)
+        out = H.davidson(k, prec=api.MIXED, mode=mode, tol=args.tol, switch_tol=1e-4, want_vectors=False)
         torch.cuda.synchronize()
         times.append(time.time() - ts)
+        print(f"[bench] mixed step {s}: {times[-1]:.2f} s", file=sys.stderr, flush=True)
         if not out["converged"]:
             raise SystemExit("mixed-precision Davidson did not converge")
-        d = float(np.abs(out["evals"] - ref["evals"]).max())
-        dmax = max(dmax, d)
+        dmax = max(dmax, float(np.abs(out["evals"] - ref["evals"]).max()))
         last = out
     wall = time.time() - t_all
     if dmax > 1e-8:
         raise SystemExit(f"mixed-precision roots differ from FP64 by {dmax:.3e} Eh (> 1e-8)")
     mixed_s = float(np.mean(times))
 
-    # dominant kernel: the FP32 stored sigma over the 16-root block
+    # dominant kernel: the FP32 sigma over the k-vector block
     n = H.n
-    ms32 = time_sigma(H, api, torch, k, api.FP32, args.sigma_reps)
-    ms64 = time_sigma(H, api, torch, k, api.FP64, args.sigma_reps)
-    bytes32 = nnz * (4 + 4) + (n + 1) * 8 + k * n * (4 + 4)
-    bytes64 = nnz * (8 + 4) + (n + 1) * 8 + k * n * (8 + 8)
+    reps = args.sigma_reps if not direct else 1
+    warm = 3 if not direct else 1
+    ms32 = time_sigma(H, api, torch, k, api.FP32, reps, mode, warm)
+    ms64 = time_sigma(H, api, torch, k, api.FP64, reps, mode, warm)
+    if direct:
+        nbf = cfg.nbf
+        n2 = (nbf * (nbf + 1) // 2) * (nbf * (nbf + 1) // 2 + 1) // 2
+        bytes32 = k * n * (4 + 4) + 16 * cfg.ncfg + 8 * n2
+        bytes64 = k * n * (8 + 8) + 16 * cfg.ncfg + 8 * n2
+        kernel = f"k_direct<float,{k}> (direct FP32 sigma, {k} vectors)"
+    else:
+        bytes32 = nnz * (4 + 4) + (n + 1) * 8 + k * n * (4 + 4)
+        bytes64 = nnz * (8 + 4) + (n + 1) * 8 + k * n * (8 + 8)
+        kernel = f"k_spmm<float,{k}> (stored FP32 sigma, {k} vectors)"
     ach32 = bytes32 / (ms32 * 1e-3) / 1e9
     ach64 = bytes64 / (ms64 * 1e-3) / 1e9
 
     cpu = None
     if not args.no_cpu:
-        import oracle
-        threads = min(os.cpu_count() or 1, 16)
-        sp = oracle.OracleSpace(cfg, ints)
-        tc = time.time()
-        sp.build_h(threads)
-        tb = time.time() - tc
-        tc = time.time()
-        oo = sp.davidson(k, rtol=args.tol, maxit=4000, guess=1)
-        td = time.time() - tc
-        cpu = {"value": round(tb + td, 3), "unit": "s", "cores": threads, "kind": "port",
-               "sample": (f"oracle (C++ restatement of the reference CPU path) on {args.workload}: H build on "
-                          f"{threads} OpenMP threads {tb:.2f} s + reference-schedule single-vector Davidson "
-                          f"(1 thread) for {k} roots to residual {args.tol:g} ({oo['iterations']} iterations) "
-                          f"{td:.2f} s; host {cpu_model()}"),
-               "max_root_diff_vs_gpu_fp64": float(np.abs(oo["evals"] - ref["evals"]).max())}
+        if direct:
+            cpu = cpu_direct(args, cfg, ints, k, last["stats"]["sigma_calls"])
+        else:
+            cpu = cpu_stored(args, cfg, ints, k, ref)
 
+    desc = (f"{args.workload}: PSOCI SEF space n={n}, ncfg={cfg.ncfg}, "
+            + (f"direct sigma ({npairs} ordered configuration pairs, {nelems} structural elements per sigma)"
+               if direct else f"stored H nnz_lower={nnz}")
+            + f", {k} roots, mixed-precision block Davidson, residual tol {args.tol:g}")
     line = {
         "metric": METRIC,
         "value": round(mixed_s, 6),
@@ -161,11 +218,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 sigma -> f64 (mixed); f64 subspace",
         "data": "synthetic (seeded integrals and configurations, reference formats)",
-        "config": {
-            "workload": f"{args.workload}: PSOCI SEF space n={n}, ncfg={cfg.ncfg}, stored H nnz_lower={nnz}, "
-                        f"{k} roots, mixed-precision block Davidson, residual tol {args.tol:g}",
-            "roots": k, "n_sef": n, "n_cfg": cfg.ncfg, "nnz_lower": nnz, "sigma": "stored",
-        },
+        "config": {"workload": desc, "roots": k, "n_sef": n, "n_cfg": cfg.ncfg,
+                   "sigma": "direct" if direct else "stored"},
         "fp64_seconds": round(fp64_s, 6),
         "speedup_vs_fp64": round(fp64_s / mixed_s, 3),
         "max_root_diff_eh": dmax,
@@ -173,17 +227,23 @@ def main():
         "iterations_mixed": last["stats"]["iterations"],
         "switch_iter": last["stats"]["switch_iter"],
         "iterations_fp64": ref["stats"]["iterations"],
-        "h_build_seconds": round(build_s, 4),
+        "sigma_calls_mixed": last["stats"]["sigma_calls"],
+        "setup_seconds": round(setup_s, 4),
         "sigma_fp32_ms": round(ms32, 5),
         "sigma_fp64_ms": round(ms64, 5),
         "sigma_fp64_gbs": round(ach64, 1),
         "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"k_spmm<float,16> (stored FP32 sigma, {k} vectors)",
+                     "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
                      "algorithmic_bytes_per_launch": int(bytes32)},
         "cpu_baseline": cpu,
         "wall_seconds_timed": round(wall, 3),
     }
+    if direct:
+        line["direct"] = {"ordered_pairs": npairs, "structural_elements_per_sigma": nelems,
+                          "elements_per_s_fp32": round(nelems / (ms32 * 1e-3), 1),
+                          "elements_per_s_fp64": round(nelems / (ms64 * 1e-3), 1),
+                          "note": "compute-bound (integer + FP64 VALU); the HBM fraction is reported for the "
+                                  "algorithmic bytes only"}
     print(json.dumps(line))
 
 

@@ -43,8 +43,10 @@ struct Ctx {
   bool have_pairs = false;
   int64_t nadj = 0;
   int64_t direct_elems = 0;  // structural SEF elements per direct sigma
-  DevBuf<int64_t> d_adjoff;   // ncfg + 1
-  DevBuf<int32_t> d_adj;
+  DevBuf<int64_t> d_adjoff;   // ncfg + 1: first record of each configuration
+  DevBuf<PairRec> d_rec;      // one record per ordered connected pair
+  DevBuf<ConfRec> d_conf;     // packed configurations
+  DevBuf<double> d_xt;        // XT[e][f][o] = (e o|o f)
   DevBuf<int64_t> d_detoff;   // ncfg
   DevBuf<uint64_t> d_detao;   // ndet_total
 

@@ -7,15 +7,24 @@
 // rebuilds the block with generateSOCIblocks and scatter-accumulates both
 // y_I and y_J into a global array.
 //
-// Here one wavefront owns configuration I and walks every connected J, in
-// both directions, so y_I is reduced in registers and written once: no
-// atomics, a deterministic result, and X rows are only gathered.  Each block
-// is evaluated in the reference's own orientation (row = the later
-// configuration), so every element is bit-identical to the stored
-// Hamiltonian's.  Lanes are laid out as (SEF row i, pair slice q) with
-// nsef_I * g = 64; SEF pairs whose |Ms| classes differ are skipped (their
-// four determinant pairs all have different spin projections, so the
-// element is exactly zero).
+// Layout on the GPU:
+//  * once per space and integrals, every connected ordered pair (I, J) gets
+//    a 24-byte record: J, the excitation orbitals and block kind of the
+//    reference orientation (row = the later configuration), and its
+//    configuration-level sums vnt[0], vnt[1] (pair_setup, computed once and
+//    reused by every sigma of a solve);
+//  * a three-index table XT[e][f][o] = (e o|o f) (nbf^3 doubles, L2 resident)
+//    supplies the single-excitation exchange integrals of any pair;
+//  * one wavefront owns configuration I and walks its records, so y_I is
+//    reduced in registers and written once: no atomics, a deterministic
+//    result, X rows only gathered.  All lanes work on the same pair: lanes
+//    are (SEF row i, column slice c) with nsef_I * g = 64, and the column
+//    configuration's determinant masks and x rows are staged in LDS with
+//    coalesced loads.
+// Each block is evaluated in the reference's own orientation, so every
+// element is bit-identical to the stored Hamiltonian's.  SEF pairs whose
+// |Ms| classes differ, and those whose determinants are more than two
+// spin-orbitals apart, are skipped: their elements are exactly zero.
 #include "ctx.hpp"
 
 namespace psoci {
@@ -40,6 +49,33 @@ __global__ void k_detao(int64_t ncfg, const int64_t* __restrict__ detoff, const 
   const uint16_t* p = pat + patoff[s];
   const uint64_t mask = sm[c];
   for (int d = lane; d < nd; d += kWave) detao[detoff[c] + d] = deposit(p[d], mask);
+}
+
+__global__ void k_confrec(int64_t ncfg, const uint64_t* __restrict__ dm, const uint64_t* __restrict__ sm,
+                          const int8_t* __restrict__ nopen, const uint8_t* __restrict__ cph,
+                          const int32_t* __restrict__ nsef, const int64_t* __restrict__ sefoff,
+                          const int64_t* __restrict__ detoff, ConfRec* __restrict__ out) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncfg) return;
+  ConfRec r;
+  r.d = dm[c];
+  r.s = sm[c];
+  r.sefoff = (int32_t)sefoff[c];
+  r.detoff = (int32_t)detoff[c];
+  r.nopen = nopen[c];
+  r.cph = cph[c];
+  r.nsef = (int16_t)nsef[c];
+  r.pad = 0;
+  out[c] = r;
+}
+
+// XT[(e * nbf + f) * nbf + o] = (e o|o f)
+__global__ void k_xt(int nbf, const double* __restrict__ two, double* __restrict__ xt) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n3 = (int64_t)nbf * nbf * nbf;
+  if (t >= n3) return;
+  const int o = (int)(t % nbf), f = (int)((t / nbf) % nbf), e = (int)(t / ((int64_t)nbf * nbf));
+  xt[t] = g2(two, e, o, o, f);
 }
 
 // Connected configurations of every I (all J, ascending), screened through
@@ -102,21 +138,32 @@ __global__ __launch_bounds__(256) void k_adj(int64_t ncfg, const uint64_t* __res
   }
 }
 
-// structural elements per sigma: sum over I of nsef_I * sum over adj(I) of nsef_J
-__global__ void k_count_elems(int64_t ncfg, const int64_t* __restrict__ adjoff, const int32_t* __restrict__ adj,
-                              const int32_t* __restrict__ nsef, unsigned long long* __restrict__ total) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-  if (I >= ncfg) return;
-  unsigned long long s = 0;
-  for (int64_t t = adjoff[I] + lane; t < adjoff[I + 1]; t += kWave) s += (unsigned long long)nsef[adj[t]];
-  for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
-  if (lane == 0) atomicAdd(total, s * (unsigned long long)nsef[I]);
-}
-
 __global__ void k_cnt64(int64_t n, const int32_t* __restrict__ a, int64_t* __restrict__ b) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) b[t] = a[t];
+}
+
+// one record per ordered pair, in the reference orientation
+__global__ void k_records(int64_t ncfg, const int64_t* __restrict__ adjoff, const int32_t* __restrict__ adj,
+                          const uint64_t* __restrict__ dm, const uint64_t* __restrict__ sm,
+                          const int8_t* __restrict__ nopen, const int32_t* __restrict__ nsef, int nbf,
+                          const double* __restrict__ one, const double* __restrict__ two, PairRec* __restrict__ rec,
+                          unsigned long long* __restrict__ nelems) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (I >= ncfg) return;
+  unsigned long long ne = 0;
+  for (int64_t t = adjoff[I] + lane; t < adjoff[I + 1]; t += kWave) {
+    const int J = adj[t];
+    const bool rowI = J <= I;
+    const int R = rowI ? (int)I : J, C = rowI ? J : (int)I;
+    PairInfo P;
+    pair_setup(P, R, C, dm[R], sm[R], nopen[R], dm[C], sm[C], nopen[C], nbf, one, two);
+    rec[t] = make_rec(P, J, rowI);
+    ne += (unsigned long long)nsef[J];
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) ne += __shfl_xor(ne, off, kWave);
+  if (lane == 0) atomicAdd(nelems, ne * (unsigned long long)nsef[I]);
 }
 
 template <typename T>
@@ -132,21 +179,25 @@ __device__ __forceinline__ float fmadd<float>(float a, float b, float c) {
 
 struct DirectArgs {
   int64_t ncfg;
-  const int64_t* adjoff;
-  const int32_t* adj;
-  const uint64_t* dm;
-  const uint64_t* sm;
-  const int8_t* nopen;
-  const uint8_t* cph;
-  const int32_t* nsef;
-  const int64_t* sefoff;
-  const int64_t* detoff;
+  const int64_t* recoff;
+  const PairRec* rec;
+  const ConfRec* conf;
   const uint64_t* detao;
+  const double* xt;
   int nbf;
   double mult;
-  const double* one;
   const double* two;
 };
+
+__device__ __forceinline__ ConfView view_of(const ConfRec& r) {
+  ConfView v;
+  v.d = r.d;
+  v.s = r.s;
+  v.nopen = r.nopen;
+  v.ph = r.cph;
+  v.pat = nullptr;
+  return v;
+}
 
 template <typename T, int K>
 __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restrict__ X, int ldx, T* __restrict__ Y,
@@ -158,21 +209,16 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
   const int wave = threadIdx.x / kWave;
   const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + wave;
   if (I >= a.ncfg) return;  // uniform per wavefront; no block-wide barrier below
-  uint64_t* __restrict__ sm_ = s_mask[wave];
+  uint64_t* __restrict__ smk = s_mask[wave];
   T* __restrict__ sx = s_x[wave];
-  const int nI = a.nsef[I];
+  const ConfRec ri = a.conf[I];
+  const ConfView A = view_of(ri);
+  const int nI = ri.nsef;
   // lanes = (row i, column slice c): every lane works on the same pair
   const int g = kWave / nI;
   const int i = lane / g, c = lane - i * g;
-  ConfView A;
-  A.d = a.dm[I];
-  A.s = a.sm[I];
-  A.nopen = a.nopen[I];
-  A.ph = a.cph[I];
-  A.pat = nullptr;
-  const int64_t dI0 = a.detoff[I];
-  const uint64_t aI0 = a.detao[dI0 + (A.nopen ? 2 * i : 0)];
-  const uint64_t aI1 = A.nopen ? a.detao[dI0 + 2 * i + 1] : 0ull;
+  const uint64_t aI0 = a.detao[ri.detoff + (A.nopen ? 2 * i : 0)];
+  const uint64_t aI1 = A.nopen ? a.detao[ri.detoff + 2 * i + 1] : 0ull;
   const int msI = abs(2 * popc(aI0) - A.nopen);
   // full alpha / beta orbital sets of determinant 2i (or the only one)
   const uint64_t fa = A.d | aI0, fb = A.d | (A.s & ~aI0);
@@ -180,44 +226,50 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
   T acc[K];
 #pragma unroll
   for (int v = 0; v < K; ++v) acc[v] = (T)0;
-  const int64_t t1 = a.adjoff[I + 1];
-  for (int64_t t = a.adjoff[I]; t < t1; ++t) {
-    const int J = a.adj[t];
-    ConfView B;
-    B.d = a.dm[J];
-    B.s = a.sm[J];
-    B.nopen = a.nopen[J];
-    B.ph = a.cph[J];
-    B.pat = nullptr;
-    // the reference's orientation: row = the later configuration
-    const bool rowI = J <= I;
+  const int64_t t1 = a.recoff[I + 1];
+  for (int64_t t = a.recoff[I]; t < t1; ++t) {
+    const PairRec r = a.rec[t];
     PairInfo P;
-    if (rowI)
-      pair_setup(P, (int)I, J, A.d, A.s, A.nopen, B.d, B.s, B.nopen, a.nbf, a.one, a.two);
-    else
-      pair_setup(P, J, (int)I, B.d, B.s, B.nopen, A.d, A.s, A.nopen, a.nbf, a.one, a.two);
+    bool rowI;
+    int J;
+    unpack_rec(r, P, J, rowI);
     if (P.kind == 0) continue;
+    const ConfRec rj = a.conf[J];
+    const ConfView B = view_of(rj);
     PairX px;
-    if (J != I) pair_x(P, rowI ? A : B, a.two, px);
-    const int nJ = a.nsef[J];
+    if (P.kind == 2) {
+      const ConfView& R = rowI ? A : B;
+      const double* __restrict__ xrow = a.xt + ((int64_t)P.e0 * a.nbf + P.f0) * a.nbf;
+      uint64_t rem = R.s;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        double v = 0.0;
+        if (rem) {
+          const int o = ctz(rem);
+          rem &= rem - 1;
+          v = xrow[o];
+        }
+        px.X[u] = v;
+      }
+    }
+    const int nJ = rj.nsef;
     const int ndJ = B.nopen ? 2 * nJ : 1;
-    const int64_t dJ0 = a.detoff[J];
-    const T* __restrict__ xJ = X + a.sefoff[J] * (int64_t)ldx;
+    const T* __restrict__ xJ = X + (int64_t)rj.sefoff * ldx;
     // stage J's determinant masks and x rows in LDS with coalesced loads; LDS
-    // operations of one wavefront complete in order, and the fence keeps the
-    // compiler from moving the reads above the writes
+    // operations of one wavefront complete in order, and the fences keep the
+    // compiler from moving reads and writes across each other
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    for (int m = lane; m < ndJ; m += kWave) sm_[m] = a.detao[dJ0 + m];
+    for (int m = lane; m < ndJ; m += kWave) smk[m] = a.detao[rj.detoff + m];
     for (int e = lane; e < nJ * K; e += kWave) {
-      const int r = e / K, v = e - r * K;
-      sx[e] = xJ[(int64_t)r * ldx + v];
+      const int row = e / K, v = e - row * K;
+      sx[e] = xJ[(int64_t)row * ldx + v];
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     for (int j = c; j < nJ; j += g) {
-      const uint64_t aJ0 = sm_[B.nopen ? 2 * j : 0];
+      const uint64_t aJ0 = smk[B.nopen ? 2 * j : 0];
       if (abs(2 * popc(aJ0) - B.nopen) != msI) continue;
-      const uint64_t aJ1 = B.nopen ? sm_[2 * j + 1] : 0ull;
+      const uint64_t aJ1 = B.nopen ? smk[2 * j + 1] : 0ull;
       // exact zero test: every determinant pair of the element is det 2i (or
       // its spin flip) against det 2j or 2j+1 (or their flips); a pair is
       // nonzero only with equal alpha counts and at most two moved
@@ -230,14 +282,13 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
           near = popc(ha) == na && popc(ha & ~fa) + popc(hb & ~fb) <= 2;
         }
         if (!near && !B.nopen && A.nopen) {
-          // closed-shell column: test det 2i+1 as well
           const uint64_t ea = A.d | aI1, eb = A.d | (A.s & ~aI1);
           near = popc(ga) == popc(ea) && popc(ga & ~ea) + popc(gb & ~eb) <= 2;
         }
         if (!near) continue;
       }
       double h;
-      if (J == I)
+      if (P.kind == 1)
         h = sef_elem_ao(P, A, B, aI0, aI1, aJ0, aJ1, a.mult, a.mult, a.two);
       else if (rowI)
         h = sef_elem_x(P, px, A, B, aI0, aI1, aJ0, aJ1, a.mult, a.mult, a.two);
@@ -256,7 +307,7 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
     for (int v = 0; v < K; ++v) acc[v] += __shfl_xor(acc[v], off, kWave);
   }
   if (c == 0) {
-    T* y = Y + (a.sefoff[I] + i) * (int64_t)ldy;
+    T* y = Y + ((int64_t)ri.sefoff + i) * (int64_t)ldy;
 #pragma unroll
     for (int v = 0; v < K; ++v) y[v] = acc[v];
   }
@@ -265,19 +316,13 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
 DirectArgs direct_args(Ctx& c) {
   DirectArgs a;
   a.ncfg = c.sp.ncfg;
-  a.adjoff = c.d_adjoff.p;
-  a.adj = c.d_adj.p;
-  a.dm = c.d_dm.p;
-  a.sm = c.d_sm.p;
-  a.nopen = c.d_nopen.p;
-  a.cph = c.d_cph.p;
-  a.nsef = c.d_nsef.p;
-  a.sefoff = c.d_sefoff.p;
-  a.detoff = c.d_detoff.p;
+  a.recoff = c.d_adjoff.p;
+  a.rec = c.d_rec.p;
+  a.conf = c.d_conf.p;
   a.detao = c.d_detao.p;
+  a.xt = c.d_xt.p;
   a.nbf = c.sp.nbf;
   a.mult = c.sp.mult;
-  a.one = c.d_one.p;
   a.two = c.d_two.p;
   return a;
 }
@@ -321,13 +366,25 @@ void build_pairs(Ctx& c) {
   const Space& sp = c.sp;
   PSOCI_CHECK(sp.max_open <= 8, PSOCI_ERR_UNSUPPORTED,
               "direct sigma supports at most 8 open shells (64 SEFs per configuration)");
+  PSOCI_CHECK(sp.nbf <= 63, PSOCI_ERR_UNSUPPORTED, "direct sigma supports at most 63 orbitals");
+  PSOCI_CHECK(sp.ndet_total < ((int64_t)1 << 31), PSOCI_ERR_UNSUPPORTED, "more than 2^31 determinants");
   const int64_t ncfg = sp.ncfg;
   c.d_detoff.upload(sp.det_off, c.stream);
   c.d_detao.alloc(sp.ndet_total);
   hipLaunchKernelGGL(k_detao, dim3(nblk(ncfg, 4)), dim3(256), 0, c.stream, ncfg, c.d_detoff.p, c.d_sm.p, c.d_nopen.p,
                      c.d_pat.p, c.d_patoff.p, c.d_detao.p);
   PSOCI_HIP(hipGetLastError());
-  DevBuf<int32_t> cnt;
+  c.d_conf.alloc(ncfg);
+  hipLaunchKernelGGL(k_confrec, dim3(nblk(ncfg, 256)), dim3(256), 0, c.stream, ncfg, c.d_dm.p, c.d_sm.p, c.d_nopen.p,
+                     c.d_cph.p, c.d_nsef.p, c.d_sefoff.p, c.d_detoff.p, c.d_conf.p);
+  PSOCI_HIP(hipGetLastError());
+  const int nbf = sp.nbf;
+  const int64_t n3 = (int64_t)nbf * nbf * nbf;
+  c.d_xt.alloc(n3);
+  hipLaunchKernelGGL(k_xt, dim3(nblk(n3, 256)), dim3(256), 0, c.stream, nbf, c.d_two.p, c.d_xt.p);
+  PSOCI_HIP(hipGetLastError());
+  // adjacency: count, scan, fill
+  DevBuf<int32_t> cnt, adj;
   DevBuf<int64_t> cnt64;
   cnt.alloc(ncfg);
   cnt64.alloc(ncfg + 1);
@@ -342,15 +399,17 @@ void build_pairs(Ctx& c) {
   int64_t total = 0;
   PSOCI_HIP(hipMemcpyAsync(&total, c.d_adjoff.p + ncfg, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
   c.sync();
-  c.d_adj.alloc(total);
+  adj.alloc(total);
   hipLaunchKernelGGL(k_adj<true>, dim3(grid), dim3(256), 0, c.stream, ncfg, c.d_dm.p, c.d_sm.p, c.d_nopen.p, nullptr,
-                     c.d_adjoff.p, c.d_adj.p);
+                     c.d_adjoff.p, adj.p);
   PSOCI_HIP(hipGetLastError());
+  // pair records (the adjacency is folded into them and released)
+  c.d_rec.alloc(total);
   DevBuf<unsigned long long> ne;
   ne.alloc(1);
   PSOCI_HIP(hipMemsetAsync(ne.p, 0, sizeof(unsigned long long), c.stream));
-  hipLaunchKernelGGL(k_count_elems, dim3(nblk(ncfg, 4)), dim3(256), 0, c.stream, ncfg, c.d_adjoff.p, c.d_adj.p,
-                     c.d_nsef.p, ne.p);
+  hipLaunchKernelGGL(k_records, dim3(nblk(ncfg, 4)), dim3(256), 0, c.stream, ncfg, c.d_adjoff.p, adj.p, c.d_dm.p,
+                     c.d_sm.p, c.d_nopen.p, c.d_nsef.p, nbf, c.d_one.p, c.d_two.p, c.d_rec.p, ne.p);
   PSOCI_HIP(hipGetLastError());
   unsigned long long nelems = 0;
   ne.download(&nelems, 1, c.stream);

@@ -35,6 +35,27 @@ struct PairInfo {
   double v0, v1;  // vnt[0], vnt[1]
 };
 
+// Packed configuration (32 bytes: one load per configuration).
+struct ConfRec {
+  uint64_t d, s;
+  int32_t sefoff;
+  int32_t detoff;
+  int8_t nopen;
+  uint8_t cph;
+  int16_t nsef;
+  int32_t pad;
+};
+
+// One ordered configuration pair (I, J) of the direct sigma: J, and the
+// reference-orientation block data (row = the later configuration): kind,
+// excitation orbitals, vnt[0], vnt[1].  meta bits: kind 0-1, row-is-I 2,
+// e0 3-8, e1 9-14, f0 15-20, f1 21-26 (63 = none).
+struct PairRec {
+  int32_t J;
+  uint32_t meta;
+  double v0, v1;
+};
+
 struct ConfView {
   uint64_t d, s;
   int nopen;
@@ -186,6 +207,30 @@ PSOCI_HD void pair_setup(PairInfo& P, int I, int J, uint64_t dI, uint64_t sI, in
     return;
   }
   // identical occupations under two indices: the reference computes no block
+}
+
+PSOCI_HD PairRec make_rec(const PairInfo& P, int J, bool rowI) {
+  auto o6 = [](int v) { return (uint32_t)(v < 0 ? 63 : v); };
+  PairRec r;
+  r.J = J;
+  r.meta = (uint32_t)P.kind | ((rowI ? 1u : 0u) << 2) | (o6(P.e0) << 3) | (o6(P.e1) << 9) | (o6(P.f0) << 15) |
+           (o6(P.f1) << 21);
+  r.v0 = P.v0;
+  r.v1 = P.v1;
+  return r;
+}
+
+PSOCI_HD void unpack_rec(const PairRec& r, PairInfo& P, int& J, bool& rowI) {
+  auto u6 = [](uint32_t v) { return (int8_t)(v == 63 ? -1 : (int)v); };
+  J = r.J;
+  P.kind = (int8_t)(r.meta & 3u);
+  rowI = (r.meta >> 2) & 1u;
+  P.e0 = u6((r.meta >> 3) & 63u);
+  P.e1 = u6((r.meta >> 9) & 63u);
+  P.f0 = u6((r.meta >> 15) & 63u);
+  P.f1 = u6((r.meta >> 21) & 63u);
+  P.v0 = r.v0;
+  P.v1 = r.v1;
 }
 
 // One determinant-pair element (generateSEFCIG, PsociHamiltonian.C:860-1118).
