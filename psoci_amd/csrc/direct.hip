@@ -427,14 +427,4 @@ void sigma_direct(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int 
     direct_any_k<double>(c, k, static_cast<const double*>(X), ldx, static_cast<double*>(Y), ldy);
 }
 
-void rdm_grad(Ctx& c, int k, const double* dC, const double* w, double* d_g1, double* d_g2) {
-  (void)c;
-  (void)k;
-  (void)dC;
-  (void)w;
-  (void)d_g1;
-  (void)d_g2;
-  throw Error(PSOCI_ERR_UNSUPPORTED, "the adjoint is not available in this build");
-}
-
 }  // namespace psoci
