@@ -82,6 +82,9 @@ struct Space {
   int64_t nsef_total = 0, ndet_total = 0;
   std::vector<uint16_t> pat;  // determinant patterns, all open-shell counts
   int pat_off[12] = {0};      // pat_off[s]: first pattern of s open shells
+  // didx[(1 << s) - 1 + p]: index of the determinant with alpha pattern p
+  // among those of a configuration with s open shells (-1: not in the class)
+  std::vector<int16_t> didx;
   int max_open = 0;
 };
 

@@ -45,10 +45,12 @@ struct Ctx {
   int64_t direct_elems = 0;  // structural SEF elements per direct sigma
   DevBuf<int64_t> d_adjoff;   // ncfg + 1: first record of each configuration
   DevBuf<PairRec> d_rec;      // one record per ordered connected pair
+  DevBuf<double> d_cdiag;     // ncfg: vnt[0] of each diagonal block
   DevBuf<ConfRec> d_conf;     // packed configurations
   DevBuf<double> d_xt;        // XT[e][f][o] = (e o|o f)
   DevBuf<int64_t> d_detoff;   // ncfg
   DevBuf<uint64_t> d_detao;   // ndet_total
+  DevBuf<int16_t> d_didx;     // determinant index by open-shell pattern (Space::didx)
 
   // scratch
   void* scan_tmp = nullptr;

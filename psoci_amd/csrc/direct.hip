@@ -17,15 +17,16 @@
 //    supplies the single-excitation exchange integrals of any pair;
 //  * one wavefront owns configuration I and walks its records, so y_I is
 //    reduced in registers and written once: no atomics, a deterministic
-//    result, X rows only gathered.  All lanes work on the same pair: lanes
-//    are (SEF row i, column slice c) with nsef_I * g = 64, and the column
-//    configuration's determinant masks and x rows are staged in LDS with
-//    coalesced loads.
-// Each block is evaluated in the reference's own orientation, so every
-// element is bit-identical to the stored Hamiltonian's.  SEF pairs whose
-// |Ms| classes differ, and those whose determinants are more than two
-// spin-orbitals apart, are skipped: their elements are exactly zero.
+//    result, X rows only gathered.  Lanes are (record slot, SEF i of I):
+//    64 / nsef_I records are in flight per step, and each lane takes
+//    determinant 2i of I and visits only the determinants of J that it
+//    connects to (detdrive.hpp), so no lane tests SEF pairs whose elements
+//    vanish (about 94% of the structural block entries of an MR-CISD space).
+// Elements are the reference's determinant-pair values (detval_ao/detval_x,
+// reference orientation) combined into SEF elements as D(2i,2j) +
+// mult D(2i,2j+1), equal to transformCDCt's to rounding.
 #include "ctx.hpp"
+#include "detdrive.hpp"
 
 namespace psoci {
 
@@ -148,7 +149,7 @@ __global__ void k_records(int64_t ncfg, const int64_t* __restrict__ adjoff, cons
                           const uint64_t* __restrict__ dm, const uint64_t* __restrict__ sm,
                           const int8_t* __restrict__ nopen, const int32_t* __restrict__ nsef, int nbf,
                           const double* __restrict__ one, const double* __restrict__ two, PairRec* __restrict__ rec,
-                          unsigned long long* __restrict__ nelems) {
+                          double* __restrict__ cdiag, unsigned long long* __restrict__ nelems) {
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (I >= ncfg) return;
@@ -160,6 +161,7 @@ __global__ void k_records(int64_t ncfg, const int64_t* __restrict__ adjoff, cons
     PairInfo P;
     pair_setup(P, R, C, dm[R], sm[R], nopen[R], dm[C], sm[C], nopen[C], nbf, one, two);
     rec[t] = make_rec(P, J, rowI);
+    if (J == (int)I) cdiag[I] = P.v0;
     ne += (unsigned long long)nsef[J];
   }
   for (int off = kWave / 2; off > 0; off >>= 1) ne += __shfl_xor(ne, off, kWave);
@@ -187,6 +189,9 @@ struct DirectArgs {
   int nbf;
   double mult;
   const double* two;
+  const int16_t* didx;
+  int ndidx;
+  const double* cdiag;
 };
 
 __device__ __forceinline__ ConfView view_of(const ConfRec& r) {
@@ -199,48 +204,97 @@ __device__ __forceinline__ ConfView view_of(const ConfRec& r) {
   return v;
 }
 
-template <typename T, int K>
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  typedef f32x4 type;
+};
+template <>
+struct Vec16<double> {
+  typedef f64x2 type;
+};
+
+// acc += hv * x[0..K): 16-byte loads when the rows are aligned
+template <typename T, int K, bool VEC>
+__device__ __forceinline__ void axpy_row(T hv, const T* __restrict__ x, T (&acc)[K]) {
+  constexpr int W = 16 / (int)sizeof(T);
+  if constexpr (VEC && K % W == 0) {
+    typedef typename Vec16<T>::type V;
+    const V* __restrict__ xv = reinterpret_cast<const V*>(x);
+#pragma unroll
+    for (int q = 0; q < K / W; ++q) {
+      const V u = xv[q];
+#pragma unroll
+      for (int w = 0; w < W; ++w) acc[q * W + w] = fmadd(hv, u[w], acc[q * W + w]);
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < K; ++v) acc[v] = fmadd(hv, x[v], acc[v]);
+  }
+}
+
+// One wavefront per row configuration I; lanes are (record slot, SEF i of I)
+// with nsef_I * nslot = 64, so 64 / nsef_I records are in flight per step.
+// Each lane takes determinant 2i of I and, per record, enumerates the
+// determinants of J it connects to (for_each_conn), adding
+// coef * D(2i, b) * x_J[b / 2].  Lanes of one i are summed across slots at
+// the end; the order is fixed, so results are deterministic.
+template <typename T, int K, bool VEC>
 __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restrict__ X, int ldx, T* __restrict__ Y,
                                                 int ldy) {
-  // per wavefront: the current pair's column determinants and x rows
-  __shared__ uint64_t s_mask[4][128];
-  __shared__ T s_x[4][64 * K];
+  __shared__ int16_t s_didx[2047];
+  for (int t = threadIdx.x; t < a.ndidx; t += blockDim.x) s_didx[t] = a.didx[t];
+  __syncthreads();
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + wave;
   if (I >= a.ncfg) return;  // uniform per wavefront; no block-wide barrier below
-  uint64_t* __restrict__ smk = s_mask[wave];
-  T* __restrict__ sx = s_x[wave];
   const ConfRec ri = a.conf[I];
   const ConfView A = view_of(ri);
   const int nI = ri.nsef;
-  // lanes = (row i, column slice c): every lane works on the same pair
-  const int g = kWave / nI;
-  const int i = lane / g, c = lane - i * g;
-  const uint64_t aI0 = a.detao[ri.detoff + (A.nopen ? 2 * i : 0)];
-  const uint64_t aI1 = A.nopen ? a.detao[ri.detoff + 2 * i + 1] : 0ull;
-  const int msI = abs(2 * popc(aI0) - A.nopen);
-  // full alpha / beta orbital sets of determinant 2i (or the only one)
-  const uint64_t fa = A.d | aI0, fb = A.d | (A.s & ~aI0);
-  const int na = popc(fa);
+  const int nslot = kWave / nI;
+  const int slot = lane / nI, i = lane - slot * nI;
+  const uint64_t aa = A.nopen ? a.detao[ri.detoff + 2 * i] : 0ull;
   T acc[K];
 #pragma unroll
   for (int v = 0; v < K; ++v) acc[v] = (T)0;
+  if (slot == 0) {
+    // the diagonal block (J = I): determinant 2i itself and its alpha-beta
+    // spin exchanges, from the configuration's vnt[0] and the integral table
+    PairInfo P;
+    P.I = P.J = (int32_t)I;
+    P.kind = 1;
+    P.e0 = P.e1 = P.f0 = P.f1 = -1;
+    P.v0 = a.cdiag[I];
+    P.v1 = 0.0;
+    const PairCoef pc = pair_coef(A.nopen, A.nopen, a.mult);
+    const T* __restrict__ xI = X + (int64_t)ri.sefoff * ldx;
+    for_each_conn(A, A, aa, s_didx + (1 << A.nopen) - 1, [&](int b, uint64_t ab) {
+      const double v = detval_ao(P, A, A, aa, ab, a.two);
+      if (v == 0.0) return;
+      const T hv = (T)(((b & 1) ? pc.co : pc.ce) * v);
+      axpy_row<T, K, VEC>(hv, xI + (int64_t)(b >> 1) * ldx, acc);
+    });
+  }
   const int64_t t1 = a.recoff[I + 1];
-  for (int64_t t = a.recoff[I]; t < t1; ++t) {
-    const PairRec r = a.rec[t];
+  for (int64_t t0 = a.recoff[I]; t0 < t1; t0 += nslot) {
+    const int64_t t = t0 + slot;
+    if (t >= t1) continue;
     PairInfo P;
     bool rowI;
     int J;
-    unpack_rec(r, P, J, rowI);
-    if (P.kind == 0) continue;
+    unpack_rec(a.rec[t], P, J, rowI);
+    if (P.kind < 2) continue;  // skipped by the reference, or the diagonal block (done above)
     const ConfRec rj = a.conf[J];
     const ConfView B = view_of(rj);
     PairX px;
     if (P.kind == 2) {
-      const ConfView& R = rowI ? A : B;
+      // (e0 o|o f0) for the open shells o of the reference-orientation row
       const double* __restrict__ xrow = a.xt + ((int64_t)P.e0 * a.nbf + P.f0) * a.nbf;
-      uint64_t rem = R.s;
+      uint64_t rem = rowI ? A.s : B.s;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         double v = 0.0;
@@ -252,61 +306,20 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
         px.X[u] = v;
       }
     }
-    const int nJ = rj.nsef;
-    const int ndJ = B.nopen ? 2 * nJ : 1;
+    const PairCoef pc = pair_coef(A.nopen, B.nopen, a.mult);
     const T* __restrict__ xJ = X + (int64_t)rj.sefoff * ldx;
-    // stage J's determinant masks and x rows in LDS with coalesced loads; LDS
-    // operations of one wavefront complete in order, and the fences keep the
-    // compiler from moving reads and writes across each other
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    for (int m = lane; m < ndJ; m += kWave) smk[m] = a.detao[rj.detoff + m];
-    for (int e = lane; e < nJ * K; e += kWave) {
-      const int row = e / K, v = e - row * K;
-      sx[e] = xJ[(int64_t)row * ldx + v];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    for (int j = c; j < nJ; j += g) {
-      const uint64_t aJ0 = smk[B.nopen ? 2 * j : 0];
-      if (abs(2 * popc(aJ0) - B.nopen) != msI) continue;
-      const uint64_t aJ1 = B.nopen ? smk[2 * j + 1] : 0ull;
-      // exact zero test: every determinant pair of the element is det 2i (or
-      // its spin flip) against det 2j or 2j+1 (or their flips); a pair is
-      // nonzero only with equal alpha counts and at most two moved
-      // spin-orbitals
-      {
-        const uint64_t ga = B.d | aJ0, gb = B.d | (B.s & ~aJ0);
-        bool near = popc(ga) == na && popc(ga & ~fa) + popc(gb & ~fb) <= 2;
-        if (!near && B.nopen) {
-          const uint64_t ha = B.d | aJ1, hb = B.d | (B.s & ~aJ1);
-          near = popc(ha) == na && popc(ha & ~fa) + popc(hb & ~fb) <= 2;
-        }
-        if (!near && !B.nopen && A.nopen) {
-          const uint64_t ea = A.d | aI1, eb = A.d | (A.s & ~aI1);
-          near = popc(ga) == popc(ea) && popc(ga & ~ea) + popc(gb & ~eb) <= 2;
-        }
-        if (!near) continue;
-      }
-      double h;
-      if (P.kind == 1)
-        h = sef_elem_ao(P, A, B, aI0, aI1, aJ0, aJ1, a.mult, a.mult, a.two);
-      else if (rowI)
-        h = sef_elem_x(P, px, A, B, aI0, aI1, aJ0, aJ1, a.mult, a.mult, a.two);
-      else
-        h = sef_elem_x(P, px, B, A, aJ0, aJ1, aI0, aI1, a.mult, a.mult, a.two);
-      if (h == 0.0) continue;
-      const T hv = (T)h;
-      const T* x = sx + j * K;
-#pragma unroll
-      for (int v = 0; v < K; ++v) acc[v] = fmadd(hv, x[v], acc[v]);
-    }
-    __builtin_amdgcn_wave_barrier();
+    for_each_conn(A, B, aa, s_didx + (1 << B.nopen) - 1, [&](int b, uint64_t ab) {
+      const double v = conn_value(P, px, rowI, A, B, aa, ab, a.two);
+      if (v == 0.0) return;
+      const T hv = (T)(((b & 1) ? pc.co : pc.ce) * v);
+      axpy_row<T, K, VEC>(hv, xJ + (int64_t)(b >> 1) * ldx, acc);
+    });
   }
-  for (int off = g >> 1; off > 0; off >>= 1) {
+  for (int off = nI; off < kWave; off <<= 1) {
 #pragma unroll
     for (int v = 0; v < K; ++v) acc[v] += __shfl_xor(acc[v], off, kWave);
   }
-  if (c == 0) {
+  if (slot == 0) {
     T* y = Y + ((int64_t)ri.sefoff + i) * (int64_t)ldy;
 #pragma unroll
     for (int v = 0; v < K; ++v) y[v] = acc[v];
@@ -324,13 +337,20 @@ DirectArgs direct_args(Ctx& c) {
   a.nbf = c.sp.nbf;
   a.mult = c.sp.mult;
   a.two = c.d_two.p;
+  a.didx = c.d_didx.p;
+  a.ndidx = (2 << c.sp.max_open) - 1;
+  a.cdiag = c.d_cdiag.p;
   return a;
 }
 
 template <typename T, int K>
 void launch_direct(Ctx& c, const T* X, int ldx, T* Y, int ldy) {
   DirectArgs a = direct_args(c);
-  hipLaunchKernelGGL((k_direct<T, K>), dim3(nblk(a.ncfg, 4)), dim3(256), 0, c.stream, a, X, ldx, Y, ldy);
+  const bool vec = ((uintptr_t)X % 16 == 0) && ((size_t)ldx * sizeof(T)) % 16 == 0;
+  if (vec)
+    hipLaunchKernelGGL((k_direct<T, K, true>), dim3(nblk(a.ncfg, 4)), dim3(256), 0, c.stream, a, X, ldx, Y, ldy);
+  else
+    hipLaunchKernelGGL((k_direct<T, K, false>), dim3(nblk(a.ncfg, 4)), dim3(256), 0, c.stream, a, X, ldx, Y, ldy);
   PSOCI_HIP(hipGetLastError());
 }
 
@@ -405,11 +425,12 @@ void build_pairs(Ctx& c) {
   PSOCI_HIP(hipGetLastError());
   // pair records (the adjacency is folded into them and released)
   c.d_rec.alloc(total);
+  c.d_cdiag.alloc(ncfg);
   DevBuf<unsigned long long> ne;
   ne.alloc(1);
   PSOCI_HIP(hipMemsetAsync(ne.p, 0, sizeof(unsigned long long), c.stream));
   hipLaunchKernelGGL(k_records, dim3(nblk(ncfg, 4)), dim3(256), 0, c.stream, ncfg, c.d_adjoff.p, adj.p, c.d_dm.p,
-                     c.d_sm.p, c.d_nopen.p, c.d_nsef.p, nbf, c.d_one.p, c.d_two.p, c.d_rec.p, ne.p);
+                     c.d_sm.p, c.d_nopen.p, c.d_nsef.p, nbf, c.d_one.p, c.d_two.p, c.d_rec.p, c.d_cdiag.p, ne.p);
   PSOCI_HIP(hipGetLastError());
   unsigned long long nelems = 0;
   ne.download(&nelems, 1, c.stream);

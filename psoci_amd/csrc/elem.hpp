@@ -362,7 +362,10 @@ PSOCI_HD double pick8(const PairX& px, int t) {
   return v;
 }
 
-// detval_ao() for off-diagonal pairs (kind 2 or 3) without integral loads
+// detval_ao() for off-diagonal pairs (kind 2 or 3) without integral loads.
+// SAME_OK = false drops the identical-determinant branch, which a pair of
+// distinct configurations never reaches.
+template <bool SAME_OK = true>
 PSOCI_HD double detval_x(const PairInfo& P, const PairX& px, const ConfView& A, const ConfView& B, uint64_t ao,
                          uint64_t bo, const double* __restrict__ two) {
   uint64_t aI = A.d | ao, bI = A.d | (A.s & ~ao);
@@ -371,7 +374,10 @@ PSOCI_HD double detval_x(const PairInfo& P, const PairX& px, const ConfView& A, 
   uint64_t xa = aJ & ~aI, xb = bJ & ~bI;
   int ma = popc(xa), mb = popc(xb);
   int tot = ma + mb;
-  if (tot > 2 || tot == 0) return tot == 0 ? detval_ao(P, A, B, ao, bo, two) : 0.0;
+  if (tot > 2 || tot == 0) {
+    if constexpr (SAME_OK) return tot == 0 ? detval_ao(P, A, B, ao, bo, two) : 0.0;
+    return 0.0;
+  }
   int cnt = A.ph + B.ph;
   double v;
   if (tot == 1) {

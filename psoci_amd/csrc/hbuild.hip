@@ -333,6 +333,7 @@ void upload_space(Ctx& c) {
   c.d_pat.upload(sp.pat, c.stream);
   std::vector<int32_t> po(sp.pat_off, sp.pat_off + 12);
   c.d_patoff.upload(po, c.stream);
+  c.d_didx.upload(sp.didx, c.stream);
   c.sync();
 }
 

@@ -71,6 +71,11 @@ void build_space(Space& sp, int nbf, int nelec, int ksym, int64_t ncfg, const ui
     patterns_for(s, sp.pat);
   }
   sp.pat_off[11] = (int)sp.pat.size();
+  sp.didx.assign(2047, (int16_t)-1);
+  for (int s = 0; s <= 10; s += 2) {
+    const int nd = 1 << (s > 0 ? s - 1 : 0);
+    for (int d = 0; d < nd; ++d) sp.didx[(1 << s) - 1 + sp.pat[sp.pat_off[s] + d]] = (int16_t)d;
+  }
   sp.dm.resize(ncfg);
   sp.sm.resize(ncfg);
   sp.nopen.resize(ncfg);

@@ -78,3 +78,57 @@ int hc_connected(void* h, int64_t I, int64_t J) {
 }
 
 }
+
+#include "../../psoci_amd/csrc/detdrive.hpp"
+
+extern "C" {
+
+// sigma row block y_I = sum_J H_IJ x_J (k columns, row-major) two ways:
+// mode 0 SEF pair by SEF pair with sef_elem (reference orientation), mode 1
+// determinant-driven (for_each_conn); every J within two moved electrons
+int hc_row(void* h, const double* one, const double* two, int64_t I, int k, const double* X, double* y, int mode) {
+  const Space& sp = *static_cast<Space*>(h);
+  const ConfView A = view(sp, I);
+  const int ni = sp.nsef[I];
+  for (int t = 0; t < ni * k; ++t) y[t] = 0.0;
+  for (int64_t J = 0; J < sp.ncfg; ++J) {
+    int dn = sp.nopen[I] - sp.nopen[J];
+    if (dn > 4 || dn < -4) continue;
+    if (moved(sp.dm[I], sp.sm[I], sp.dm[J], sp.sm[J]) > 2) continue;
+    const bool rowI = I >= J;
+    const int64_t R = rowI ? I : J, Cc = rowI ? J : I;
+    PairInfo P;
+    pair_setup(P, (int)R, (int)Cc, sp.dm[R], sp.sm[R], sp.nopen[R], sp.dm[Cc], sp.sm[Cc], sp.nopen[Cc], sp.nbf, one,
+               two);
+    if (!P.kind) continue;
+    const ConfView B = view(sp, J);
+    const int nj = sp.nsef[J];
+    const double* xJ = X + sp.sef_off[J] * k;
+    if (mode == 0) {
+      const ConfView RA = view(sp, R), CB = view(sp, Cc);
+      for (int i = 0; i < ni; ++i)
+        for (int j = 0; j < nj; ++j) {
+          double v = rowI ? sef_elem(P, RA, CB, i, j, sp.mult, sp.mult, two) : sef_elem(P, RA, CB, j, i, sp.mult, sp.mult, two);
+          for (int c = 0; c < k; ++c) y[i * k + c] += v * xJ[j * k + c];
+        }
+    } else {
+      PairX px;
+      pair_x(P, rowI ? A : B, two, px);
+      const PairCoef pc = pair_coef(A.nopen, B.nopen, sp.mult);
+      const int16_t* dj = sp.didx.data() + (1 << B.nopen) - 1;
+      for (int i = 0; i < ni; ++i) {
+        const uint64_t aa = A.nopen ? deposit(A.pat[2 * i], A.s) : 0ull;
+        for_each_conn(A, B, aa, dj, [&](int b, uint64_t ab) {
+          if (b < 0) throw Error(PSOCI_ERR_NUMERIC, "determinant outside the class");
+          const double v = P.kind == 1 ? detval_ao(P, A, A, aa, ab, two) : conn_value(P, px, rowI, A, B, aa, ab, two);
+          const double cf = (b & 1) ? pc.co : pc.ce;
+          const double hv = cf * v;
+          for (int c = 0; c < k; ++c) y[i * k + c] += hv * xJ[(b >> 1) * k + c];
+        });
+      }
+    }
+  }
+  return 0;
+}
+
+}

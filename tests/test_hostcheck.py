@@ -20,7 +20,7 @@ OUT = os.path.join(ROOT, "tests", "hostcheck", "_build", "libhostcheck.so")
 @pytest.fixture(scope="module")
 def hc():
     deps = [SRC, os.path.join(ROOT, "psoci_amd", "csrc", "space.cpp"), os.path.join(ROOT, "psoci_amd", "csrc", "elem.hpp"),
-            os.path.join(ROOT, "psoci_amd", "csrc", "common.hpp")]
+            os.path.join(ROOT, "psoci_amd", "csrc", "common.hpp"), os.path.join(ROOT, "psoci_amd", "csrc", "detdrive.hpp")]
     if not os.path.exists(OUT) or os.path.getmtime(OUT) < max(os.path.getmtime(d) for d in deps):
         os.makedirs(os.path.dirname(OUT), exist_ok=True)
         if not os.path.exists("/opt/rocm/bin/hipcc"):
@@ -36,6 +36,7 @@ def hc():
     L.hc_block.argtypes = [P, P, P, ctypes.c_int64, ctypes.c_int64, P]
     L.hc_block_x.argtypes = [P, P, P, ctypes.c_int64, ctypes.c_int64, P]
     L.hc_connected.argtypes = [P, ctypes.c_int64, ctypes.c_int64]
+    L.hc_row.argtypes = [P, P, P, ctypes.c_int64, ctypes.c_int, P, P, ctypes.c_int]
     return L
 
 
@@ -92,3 +93,48 @@ def test_mrci_sampled_blocks_bit_identical(hc):
     rng = np.random.default_rng(3)
     rows = rng.choice(cfg.ncfg, 25, replace=False)
     assert _compare(hc, cfg, ints, rows) > 100
+
+
+def _rows_agree(L, cfg, ints, rows, k=3):
+    """Determinant-driven sigma rows (direct kernel's enumeration) against SEF
+    pair-by-pair rows built from the bit-identical elements above."""
+    h = L.hc_space_new(cfg.nbf, cfg.nelec, cfg.ksym, cfg.ncfg, _p(cfg.occ), _p(cfg.jsym))
+    assert h
+    try:
+        sp = oracle.OracleSpace(cfg, ints)
+        X = np.random.default_rng(5).standard_normal((sp.n, k))
+        for I in rows:
+            I = int(I)
+            ni = int(sp.nsef[I])
+            y0 = np.zeros((ni, k))
+            y1 = np.zeros((ni, k))
+            assert L.hc_row(h, _p(ints.valone), _p(ints.valtwo), I, k, _p(X), _p(y0), 0) == 0
+            assert L.hc_row(h, _p(ints.valone), _p(ints.valtwo), I, k, _p(X), _p(y1), 1) == 0
+            scale = max(1.0, np.abs(y0).max())
+            assert np.abs(y1 - y0).max() <= 1e-13 * scale, (I, np.abs(y1 - y0).max())
+    finally:
+        L.hc_space_free(h)
+
+
+def test_determinant_driven_rows_fci6(hc, fci6):
+    cfg, ints, _ = fci6
+    _rows_agree(hc, cfg, ints, range(cfg.ncfg))
+
+
+def test_determinant_driven_rows_opposite_sef_sign(hc, fci6):
+    """SEFs (D_2k - D_2k+1)/sqrt(2): configuration and state symmetry of
+    opposite parity (ksym 2 over jsym 1)."""
+    cfg, ints, _ = fci6
+    cfg = inputs.fci_configs(cfg.nbf, cfg.nelec, jsym=1, ksym=2)
+    _rows_agree(hc, cfg, ints, range(cfg.ncfg))
+
+
+def test_determinant_driven_rows_fci10_mrci(hc, fci10):
+    cfg, ints, _ = fci10
+    nopen = (cfg.occ == 1).sum(1)
+    rng = np.random.default_rng(12)
+    rows = list(rng.choice(cfg.ncfg, 25, replace=False)) + list(np.where(nopen == 8)[0][:3]) + [0, cfg.ncfg - 1]
+    _rows_agree(hc, cfg, ints, rows)
+    cfg = inputs.mrcisd_configs(4, 6, 4)
+    ints = inputs.make_integrals(10, 99, n_internal=4)
+    _rows_agree(hc, cfg, ints, np.random.default_rng(4).choice(cfg.ncfg, 30, replace=False))
