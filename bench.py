@@ -8,7 +8,7 @@ run once beside it; every root of each timed solve must agree with it to
 1e-8 Eh or the run fails.
 
 Prints ONE JSON line (rank 0, N = 1).  Usage:
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload fci10]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mrci10x24]
 """
 from __future__ import annotations
 
@@ -31,9 +31,9 @@ METRIC = "time to 16 roots at 1e-8 Eh, FP32-sigma vs FP64 on GPU; sigma HBM GB/s
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="fci10")
+    ap.add_argument("--steps", type=int, default=None, help="timed mixed solves (default 1 direct, 3 stored)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed mixed solves (default 0 direct, 1 stored)")
+    ap.add_argument("--workload", default="mrci10x24")
     ap.add_argument("--roots", type=int, default=16)
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -71,6 +71,22 @@ def time_sigma(H, api, torch, k, prec, reps, mode, warm=3):
     e1.record(stream)
     e1.synchronize()
     return e0.elapsed_time(e1) / reps  # ms
+
+
+def measured_traffic(workload, k):
+    """HBM bytes per launch of the dominant kernel on this workload, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in profiles/ (each
+    scaled by the same counter's reading for a copy kernel of known size)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    rec = table.get(f"{workload}/k{k}")
+    if not rec:
+        return None
+    return {"bytes_per_launch": int(rec["bytes_per_launch"]), "source": rec["source"]}
 
 
 def cpu_stored(args, cfg, ints, k, ref):
@@ -131,6 +147,10 @@ def main():
 
     cfg, ints, _ = inputs.workload(args.workload)
     direct = inputs.WORKLOADS[args.workload]["kind"] != "fci"
+    if args.steps is None:
+        args.steps = 1 if direct else 3
+    if args.warmup is None:
+        args.warmup = 0 if direct else 1
     mode = api.DIRECT if direct else api.STORED
     k = args.roots
     n_sef = int(cfg.nsef().sum())
@@ -177,7 +197,7 @@ def main():
 
     # dominant kernel: the FP32 sigma over the k-vector block
     n = H.n
-    reps = args.sigma_reps if not direct else 1
+    reps = args.sigma_reps if not direct else 2
     warm = 3 if not direct else 1
     ms32 = time_sigma(H, api, torch, k, api.FP32, reps, mode, warm)
     ms64 = time_sigma(H, api, torch, k, api.FP64, reps, mode, warm)
@@ -186,13 +206,14 @@ def main():
         n2 = (nbf * (nbf + 1) // 2) * (nbf * (nbf + 1) // 2 + 1) // 2
         bytes32 = k * n * (4 + 4) + 16 * cfg.ncfg + 8 * n2
         bytes64 = k * n * (8 + 8) + 16 * cfg.ncfg + 8 * n2
-        kernel = f"k_direct<float,{k}> (direct FP32 sigma, {k} vectors)"
+        kernel = f"k_direct<float,{min(k, 16)},true> (direct FP32 sigma, {k} vectors)"
     else:
         bytes32 = nnz * (4 + 4) + (n + 1) * 8 + k * n * (4 + 4)
         bytes64 = nnz * (8 + 4) + (n + 1) * 8 + k * n * (8 + 8)
         kernel = f"k_spmm<float,{k}> (stored FP32 sigma, {k} vectors)"
     ach32 = bytes32 / (ms32 * 1e-3) / 1e9
     ach64 = bytes64 / (ms64 * 1e-3) / 1e9
+    traffic = measured_traffic(args.workload, k)
 
     cpu = None
     if not args.no_cpu:
@@ -233,11 +254,15 @@ def main():
         "sigma_fp64_ms": round(ms64, 5),
         "sigma_fp64_gbs": round(ach64, 1),
         "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+                     "frac": round(ach32 / HBM_PEAK_GBS, 4),
+                     "traffic": traffic["bytes_per_launch"] if traffic else None, "kernel": kernel,
                      "algorithmic_bytes_per_launch": int(bytes32)},
         "cpu_baseline": cpu,
         "wall_seconds_timed": round(wall, 3),
     }
+    if traffic:
+        line["roofline"]["traffic_source"] = traffic["source"]
+        line["roofline"]["traffic_gbs"] = round(traffic["bytes_per_launch"] / (ms32 * 1e-3) / 1e9, 1)
     if direct:
         line["direct"] = {"ordered_pairs": npairs, "structural_elements_per_sigma": nelems,
                           "elements_per_s_fp32": round(nelems / (ms32 * 1e-3), 1),

@@ -46,6 +46,8 @@ struct Ctx {
   DevBuf<int64_t> d_adjoff;   // ncfg + 1: first record of each configuration
   DevBuf<PairRec> d_rec;      // one record per ordered connected pair
   DevBuf<double> d_cdiag;     // ncfg: vnt[0] of each diagonal block
+  DevBuf<int32_t> d_ndbl;     // ncfg: double-excitation records (first in each segment)
+  DevBuf<int32_t> d_nsgl;     // ncfg: single-excitation records (next)
   DevBuf<ConfRec> d_conf;     // packed configurations
   DevBuf<double> d_xt;        // XT[e][f][o] = (e o|o f)
   DevBuf<int64_t> d_detoff;   // ncfg

@@ -44,62 +44,90 @@ PSOCI_HD PairCoef pair_coef(int nopen_i, int nopen_j, double mult) {
   return c;
 }
 
-// Calls f(b, ab) for every determinant b of J (index within J, alpha open
-// shells ab) within two spin-orbital replacements of determinant `aa` of I.
-// didxJ: determinant index by alpha pattern for J's open-shell count.  The
-// candidates are walked by one flat loop (a flip set on the common open
-// shells times a subset of the new open shells), so f is expanded once.
-template <class F>
+// Calls f(b, ab) for every determinant b of a configuration J != I (index
+// within J, alpha open shells ab) within two spin-orbital replacements of
+// determinant `aa` of I.  didxJ: determinant index by alpha pattern for J's
+// open-shell count.  The spins of the new open shells run over the subsets
+// of exactly the admissible size (next subset by Gosper's rule), and a
+// common open shell is flipped only where such a subset exists, so every
+// candidate visited is a connection.
+template <int EX = 0, class F>
 PSOCI_HD void for_each_conn(const ConfView& I, const ConfView& J, uint64_t aa, const int16_t* __restrict__ didxJ,
                             F&& f) {
   const uint64_t C = I.s & J.s, N = J.s & ~I.s;
-  const int ex = moved(I.d, I.s, J.d, J.s);
+  // EX: the number of moved electrons when the caller knows it (1 or 2)
+  const int ex = EX ? EX : moved(I.d, I.s, J.d, J.s);
   const int nb = popc(I.d) - popc(J.d) + popc(aa);
   const uint64_t base = aa & C;
-  const int nn = popc(N);  // at most four: each new open shell is a moved electron's end
-  // flip sets: {} always; each common open shell when one electron moves;
-  // an alpha-beta pair of common open shells when none does
-  uint64_t remP = ex == 0 ? base : (ex == 1 ? C : 0ull);
-  const uint64_t allQ = ex == 0 ? (C & ~base) : 0ull;
-  uint64_t remQ = 0, curP = 0, flip = 0;
-  int q = 0;
+  // new open shells (at most four: each ends a moved electron), packed as
+  // orbital (6 bits) and bit of J's pattern (4 bits) per entry
+  uint32_t norb = 0, npos = 0;
+  int nn = 0;
+  for (uint64_t r = N; r; r &= r - 1, ++nn) {
+    const int o = ctz(r);
+    norb |= (uint32_t)o << (6 * nn);
+    npos |= (uint32_t)popc(J.s & ((1ull << o) - 1)) << (4 * nn);
+  }
+  uint32_t pbase = 0;
+  for (uint64_t r = base; r; r &= r - 1) pbase |= 1u << popc(J.s & ((1ull << ctz(r)) - 1));
+  const int need0 = nb - popc(base);
+  // one electron moved: a common open shell may change spin; alpha-to-beta
+  // needs one more alpha among the new shells, beta-to-alpha one fewer
+  uint64_t flips = 0;
+  if (ex == 1) flips = (need0 + 1 <= nn ? base : 0ull) | (need0 >= 1 ? (C & ~base) : 0ull);
+  uint64_t flip = 0;
+  uint32_t pflip = 0;
+  int need = need0;
   for (;;) {
-    const uint64_t cb = base ^ flip;
-    if (popc((uint64_t)q) == nb - popc(cb)) {
-      uint64_t ab = cb, r = N;
-      for (int t = 0; t < 4 && r; ++t, r &= r - 1)
-        if ((q >> t) & 1) ab |= r & (~r + 1);
-      // J's pattern of ab: one bit per open shell of J, ascending
-      uint32_t p = 0;
-      for (uint64_t u = ab; u; u &= u - 1) p |= 1u << popc(J.s & ((u & (~u + 1)) - 1));
-      f((int)didxJ[p], ab);
-    }
-    // next candidate
-    if (++q < (1 << nn)) continue;
-    q = 0;
-    if (ex == 0) {
-      if (!remQ) {
-        if (!remP) break;
-        curP = remP & (~remP + 1);
-        remP &= remP - 1;
-        remQ = allQ;
-        if (!remQ) break;
+    if (need >= 0 && need <= nn) {
+      uint32_t q = (1u << need) - 1u;
+      const uint32_t qend = 1u << nn;
+      while (q < qend) {
+        uint64_t ab = base ^ flip;
+        uint32_t p = pbase ^ pflip;
+        for (uint32_t u = q; u; u &= u - 1) {
+          const int t = __builtin_ctz(u);
+          ab |= 1ull << ((norb >> (6 * t)) & 63u);
+          p |= 1u << ((npos >> (4 * t)) & 15u);
+        }
+        f((int)didxJ[p], ab);
+        if (!q) break;
+        const uint32_t c = q & (0u - q), r = q + c;
+        q = (((r ^ q) >> 2) >> __builtin_ctz(c)) | r;
       }
-      flip = curP | (remQ & (~remQ + 1));
-      remQ &= remQ - 1;
-    } else {
-      if (!remP) break;
-      flip = remP & (~remP + 1);
-      remP &= remP - 1;
+    }
+    if (!flips) break;
+    flip = flips & (~flips + 1);
+    flips &= flips - 1;
+    pflip = 1u << popc(J.s & (flip - 1));
+    need = nb - popc(base ^ flip);
+  }
+}
+
+// The diagonal block (J = I): determinant aa itself and its alpha-beta spin
+// exchanges (one alpha and one beta open shell trade spins).
+template <class F>
+PSOCI_HD void for_each_conn_diag(const ConfView& I, uint64_t aa, const int16_t* __restrict__ didxI, F&& f) {
+  uint32_t pa = 0;
+  for (uint64_t r = aa; r; r &= r - 1) pa |= 1u << popc(I.s & ((1ull << ctz(r)) - 1));
+  f((int)didxI[pa], aa);
+  for (uint64_t p = aa; p; p &= p - 1) {
+    const uint64_t lp = p & (~p + 1);
+    const uint32_t pp = pa ^ (1u << popc(I.s & (lp - 1)));
+    for (uint64_t q = I.s & ~aa; q; q &= q - 1) {
+      const uint64_t lq = q & (~q + 1);
+      f((int)didxI[pp ^ (1u << popc(I.s & (lq - 1)))], aa ^ lp ^ lq);
     }
   }
 }
 
 // Element D(a, b) between determinant aa of I (row) and ab of a different
 // configuration J, evaluated in the reference orientation (row = the later
-// configuration; rowI says whether that is I) from the pair's staged values.
-PSOCI_HD double conn_value(const PairInfo& P, const PairX& px, bool rowI, const ConfView& I, const ConfView& J,
-                           uint64_t aa, uint64_t ab, const double* __restrict__ two) {
+// configuration; rowI says whether that is I); xrow: the pair's row of the
+// three-index exchange table (single excitations).
+template <int KIND = 0>
+PSOCI_HD double conn_value(const PairInfo& P, const double* __restrict__ xrow, bool rowI, const ConfView& I,
+                           const ConfView& J, uint64_t aa, uint64_t ab, const double* __restrict__ two) {
   ConfView R, C;
   R.d = rowI ? I.d : J.d;
   R.s = rowI ? I.s : J.s;
@@ -110,7 +138,7 @@ PSOCI_HD double conn_value(const PairInfo& P, const PairX& px, bool rowI, const 
   C.ph = rowI ? J.ph : I.ph;
   C.nopen = rowI ? J.nopen : I.nopen;
   R.pat = C.pat = nullptr;
-  return detval_x<false>(P, px, R, C, rowI ? aa : ab, rowI ? ab : aa, two);
+  return detval_x<false, KIND>(P, xrow, R, C, rowI ? aa : ab, rowI ? ab : aa, two);
 }
 
 }  // namespace psoci

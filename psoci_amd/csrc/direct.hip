@@ -144,25 +144,100 @@ __global__ void k_cnt64(int64_t n, const int32_t* __restrict__ a, int64_t* __res
   if (t < n) b[t] = a[t];
 }
 
-// one record per ordered pair, in the reference orientation
-__global__ void k_records(int64_t ncfg, const int64_t* __restrict__ adjoff, const int32_t* __restrict__ adj,
-                          const uint64_t* __restrict__ dm, const uint64_t* __restrict__ sm,
-                          const int8_t* __restrict__ nopen, const int32_t* __restrict__ nsef, int nbf,
-                          const double* __restrict__ one, const double* __restrict__ two, PairRec* __restrict__ rec,
-                          double* __restrict__ cdiag, unsigned long long* __restrict__ nelems) {
+// Record order within a configuration's segment: doubles, then singles, then
+// the rest (pairs the reference skips, and the diagonal block); within each
+// group by the number of open shells new in J and by J's open-shell count, so
+// that the records the sigma kernel runs side by side enumerate alike.
+constexpr int kNClass = 90;
+__device__ __forceinline__ int rec_class(const PairInfo& P, uint64_t sI, uint64_t sJ, int nJ) {
+  const int g = P.kind == 3 ? 0 : (P.kind == 2 ? 1 : 2);
+  return g * 30 + popc(sJ & ~sI) * 6 + (nJ >> 1);
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+  // LDS operations of one wavefront complete in order; the fences keep the
+  // compiler from moving LDS reads and writes across this point
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// One record per ordered pair, in the reference orientation, sorted by class
+// (stable: adjacency order within a class).  A wavefront per configuration:
+// a class histogram in LDS, its scan, then placement by ballot rank.
+__global__ __launch_bounds__(256) void k_records(int64_t ncfg, const int64_t* __restrict__ adjoff,
+                                                 const int32_t* __restrict__ adj, const uint64_t* __restrict__ dm,
+                                                 const uint64_t* __restrict__ sm, const int8_t* __restrict__ nopen,
+                                                 const int32_t* __restrict__ nsef, int nbf,
+                                                 const double* __restrict__ one, const double* __restrict__ two,
+                                                 PairRec* __restrict__ rec, double* __restrict__ cdiag,
+                                                 int32_t* __restrict__ ndbl, int32_t* __restrict__ nsgl,
+                                                 unsigned long long* __restrict__ nelems) {
+  __shared__ int s_base[4][kNClass];
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-  if (I >= ncfg) return;
-  unsigned long long ne = 0;
-  for (int64_t t = adjoff[I] + lane; t < adjoff[I + 1]; t += kWave) {
-    const int J = adj[t];
-    const bool rowI = J <= I;
+  const int wave = threadIdx.x / kWave;
+  const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + wave;
+  if (I >= ncfg) return;  // uniform per wavefront; wavefront-level synchronisation only
+  int* base = s_base[wave];
+  for (int c = lane; c < kNClass; c += kWave) base[c] = 0;
+  wave_sync_lds();
+  const int64_t a0 = adjoff[I], a1 = adjoff[I + 1];
+  const uint64_t dI = dm[I], sI = sm[I];
+  const int nI = nopen[I];
+  auto setup = [&](int J, PairInfo& P, bool& rowI) {
+    rowI = J <= I;
     const int R = rowI ? (int)I : J, C = rowI ? J : (int)I;
+    pair_setup(P, R, C, rowI ? dI : dm[J], rowI ? sI : sm[J], rowI ? nI : nopen[J], rowI ? dm[J] : dI,
+               rowI ? sm[J] : sI, rowI ? nopen[J] : nI, nbf, one, two);
+  };
+  for (int64_t t = a0 + lane; t < a1; t += kWave) {
+    const int J = adj[t];
     PairInfo P;
-    pair_setup(P, R, C, dm[R], sm[R], nopen[R], dm[C], sm[C], nopen[C], nbf, one, two);
-    rec[t] = make_rec(P, J, rowI);
-    if (J == (int)I) cdiag[I] = P.v0;
-    ne += (unsigned long long)nsef[J];
+    bool rowI;
+    setup(J, P, rowI);
+    atomicAdd(&base[rec_class(P, sI, sm[J], nopen[J])], 1);
+  }
+  wave_sync_lds();
+  if (lane == 0) {
+    int run = 0;
+    for (int c = 0; c < kNClass; ++c) {
+      const int n = base[c];
+      base[c] = run;
+      run += n;
+    }
+    ndbl[I] = base[30];
+    nsgl[I] = base[60] - base[30];
+  }
+  wave_sync_lds();
+  const uint64_t below = (1ull << lane) - 1ull;
+  unsigned long long ne = 0;
+  for (int64_t t0 = a0; t0 < a1; t0 += kWave) {
+    const int64_t t = t0 + lane;
+    bool act = t < a1;
+    int J = 0, cls = -1;
+    PairInfo P;
+    bool rowI = false;
+    if (act) {
+      J = adj[t];
+      setup(J, P, rowI);
+      cls = rec_class(P, sI, sm[J], nopen[J]);
+      ne += (unsigned long long)nsef[J];
+    }
+    uint64_t pend = __ballot(act);
+    while (pend) {
+      const int k0 = __shfl(cls, ctz(pend), kWave);
+      const uint64_t m = __ballot(act && cls == k0);
+      if (act && cls == k0) {
+        const int pos = base[k0] + popc(m & below);
+        rec[a0 + pos] = make_rec(P, J, rowI);
+        if (J == (int)I) cdiag[I] = P.v0;
+        act = false;
+      }
+      wave_sync_lds();
+      if (lane == 0) base[k0] += popc(m);
+      wave_sync_lds();
+      pend &= ~m;
+    }
   }
   for (int off = kWave / 2; off > 0; off >>= 1) ne += __shfl_xor(ne, off, kWave);
   if (lane == 0) atomicAdd(nelems, ne * (unsigned long long)nsef[I]);
@@ -192,6 +267,8 @@ struct DirectArgs {
   const int16_t* didx;
   int ndidx;
   const double* cdiag;
+  const int32_t* ndbl;
+  const int32_t* nsgl;
 };
 
 __device__ __forceinline__ ConfView view_of(const ConfRec& r) {
@@ -236,6 +313,29 @@ __device__ __forceinline__ void axpy_row(T hv, const T* __restrict__ x, T (&acc)
   }
 }
 
+// Contribution of one pair record (kind KIND) to the lane's row, determinant
+// aa of I: coef * D(aa, b) * x_J[b / 2] for every connected determinant b.
+template <typename T, int K, bool VEC, int KIND>
+__device__ __forceinline__ void pair_rows(const DirectArgs& a, const PairRec& r, const ConfView& A, uint64_t aa,
+                                          const int16_t* __restrict__ s_didx, const T* __restrict__ X, int ldx,
+                                          T (&acc)[K]) {
+  PairInfo P;
+  bool rowI;
+  int J;
+  unpack_rec(r, P, J, rowI);
+  const ConfRec rj = a.conf[J];
+  const ConfView B = view_of(rj);
+  const double* __restrict__ xrow = KIND == 2 ? a.xt + ((int64_t)P.e0 * a.nbf + P.f0) * a.nbf : nullptr;
+  const PairCoef pc = pair_coef(A.nopen, B.nopen, a.mult);
+  const T* __restrict__ xJ = X + (int64_t)rj.sefoff * ldx;
+  for_each_conn<KIND == 3 ? 2 : 1>(A, B, aa, s_didx + (1 << B.nopen) - 1, [&](int b, uint64_t ab) {
+    const double v = conn_value<KIND>(P, xrow, rowI, A, B, aa, ab, a.two);
+    if (v == 0.0) return;
+    const T hv = (T)(((b & 1) ? pc.co : pc.ce) * v);
+    axpy_row<T, K, VEC>(hv, xJ + (int64_t)(b >> 1) * ldx, acc);
+  });
+}
+
 // One wavefront per row configuration I; lanes are (record slot, SEF i of I)
 // with nsef_I * nslot = 64, so 64 / nsef_I records are in flight per step.
 // Each lane takes determinant 2i of I and, per record, enumerates the
@@ -272,48 +372,23 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
     P.v1 = 0.0;
     const PairCoef pc = pair_coef(A.nopen, A.nopen, a.mult);
     const T* __restrict__ xI = X + (int64_t)ri.sefoff * ldx;
-    for_each_conn(A, A, aa, s_didx + (1 << A.nopen) - 1, [&](int b, uint64_t ab) {
+    for_each_conn_diag(A, aa, s_didx + (1 << A.nopen) - 1, [&](int b, uint64_t ab) {
       const double v = detval_ao(P, A, A, aa, ab, a.two);
       if (v == 0.0) return;
       const T hv = (T)(((b & 1) ? pc.co : pc.ce) * v);
       axpy_row<T, K, VEC>(hv, xI + (int64_t)(b >> 1) * ldx, acc);
     });
   }
-  const int64_t t1 = a.recoff[I + 1];
-  for (int64_t t0 = a.recoff[I]; t0 < t1; t0 += nslot) {
+  // double excitations, then single excitations (the segment's order)
+  const int64_t r0 = a.recoff[I];
+  const int64_t r1 = r0 + a.ndbl[I], r2 = r1 + a.nsgl[I];
+  for (int64_t t0 = r0; t0 < r1; t0 += nslot) {
     const int64_t t = t0 + slot;
-    if (t >= t1) continue;
-    PairInfo P;
-    bool rowI;
-    int J;
-    unpack_rec(a.rec[t], P, J, rowI);
-    if (P.kind < 2) continue;  // skipped by the reference, or the diagonal block (done above)
-    const ConfRec rj = a.conf[J];
-    const ConfView B = view_of(rj);
-    PairX px;
-    if (P.kind == 2) {
-      // (e0 o|o f0) for the open shells o of the reference-orientation row
-      const double* __restrict__ xrow = a.xt + ((int64_t)P.e0 * a.nbf + P.f0) * a.nbf;
-      uint64_t rem = rowI ? A.s : B.s;
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        double v = 0.0;
-        if (rem) {
-          const int o = ctz(rem);
-          rem &= rem - 1;
-          v = xrow[o];
-        }
-        px.X[u] = v;
-      }
-    }
-    const PairCoef pc = pair_coef(A.nopen, B.nopen, a.mult);
-    const T* __restrict__ xJ = X + (int64_t)rj.sefoff * ldx;
-    for_each_conn(A, B, aa, s_didx + (1 << B.nopen) - 1, [&](int b, uint64_t ab) {
-      const double v = conn_value(P, px, rowI, A, B, aa, ab, a.two);
-      if (v == 0.0) return;
-      const T hv = (T)(((b & 1) ? pc.co : pc.ce) * v);
-      axpy_row<T, K, VEC>(hv, xJ + (int64_t)(b >> 1) * ldx, acc);
-    });
+    if (t < r1) pair_rows<T, K, VEC, 3>(a, a.rec[t], A, aa, s_didx, X, ldx, acc);
+  }
+  for (int64_t t0 = r1; t0 < r2; t0 += nslot) {
+    const int64_t t = t0 + slot;
+    if (t < r2) pair_rows<T, K, VEC, 2>(a, a.rec[t], A, aa, s_didx, X, ldx, acc);
   }
   for (int off = nI; off < kWave; off <<= 1) {
 #pragma unroll
@@ -340,6 +415,8 @@ DirectArgs direct_args(Ctx& c) {
   a.didx = c.d_didx.p;
   a.ndidx = (2 << c.sp.max_open) - 1;
   a.cdiag = c.d_cdiag.p;
+  a.ndbl = c.d_ndbl.p;
+  a.nsgl = c.d_nsgl.p;
   return a;
 }
 
@@ -426,11 +503,14 @@ void build_pairs(Ctx& c) {
   // pair records (the adjacency is folded into them and released)
   c.d_rec.alloc(total);
   c.d_cdiag.alloc(ncfg);
+  c.d_ndbl.alloc(ncfg);
+  c.d_nsgl.alloc(ncfg);
   DevBuf<unsigned long long> ne;
   ne.alloc(1);
   PSOCI_HIP(hipMemsetAsync(ne.p, 0, sizeof(unsigned long long), c.stream));
   hipLaunchKernelGGL(k_records, dim3(nblk(ncfg, 4)), dim3(256), 0, c.stream, ncfg, c.d_adjoff.p, adj.p, c.d_dm.p,
-                     c.d_sm.p, c.d_nopen.p, c.d_nsef.p, nbf, c.d_one.p, c.d_two.p, c.d_rec.p, c.d_cdiag.p, ne.p);
+                     c.d_sm.p, c.d_nopen.p, c.d_nsef.p, nbf, c.d_one.p, c.d_two.p, c.d_rec.p, c.d_cdiag.p, c.d_ndbl.p,
+                     c.d_nsgl.p, ne.p);
   PSOCI_HIP(hipGetLastError());
   unsigned long long nelems = 0;
   ne.download(&nelems, 1, c.stream);

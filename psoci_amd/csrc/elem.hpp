@@ -96,8 +96,8 @@ PSOCI_HD uint64_t deposit(uint32_t p, uint64_t mask) {
 
 // orbitals strictly between a and b
 PSOCI_HD uint64_t between(int a, int b) {
-  int lo = a < b ? a : b, hi = a < b ? b : a;
-  if (hi - lo <= 1) return 0ull;
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  // empty for hi - lo <= 1 without a branch (2 << 63 wraps to 0)
   return ((1ull << hi) - 1ull) & ~((2ull << lo) - 1ull);
 }
 
@@ -325,7 +325,7 @@ PSOCI_HD double sef_elem_ao(const PairInfo& P, const ConfView& A, const ConfView
 }
 
 // ---------------------------------------------------------------------------
-// Load-free evaluation for the off-diagonal pairs of the direct sigma.
+// Off-diagonal elements from a three-index table.
 //
 // For a single excitation e0 -> f0 every two-electron integral a determinant
 // pair needs is (e0 c|c f0) for an open shell c of the row configuration:
@@ -333,78 +333,59 @@ PSOCI_HD double sef_elem_ao(const PairInfo& P, const ConfView& A, const ConfView
 // its alpha-beta branch (the moved electron flips spin while c flips the
 // other way) reads (f0 c|c e0) or (c e0|f0 c), the same canonical slot.  For
 // a double excitation the alpha-beta branch reads vnt[0] or vnt[1].  So with
-// X[t] = (e0 o_t|o_t f0) for the row configuration's open shells o_t staged in
-// registers once per pair, the element loop touches no memory; the values
-// and their summation order are the reference's, so elements stay
-// bit-identical to detval_ao().
-struct PairX {
-  double X[8];
-};
+// xrow[o] = (e0 o|o f0) (row (e0, f0) of XT[e][f][o] = (e o|o f), nbf^3
+// doubles) the element needs no canonical-index arithmetic; the values and
+// their summation order are the reference's, so elements stay bit-identical
+// to detval_ao().
 
-PSOCI_HD void pair_x(const PairInfo& P, const ConfView& A, const double* __restrict__ two, PairX& px) {
-  uint64_t rem = A.s;
-#pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    double v = 0.0;
-    if (rem) {
-      int o = ctz(rem);
-      rem &= rem - 1;
-      if (P.kind == 2) v = g2(two, P.e0, o, o, P.f0);
-    }
-    px.X[t] = v;
-  }
+// XT[(e * nbf + f) * nbf + o] = (e o|o f), on the host
+inline void make_xt(int nbf, const double* two, double* xt) {
+  for (int e = 0; e < nbf; ++e)
+    for (int f = 0; f < nbf; ++f)
+      for (int o = 0; o < nbf; ++o) xt[((int64_t)e * nbf + f) * nbf + o] = g2(two, e, o, o, f);
 }
 
-PSOCI_HD double pick8(const PairX& px, int t) {
-  double v = px.X[0];
-#pragma unroll
-  for (int u = 1; u < 8; ++u) v = (t == u) ? px.X[u] : v;
-  return v;
-}
-
-// detval_ao() for off-diagonal pairs (kind 2 or 3) without integral loads.
-// SAME_OK = false drops the identical-determinant branch, which a pair of
-// distinct configurations never reaches.
-template <bool SAME_OK = true>
-PSOCI_HD double detval_x(const PairInfo& P, const PairX& px, const ConfView& A, const ConfView& B, uint64_t ao,
-                         uint64_t bo, const double* __restrict__ two) {
+// detval_ao() for off-diagonal pairs (kind 2 or 3); xrow as above (unused
+// for kind 3).  SAME_OK = false drops the identical-determinant branch, which
+// a pair of distinct configurations never reaches.  KIND = 2 or 3 promises
+// that kind and a determinant pair within two replacements (as enumerated by
+// for_each_conn), dropping the branches that cannot occur: a double
+// excitation is always two replacements, and the two-replacement pairs of a
+// single excitation are alpha-beta (the moved electron and a flipped open
+// shell trade spins).
+template <bool SAME_OK = true, int KIND = 0>
+PSOCI_HD double detval_x(const PairInfo& P, const double* __restrict__ xrow, const ConfView& A, const ConfView& B,
+                         uint64_t ao, uint64_t bo, const double* __restrict__ two) {
   uint64_t aI = A.d | ao, bI = A.d | (A.s & ~ao);
   uint64_t aJ = B.d | bo, bJ = B.d | (B.s & ~bo);
-  if (popc(aI) != popc(aJ)) return 0.0;
   uint64_t xa = aJ & ~aI, xb = bJ & ~bI;
   int ma = popc(xa), mb = popc(xb);
   int tot = ma + mb;
-  if (tot > 2 || tot == 0) {
-    if constexpr (SAME_OK) return tot == 0 ? detval_ao(P, A, B, ao, bo, two) : 0.0;
-    return 0.0;
+  if constexpr (KIND == 0) {
+    if (popc(aI) != popc(aJ)) return 0.0;
+    if (tot > 2 || tot == 0) {
+      if constexpr (SAME_OK) return tot == 0 ? detval_ao(P, A, B, ao, bo, two) : 0.0;
+      return 0.0;
+    }
   }
   int cnt = A.ph + B.ph;
   double v;
-  if (tot == 1) {
+  if (KIND != 3 && tot == 1) {
     uint64_t S = ma ? aI : bI;
     cnt += popc(S & between(P.e0, P.f0));
     v = P.v0;
-    const uint64_t T = S & ~A.d & ~(1ull << P.e0) & ~(1ull << P.f0);
-    uint64_t rem = A.s;
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      if (rem) {
-        int o = ctz(rem);
-        rem &= rem - 1;
-        if ((T >> o) & 1ull) v -= px.X[t];
-      }
-    }
-  } else if (ma == 1) {
+    for (uint64_t T = S & ~A.d & ~(1ull << P.e0) & ~(1ull << P.f0); T; T &= T - 1) v -= xrow[ctz(T)];
+  } else if (KIND == 2 || ma == 1) {
     int iex2 = ctz(xa), jex2 = ctz(xb);
     int iex3 = ctz(aI & ~aJ), jex3 = ctz(bI & ~bJ);
     cnt += popc(aI & between(iex2, iex3)) + popc(bI & between(jex2, jex3));
-    if (P.kind == 3) {
+    if (KIND == 3 || (KIND == 0 && P.kind == 3)) {
       // (jex2 jex3|iex2 iex3): alpha lost iex3, gained iex2
       v = ((iex3 == P.e0) == (iex2 == P.f0)) ? P.v0 : P.v1;
     } else {
       int c = (iex2 == jex3) ? iex2 : ((iex3 == jex2) ? iex3 : -1);
       if (c >= 0 && ((A.s >> c) & 1ull) && c != P.e0 && c != P.f0)
-        v = pick8(px, popc(A.s & ((1ull << c) - 1ull)));
+        v = xrow[c];
       else
         v = g2(two, jex2, jex3, iex2, iex3);
     }
@@ -417,20 +398,21 @@ PSOCI_HD double detval_x(const PairInfo& P, const PairX& px, const ConfView& A, 
 }
 
 // sef_elem_ao() for off-diagonal pairs, same operation order
-PSOCI_HD double sef_elem_x(const PairInfo& P, const PairX& px, const ConfView& A, const ConfView& B, uint64_t aI0,
+PSOCI_HD double sef_elem_x(const PairInfo& P, const double* __restrict__ xrow, const ConfView& A, const ConfView& B,
+                           uint64_t aI0,
                            uint64_t aI1, uint64_t aJ0, uint64_t aJ1, double multi, double multj,
                            const double* __restrict__ two) {
   double ds0, ds1 = 0.0;
   if (B.nopen == 0) {
-    ds0 = detval_x(P, px, A, B, aI0, aJ0, two);
-    if (A.nopen) ds1 = detval_x(P, px, A, B, aI1, aJ0, two);
+    ds0 = detval_x(P, xrow, A, B, aI0, aJ0, two);
+    if (A.nopen) ds1 = detval_x(P, xrow, A, B, aI1, aJ0, two);
   } else {
-    double t = detval_x(P, px, A, B, aI0, aJ0, two);
-    t += multj * detval_x(P, px, A, B, aI0, aJ1, two);
+    double t = detval_x(P, xrow, A, B, aI0, aJ0, two);
+    t += multj * detval_x(P, xrow, A, B, aI0, aJ1, two);
     ds0 = t * kRthlf;
     if (A.nopen) {
-      double u = detval_x(P, px, A, B, aI1, aJ0, two);
-      u += multj * detval_x(P, px, A, B, aI1, aJ1, two);
+      double u = detval_x(P, xrow, A, B, aI1, aJ0, two);
+      u += multj * detval_x(P, xrow, A, B, aI1, aJ1, two);
       ds1 = u * kRthlf;
     }
   }

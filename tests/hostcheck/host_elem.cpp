@@ -46,7 +46,7 @@ int hc_block(void* h, const double* one, const double* two, int64_t I, int64_t J
   return P.kind ? 1 : 0;
 }
 
-// block (I, J) through the load-free off-diagonal path (sef_elem_x);
+// block (I, J) through the three-index-table off-diagonal path (sef_elem_x);
 // returns the block status, or -1 for a diagonal pair (not covered)
 int hc_block_x(void* h, const double* one, const double* two, int64_t I, int64_t J, double* out) {
   const Space& sp = *static_cast<Space*>(h);
@@ -54,8 +54,9 @@ int hc_block_x(void* h, const double* one, const double* two, int64_t I, int64_t
   PairInfo P;
   pair_setup(P, (int)I, (int)J, sp.dm[I], sp.sm[I], sp.nopen[I], sp.dm[J], sp.sm[J], sp.nopen[J], sp.nbf, one, two);
   ConfView A = view(sp, I), B = view(sp, J);
-  PairX px;
-  pair_x(P, A, two, px);
+  std::vector<double> xt((size_t)sp.nbf * sp.nbf * sp.nbf);
+  make_xt(sp.nbf, two, xt.data());
+  const double* xrow = xt.data() + ((int64_t)(P.kind == 2 ? P.e0 : 0) * sp.nbf + (P.kind == 2 ? P.f0 : 0)) * sp.nbf;
   int ni = sp.nsef[I], nj = sp.nsef[J];
   for (int i = 0; i < ni; ++i)
     for (int j = 0; j < nj; ++j) {
@@ -63,7 +64,7 @@ int hc_block_x(void* h, const double* one, const double* two, int64_t I, int64_t
       if (P.kind) {
         uint64_t aI0 = deposit(A.pat[A.nopen ? 2 * i : 0], A.s), aI1 = A.nopen ? deposit(A.pat[2 * i + 1], A.s) : 0;
         uint64_t aJ0 = deposit(B.pat[B.nopen ? 2 * j : 0], B.s), aJ1 = B.nopen ? deposit(B.pat[2 * j + 1], B.s) : 0;
-        v = sef_elem_x(P, px, A, B, aI0, aI1, aJ0, aJ1, sp.mult, sp.mult, two);
+        v = sef_elem_x(P, xrow, A, B, aI0, aI1, aJ0, aJ1, sp.mult, sp.mult, two);
       }
       out[i * nj + j] = v;
     }
@@ -91,6 +92,8 @@ int hc_row(void* h, const double* one, const double* two, int64_t I, int k, cons
   const ConfView A = view(sp, I);
   const int ni = sp.nsef[I];
   for (int t = 0; t < ni * k; ++t) y[t] = 0.0;
+  std::vector<double> xt((size_t)sp.nbf * sp.nbf * sp.nbf);
+  make_xt(sp.nbf, two, xt.data());
   for (int64_t J = 0; J < sp.ncfg; ++J) {
     int dn = sp.nopen[I] - sp.nopen[J];
     if (dn > 4 || dn < -4) continue;
@@ -112,19 +115,22 @@ int hc_row(void* h, const double* one, const double* two, int64_t I, int k, cons
           for (int c = 0; c < k; ++c) y[i * k + c] += v * xJ[j * k + c];
         }
     } else {
-      PairX px;
-      pair_x(P, rowI ? A : B, two, px);
+      const double* xrow = xt.data() + ((int64_t)(P.kind == 2 ? P.e0 : 0) * sp.nbf + (P.kind == 2 ? P.f0 : 0)) * sp.nbf;
       const PairCoef pc = pair_coef(A.nopen, B.nopen, sp.mult);
       const int16_t* dj = sp.didx.data() + (1 << B.nopen) - 1;
       for (int i = 0; i < ni; ++i) {
         const uint64_t aa = A.nopen ? deposit(A.pat[2 * i], A.s) : 0ull;
-        for_each_conn(A, B, aa, dj, [&](int b, uint64_t ab) {
+        auto add = [&](int b, uint64_t ab) {
           if (b < 0) throw Error(PSOCI_ERR_NUMERIC, "determinant outside the class");
-          const double v = P.kind == 1 ? detval_ao(P, A, A, aa, ab, two) : conn_value(P, px, rowI, A, B, aa, ab, two);
+          const double v = P.kind == 1 ? detval_ao(P, A, A, aa, ab, two) : conn_value(P, xrow, rowI, A, B, aa, ab, two);
           const double cf = (b & 1) ? pc.co : pc.ce;
           const double hv = cf * v;
           for (int c = 0; c < k; ++c) y[i * k + c] += hv * xJ[(b >> 1) * k + c];
-        });
+        };
+        if (I == J)
+          for_each_conn_diag(A, aa, dj, add);
+        else
+          for_each_conn(A, B, aa, dj, add);
       }
     }
   }

@@ -64,7 +64,7 @@ def _compare(L, cfg, ints, rows):
                 assert st == st2, (I, J)
                 assert np.array_equal(ref, out), (I, J, np.abs(ref - out).max())
                 if I != J:
-                    # the load-free path used by the direct sigma
+                    # the three-index-table path used by the direct sigma
                     out_x = np.zeros_like(ref)
                     st3 = L.hc_block_x(h, _p(ints.valone), _p(ints.valtwo), I, J, _p(out_x))
                     assert st3 == st, (I, J)
