@@ -38,6 +38,10 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--sigma-reps", type=int, default=50)
+    ap.add_argument("--sigma-only", action="store_true",
+                    help="time sigma launches only (no Davidson); for profiling runs")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a plain 1 GiB device copy (reference reading for the FETCH/WRITE counters)")
     return ap.parse_args()
 
 
@@ -144,6 +148,14 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(3)
+    if args.calibrate_copy:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4, "launches": 3}))
+        return
 
     cfg, ints, _ = inputs.workload(args.workload)
     direct = inputs.WORKLOADS[args.workload]["kind"] != "fci"
@@ -166,6 +178,13 @@ def main():
     torch.cuda.synchronize()
     setup_s = time.time() - t0
     print(f"[bench] setup {setup_s:.2f} s", file=sys.stderr, flush=True)
+    if args.sigma_only:
+        reps = 2 if direct else args.sigma_reps
+        ms32 = time_sigma(H, api, torch, k, api.FP32, reps, mode, 1)
+        ms64 = time_sigma(H, api, torch, k, api.FP64, reps, mode, 1)
+        print(json.dumps({"workload": args.workload, "k": k, "sigma_fp32_ms": round(ms32, 4),
+                          "sigma_fp64_ms": round(ms64, 4), "setup_seconds": round(setup_s, 3)}))
+        return
 
     # all-FP64 reference solve of the same space
     ref = H.davidson(k, prec=api.FP64, mode=mode, tol=args.tol, want_vectors=False)
@@ -265,6 +284,7 @@ def main():
         line["roofline"]["traffic_gbs"] = round(traffic["bytes_per_launch"] / (ms32 * 1e-3) / 1e9, 1)
     if direct:
         line["direct"] = {"ordered_pairs": npairs, "structural_elements_per_sigma": nelems,
+                          "schedule": H.direct_schedule(),
                           "elements_per_s_fp32": round(nelems / (ms32 * 1e-3), 1),
                           "elements_per_s_fp64": round(nelems / (ms64 * 1e-3), 1),
                           "note": "compute-bound (integer + FP64 VALU); the HBM fraction is reported for the "

@@ -110,6 +110,14 @@ int psoci_sigma(psoci_ctx* ctx, int mode, int prec, int k, const void* X, void* 
  * (PsociGAhamiltonian.C:2822-2839). */
 int psoci_direct_info(psoci_ctx* ctx, int64_t* npairs, int64_t* nelems);
 
+/* The direct sigma's launch schedule as chosen by the autotuner at pair
+ * setup: row order (0 natural, 1 by occupation string), XCD run length in
+ * workgroups, and the tuner's best sample time in ms (-1 when fixed by the
+ * PSOCI_DIRECT_ORDER / PSOCI_DIRECT_CHUNK environment).  No reference
+ * counterpart (the reference has no launch schedule); any argument may be
+ * NULL. */
+int psoci_direct_schedule(psoci_ctx* ctx, int* order, int* chunk, double* tune_ms);
+
 /* Diagonal of H in the SEF basis: computeOnlyDiagElements
  * (PsociGAhamiltonian.C:13171-13250).  diag[n] host. */
 int psoci_diag(psoci_ctx* ctx, double* diag);

@@ -90,6 +90,7 @@ def load(path: str | None = None):
         "psoci_device_ptr": (I, [V, I, ctypes.POINTER(V)]),
         "psoci_project": (I, [V, I64, I, I, V, I, V, I, V]),
         "psoci_direct_info": (I, [V, V, V]),
+        "psoci_direct_schedule": (I, [V, V, V, V]),
         "psoci_sync": (I, [V]),
     }
     for name, (res, args) in sig.items():
@@ -242,6 +243,12 @@ class Hamiltonian:
         a, b = ctypes.c_int64(), ctypes.c_int64()
         self._check(self.L.psoci_direct_info(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def direct_schedule(self):
+        """The direct sigma's autotuned schedule: dict(order, chunk, tune_ms)."""
+        o, ch, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._check(self.L.psoci_direct_schedule(self._h, ctypes.byref(o), ctypes.byref(ch), ctypes.byref(ms)))
+        return {"order": o.value, "chunk": ch.value, "tune_ms": ms.value}
 
     def stream_handle(self) -> int:
         """The hipStream_t the library launches on (as an integer)."""

@@ -165,6 +165,17 @@ int psoci_direct_info(psoci_ctx* ctx, int64_t* npairs, int64_t* nelems) {
   });
 }
 
+int psoci_direct_schedule(psoci_ctx* ctx, int* order, int* chunk, double* tune_ms) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
+    if (!c.have_pairs) psoci::build_pairs(c);
+    if (order) *order = c.direct_order ? 1 : 0;
+    if (chunk) *chunk = c.direct_chunk;
+    if (tune_ms) *tune_ms = c.direct_tune_ms;
+    return PSOCI_OK;
+  });
+}
+
 int psoci_diag(psoci_ctx* ctx, double* diag) {
   return guarded(ctx, [&](Ctx& c) {
     PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");

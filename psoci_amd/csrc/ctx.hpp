@@ -48,6 +48,10 @@ struct Ctx {
   DevBuf<double> d_cdiag;     // ncfg: vnt[0] of each diagonal block
   DevBuf<int32_t> d_ndbl;     // ncfg: double-excitation records (first in each segment)
   DevBuf<int32_t> d_nsgl;     // ncfg: single-excitation records (next)
+  DevBuf<int32_t> d_order;    // ncfg: alternative row order of the direct sigma
+  bool direct_order = false;  // use d_order
+  int direct_chunk = 0;       // XCD run length (workgroups) of the direct sigma schedule
+  double direct_tune_ms = -1; // the tuner's best sample time (-1: fixed by the environment)
   DevBuf<ConfRec> d_conf;     // packed configurations
   DevBuf<double> d_xt;        // XT[e][f][o] = (e o|o f)
   DevBuf<int64_t> d_detoff;   // ncfg
@@ -55,6 +59,7 @@ struct Ctx {
   DevBuf<int16_t> d_didx;     // determinant index by open-shell pattern (Space::didx)
 
   // scratch
+  DevBuf<char> d_pad;  // zero-padded column staging for the direct sigma
   void* scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
 

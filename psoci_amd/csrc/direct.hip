@@ -25,8 +25,12 @@
 // Elements are the reference's determinant-pair values (detval_ao/detval_x,
 // reference orientation) combined into SEF elements as D(2i,2j) +
 // mult D(2i,2j+1), equal to transformCDCt's to rounding.
+#include <algorithm>
+#include <cstdlib>
+
 #include "ctx.hpp"
 #include "detdrive.hpp"
+#include "pad.hpp"
 
 namespace psoci {
 
@@ -269,6 +273,9 @@ struct DirectArgs {
   const double* cdiag;
   const int32_t* ndbl;
   const int32_t* nsgl;
+  const int32_t* order;  // row processing order (null: natural)
+  int chunk;             // XCD run length in workgroups (0: plain round-robin)
+  int64_t row0, nrows;   // the slice of the row order to process (all rows but when tuning)
 };
 
 __device__ __forceinline__ ConfView view_of(const ConfRec& r) {
@@ -350,8 +357,18 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
   __syncthreads();
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
-  const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + wave;
-  if (I >= a.ncfg) return;  // uniform per wavefront; no block-wide barrier below
+  // workgroups are dealt to the 8 XCDs round-robin; with chunk C > 0 each
+  // XCD takes runs of C consecutive workgroups of the row order in turn (the
+  // grid is a multiple of 8C), so rows that share column configurations meet
+  // in the same L2 while the XCDs stay evenly loaded
+  int64_t wg = blockIdx.x;
+  if (a.chunk > 0) {
+    const int64_t q = blockIdx.x / 8, x = blockIdx.x % 8;
+    wg = (q / a.chunk) * 8 * a.chunk + x * a.chunk + q % a.chunk;
+  }
+  const int64_t slotw = wg * (blockDim.x / kWave) + wave;
+  if (slotw >= a.nrows) return;  // uniform per wavefront; no block-wide barrier below
+  const int64_t I = a.order ? a.order[a.row0 + slotw] : a.row0 + slotw;
   const ConfRec ri = a.conf[I];
   const ConfView A = view_of(ri);
   const int nI = ri.nsef;
@@ -417,44 +434,98 @@ DirectArgs direct_args(Ctx& c) {
   a.cdiag = c.d_cdiag.p;
   a.ndbl = c.d_ndbl.p;
   a.nsgl = c.d_nsgl.p;
+  a.order = c.direct_order ? c.d_order.p : nullptr;
+  a.chunk = c.direct_chunk;
+  a.row0 = 0;
+  a.nrows = c.sp.ncfg;
   return a;
 }
 
 template <typename T, int K>
-void launch_direct(Ctx& c, const T* X, int ldx, T* Y, int ldy) {
+void launch_direct(Ctx& c, const T* X, int ldx, T* Y, int ldy, int64_t row0 = 0, int64_t nrows = -1) {
   DirectArgs a = direct_args(c);
+  if (nrows >= 0) {
+    a.row0 = row0;
+    a.nrows = nrows;
+  }
   const bool vec = ((uintptr_t)X % 16 == 0) && ((size_t)ldx * sizeof(T)) % 16 == 0;
+  const int m = 8 * (a.chunk > 0 ? a.chunk : 1);
+  const unsigned grid = m * nblk(nblk(a.nrows, 4), m);
   if (vec)
-    hipLaunchKernelGGL((k_direct<T, K, true>), dim3(nblk(a.ncfg, 4)), dim3(256), 0, c.stream, a, X, ldx, Y, ldy);
+    hipLaunchKernelGGL((k_direct<T, K, true>), dim3(grid), dim3(256), 0, c.stream, a, X, ldx, Y, ldy);
   else
-    hipLaunchKernelGGL((k_direct<T, K, false>), dim3(nblk(a.ncfg, 4)), dim3(256), 0, c.stream, a, X, ldx, Y, ldy);
+    hipLaunchKernelGGL((k_direct<T, K, false>), dim3(grid), dim3(256), 0, c.stream, a, X, ldx, Y, ldy);
   PSOCI_HIP(hipGetLastError());
 }
 
 template <typename T>
 void direct_any_k(Ctx& c, int k, const T* X, int ldx, T* Y, int ldy) {
-  int done = 0;
-  while (done < k) {
-    const int rem = k - done;
-    const T* x = X + done;
-    T* y = Y + done;
-    if (rem >= 16) {
-      launch_direct<T, 16>(c, x, ldx, y, ldy);
-      done += 16;
-    } else if (rem >= 8) {
-      launch_direct<T, 8>(c, x, ldx, y, ldy);
-      done += 8;
-    } else if (rem >= 4) {
-      launch_direct<T, 4>(c, x, ldx, y, ldy);
-      done += 4;
-    } else if (rem >= 2) {
-      launch_direct<T, 2>(c, x, ldx, y, ldy);
-      done += 2;
-    } else {
-      launch_direct<T, 1>(c, x, ldx, y, ldy);
-      done += 1;
+  pad::blocks<T>(c, c.sp.nsef_total, k, 16, X, ldx, Y, ldy, [&](int K, const T* x, int lx, T* y, int ly) {
+    switch (K) {
+      case 16: launch_direct<T, 16>(c, x, lx, y, ly); break;
+      case 8: launch_direct<T, 8>(c, x, lx, y, ly); break;
+      case 4: launch_direct<T, 4>(c, x, lx, y, ly); break;
+      case 2: launch_direct<T, 2>(c, x, lx, y, ly); break;
+      default: launch_direct<T, 1>(c, x, lx, y, ly); break;
     }
+  });
+}
+
+// Schedule autotuner: the row order (natural, or by occupation string from
+// orbital 0) and the XCD run length decide how often column configurations
+// are found in L2, which depends on the space's structure.  Each candidate
+// runs the FP32 16-column kernel on the middle eighth of its row order
+// (at least 65,536 rows, or all) twice; the fastest is kept.
+// PSOCI_DIRECT_ORDER / PSOCI_DIRECT_CHUNK fix the choice instead.
+void tune_direct(Ctx& c) {
+  const char* eo = std::getenv("PSOCI_DIRECT_ORDER");
+  const char* ec = std::getenv("PSOCI_DIRECT_CHUNK");
+  if (eo || ec) {
+    c.direct_order = eo && eo[0] == '1';
+    c.direct_chunk = ec ? std::max(0, std::atoi(ec)) : 0;
+    c.direct_tune_ms = -1.0;
+    return;
   }
+  const int64_t ncfg = c.sp.ncfg, n = c.sp.nsef_total;
+  const int64_t rows = std::min<int64_t>(ncfg, std::max<int64_t>(ncfg / 8, 65536));
+  const int64_t row0 = (ncfg - rows) / 2;
+  const int K = 16;
+  DevBuf<float> X, Y;
+  X.alloc((size_t)n * K);
+  Y.alloc((size_t)n * K);
+  PSOCI_HIP(hipMemsetAsync(X.p, 0, (size_t)n * K * sizeof(float), c.stream));
+  hipEvent_t e0, e1;
+  PSOCI_HIP(hipEventCreate(&e0));
+  PSOCI_HIP(hipEventCreate(&e1));
+  const int chunks[] = {0, 4, 16};
+  double best = 1e300;
+  bool best_order = false;
+  int best_chunk = 0;
+  for (int o = 0; o < 2; ++o)
+    for (int ch : chunks) {
+      c.direct_order = o == 1;
+      c.direct_chunk = ch;
+      double t = 1e300;
+      for (int r = 0; r < 2; ++r) {
+        PSOCI_HIP(hipEventRecord(e0, c.stream));
+        launch_direct<float, 16>(c, X.p, K, Y.p, K, row0, rows);
+        PSOCI_HIP(hipEventRecord(e1, c.stream));
+        PSOCI_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PSOCI_HIP(hipEventElapsedTime(&ms, e0, e1));
+        t = std::min(t, (double)ms);
+      }
+      if (t < best) {
+        best = t;
+        best_order = c.direct_order;
+        best_chunk = ch;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c.direct_order = best_order;
+  c.direct_chunk = best_chunk;
+  c.direct_tune_ms = best;
 }
 
 }  // namespace
@@ -466,6 +537,24 @@ void build_pairs(Ctx& c) {
   PSOCI_CHECK(sp.nbf <= 63, PSOCI_ERR_UNSUPPORTED, "direct sigma supports at most 63 orbitals");
   PSOCI_CHECK(sp.ndet_total < ((int64_t)1 << 31), PSOCI_ERR_UNSUPPORTED, "more than 2^31 determinants");
   const int64_t ncfg = sp.ncfg;
+  {
+    // alternative row order: lexicographic in the occupation string from
+    // orbital 0 up (in an MR-CISD list with the internal orbitals first this
+    // groups rows by internal part, across external patterns)
+    std::vector<unsigned __int128> key(ncfg);
+    for (int64_t i = 0; i < ncfg; ++i) {
+      unsigned __int128 k = 0;
+      for (int o = 0; o < sp.nbf; ++o) {
+        const unsigned v = 2u * (unsigned)((sp.dm[i] >> o) & 1) + (unsigned)((sp.sm[i] >> o) & 1);
+        k |= (unsigned __int128)v << (2 * (63 - o));
+      }
+      key[i] = k;
+    }
+    std::vector<int32_t> ord(ncfg);
+    for (int64_t i = 0; i < ncfg; ++i) ord[i] = (int32_t)i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) { return key[x] > key[y]; });
+    c.d_order.upload(ord, c.stream);
+  }
   c.d_detoff.upload(sp.det_off, c.stream);
   c.d_detao.alloc(sp.ndet_total);
   hipLaunchKernelGGL(k_detao, dim3(nblk(ncfg, 4)), dim3(256), 0, c.stream, ncfg, c.d_detoff.p, c.d_sm.p, c.d_nopen.p,
@@ -518,6 +607,7 @@ void build_pairs(Ctx& c) {
   c.direct_elems = (int64_t)nelems;
   c.nadj = total;
   c.have_pairs = true;
+  tune_direct(c);
 }
 
 void sigma_direct(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy) {

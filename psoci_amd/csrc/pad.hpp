@@ -1,0 +1,58 @@
+// Column-blocked sigma launches at power-of-two kernel widths.
+//
+// The sigma kernels are instantiated for block widths 1, 2, 4, ... maxw.  A
+// block of nb columns runs as ONE launch at the next width through a
+// zero-padded staging copy, rather than as several narrower launches: a
+// launch's cost is dominated by what does not scale with the width (element
+// generation in the direct sigma, reading H in the stored one).
+#pragma once
+#include "ctx.hpp"
+
+namespace psoci {
+namespace pad {
+namespace {  // each translation unit keeps its own kernels
+
+template <typename T>
+__global__ void k_pad_cols(int64_t n, int nb, int K, const T* __restrict__ X, int ldx, T* __restrict__ Xp) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * K) return;
+  const int64_t row = t / K;
+  const int col = (int)(t - row * K);
+  Xp[t] = col < nb ? X[row * ldx + col] : (T)0;
+}
+
+template <typename T>
+__global__ void k_unpad_cols(int64_t n, int nb, int K, const T* __restrict__ Yp, T* __restrict__ Y, int ldy) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * nb) return;
+  const int64_t row = t / nb;
+  const int col = (int)(t - row * nb);
+  Y[row * ldy + col] = Yp[row * K + col];
+}
+
+// launch(K, x, ldx, y, ldy) runs the kernel of width K on n rows.
+template <typename T, class Launch>
+void blocks(Ctx& c, int64_t n, int k, int maxw, const T* X, int ldx, T* Y, int ldy, Launch&& launch) {
+  for (int done = 0; done < k; done += maxw) {
+    const int nb = k - done < maxw ? k - done : maxw;
+    int K = 1;
+    while (K < nb) K <<= 1;
+    if (K == nb) {
+      launch(K, X + done, ldx, Y + done, ldy);
+      continue;
+    }
+    c.d_pad.reserve((size_t)2 * n * K * sizeof(T));
+    T* Xp = reinterpret_cast<T*>(c.d_pad.p);
+    T* Yp = Xp + (size_t)n * K;
+    const unsigned g1 = (unsigned)((n * K + 255) / 256), g2 = (unsigned)((n * nb + 255) / 256);
+    hipLaunchKernelGGL(k_pad_cols<T>, dim3(g1), dim3(256), 0, c.stream, n, nb, K, X + done, ldx, Xp);
+    PSOCI_HIP(hipGetLastError());
+    launch(K, Xp, K, Yp, K);
+    hipLaunchKernelGGL(k_unpad_cols<T>, dim3(g2), dim3(256), 0, c.stream, n, nb, K, Yp, Y + done, ldy);
+    PSOCI_HIP(hipGetLastError());
+  }
+}
+
+}  // namespace
+}  // namespace pad
+}  // namespace psoci

@@ -9,6 +9,7 @@
 // reduces one row for all k vectors at once: the row's values and columns
 // stream from HBM once per launch, X rows are gathered from L2/MALL.
 #include "ctx.hpp"
+#include "pad.hpp"
 
 namespace psoci {
 
@@ -67,31 +68,16 @@ void launch_spmm(Ctx& c, const T* val, const T* X, int ldx, T* Y, int ldy) {
 
 template <typename T>
 void spmm_any_k(Ctx& c, const T* val, int k, const T* X, int ldx, T* Y, int ldy) {
-  int done = 0;
-  while (done < k) {
-    int rem = k - done;
-    const T* x = X + done;
-    T* y = Y + done;
-    if (rem >= 32) {
-      launch_spmm<T, 32>(c, val, x, ldx, y, ldy);
-      done += 32;
-    } else if (rem >= 16) {
-      launch_spmm<T, 16>(c, val, x, ldx, y, ldy);
-      done += 16;
-    } else if (rem >= 8) {
-      launch_spmm<T, 8>(c, val, x, ldx, y, ldy);
-      done += 8;
-    } else if (rem >= 4) {
-      launch_spmm<T, 4>(c, val, x, ldx, y, ldy);
-      done += 4;
-    } else if (rem >= 2) {
-      launch_spmm<T, 2>(c, val, x, ldx, y, ldy);
-      done += 2;
-    } else {
-      launch_spmm<T, 1>(c, val, x, ldx, y, ldy);
-      done += 1;
+  pad::blocks<T>(c, c.sp.nsef_total, k, 32, X, ldx, Y, ldy, [&](int K, const T* x, int lx, T* y, int ly) {
+    switch (K) {
+      case 32: launch_spmm<T, 32>(c, val, x, lx, y, ly); break;
+      case 16: launch_spmm<T, 16>(c, val, x, lx, y, ly); break;
+      case 8: launch_spmm<T, 8>(c, val, x, lx, y, ly); break;
+      case 4: launch_spmm<T, 4>(c, val, x, lx, y, ly); break;
+      case 2: launch_spmm<T, 2>(c, val, x, lx, y, ly); break;
+      default: launch_spmm<T, 1>(c, val, x, lx, y, ly); break;
     }
-  }
+  });
 }
 
 }  // namespace
