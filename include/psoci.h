@@ -129,7 +129,10 @@ typedef struct {
   int max_iter;       /* block iterations */
   int max_subspace;   /* basis size cap before collapse (0: 5*nroots) */
   int guess;          /* 0: unit vectors on the k lowest diagonal elements;
-                         1: unit vectors on the first k SEFs (PsociGAvectors.C:687-726) */
+                         1: unit vectors on the first k SEFs (PsociGAvectors.C:687-726);
+                         2: the caller's vectors, passed in evecs ([n][k] row-major, any
+                            linearly independent set; orthonormalised here), as a restart
+                            from stored CI vectors (PsociGAvectors.C:539-659) */
   double tol;         /* residual-norm tolerance for convergence */
   double switch_tol;  /* PSOCI_MIXED: FP32 sigma until max residual < switch_tol */
 } psoci_dav_params;

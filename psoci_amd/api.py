@@ -219,10 +219,16 @@ class Hamiltonian:
 
     def davidson(self, nroots: int, prec: int = FP64, mode: int = STORED, tol: float = 1e-6,
                  switch_tol: float = 1e-4, max_iter: int = 200, max_subspace: int = 0, guess: int = 0,
-                 want_vectors: bool = True):
+                 want_vectors: bool = True, guess_vectors=None):
+        """Block Davidson for the nroots lowest roots.  guess: 0 lowest diagonal,
+        1 first SEFs (the reference's), 2 guess_vectors ([n][nroots])."""
         p = DavParams(nroots, prec, mode, max_iter, max_subspace, guess, tol, switch_tol)
         ev = np.zeros(nroots)
         vec = np.zeros((self.n, nroots)) if want_vectors else None
+        if guess == 2:
+            if guess_vectors is None:
+                raise ValueError("guess=2 needs guess_vectors [n][nroots]")
+            vec = np.ascontiguousarray(guess_vectors, dtype=np.float64).reshape(self.n, nroots).copy()
         st = DavStats()
         rc = self.L.psoci_davidson(self._h, ctypes.byref(p), _p(ev), _p(vec), ctypes.byref(st))
         if rc not in (OK, ERR_CONVERGE):

@@ -125,8 +125,11 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
   S.switch_iter = -1;
 
   // guess vectors
+  PSOCI_CHECK(p.guess >= 0 && p.guess <= 2, PSOCI_ERR_ARG, "guess must be 0, 1 or 2");
   std::vector<int64_t> idx(k);
-  if (p.guess == 1) {
+  if (p.guess == 2) {
+    // caller's vectors, below
+  } else if (p.guess == 1) {
     std::iota(idx.begin(), idx.end(), 0);
   } else {
     std::vector<double> D(n);
@@ -138,7 +141,13 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
                       [&](int64_t a, int64_t b) { return D[a] < D[b] || (D[a] == D[b] && a < b); });
     std::copy(ord.begin(), ord.begin() + k, idx.begin());
   }
-  L.unit_cols(n, k, idx.data(), V.p, mmax);
+  if (p.guess == 2) {
+    PSOCI_CHECK(evecs, PSOCI_ERR_ARG, "guess 2 needs the starting vectors in evecs");
+    R.upload(evecs, (size_t)n * k, c.stream);
+    L.copy_cols(n, k, R.p, k, V.p, mmax);
+  } else {
+    L.unit_cols(n, k, idx.data(), V.p, mmax);
+  }
 
   bool phase32 = (p.prec == PSOCI_FP32 || p.prec == PSOCI_MIXED);
   const double switch_tol = p.switch_tol > 0 ? p.switch_tol : 1e-4;
@@ -176,8 +185,6 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
   };
 
   int m = k;
-  do_sigma(0, k);
-  update_A(0, k);
 
   std::vector<double> Y, w, Yk, Yk2, G, P;
   auto solve = [&](int mm) {
@@ -230,6 +237,14 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     for (auto& x : P) x = -x;
     L.gemm_nn(n, c0, nb, V.p, mmax, P.data(), 1.0, V.p + c0, mmax);
   };
+
+  if (p.guess == 2) {
+    // two passes of symmetric orthonormalisation of the caller's vectors
+    for (int pass = 0; pass < 2; ++pass)
+      PSOCI_CHECK(sym_orth(0, k) == k, PSOCI_ERR_ARG, "starting vectors are linearly dependent");
+  }
+  do_sigma(0, k);
+  update_A(0, k);
 
   std::vector<double> rn(k, 0.0);
   const char* tr = std::getenv("PSOCI_TRACE");

@@ -66,12 +66,21 @@ __global__ __launch_bounds__(256) void k_gemm_tn_mfma(int64_t n, int m, int b, c
   }
 }
 
-__global__ void k_reduce_chunks(int64_t nchunks, int mb, const double* __restrict__ partial, double* __restrict__ out) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= mb) return;
+// out[e] = sum over chunks of partial[c][e]: one workgroup per output entry,
+// strided partial sums then a fixed-order tree (deterministic)
+__global__ __launch_bounds__(256) void k_reduce_chunks(int64_t nchunks, int mb, const double* __restrict__ partial,
+                                                       double* __restrict__ out) {
+  __shared__ double sh[256];
+  const int e = blockIdx.x;
   double s = 0.0;
-  for (int64_t c = 0; c < nchunks; ++c) s += partial[c * mb + e];
-  out[e] = s;
+  for (int64_t c = threadIdx.x; c < nchunks; c += 256) s += partial[c * mb + e];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[e] = sh[0];
 }
 
 // Out[row][0..k) = alpha * Out + sum_t A[row][t] * Y1[t][c]  (- sum_t B[row][t] * Y2[t][c])
@@ -170,7 +179,7 @@ void Linalg::gemm_tn(int64_t n, int m, int b, const double* A, int lda, const do
   dim3 grid(nblk(tiles, 4), (unsigned)nchunks);
   hipLaunchKernelGGL(k_gemm_tn_mfma, grid, dim3(256), 0, stream, n, m, b, A, lda, B, ldb, rows_per_chunk, part.p);
   PSOCI_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce_chunks, dim3(nblk(m * b, 256)), dim3(256), 0, stream, nchunks, m * b, part.p, out_dev);
+  hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)(m * b)), dim3(256), 0, stream, nchunks, m * b, part.p, out_dev);
   PSOCI_HIP(hipGetLastError());
 }
 
