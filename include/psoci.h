@@ -135,6 +135,8 @@ typedef struct {
                             from stored CI vectors (PsociGAvectors.C:539-659) */
   double tol;         /* residual-norm tolerance for convergence */
   double switch_tol;  /* PSOCI_MIXED: FP32 sigma until max residual < switch_tol */
+  int extra_roots;    /* guard roots carried in the block beyond nroots (convergence is
+                         tested on the nroots lowest only); 0 for the plain solve */
 } psoci_dav_params;
 
 typedef struct {

@@ -37,7 +37,7 @@ class PsociError(RuntimeError):
 class DavParams(ctypes.Structure):
     _fields_ = [("nroots", ctypes.c_int), ("prec", ctypes.c_int), ("sigma_mode", ctypes.c_int),
                 ("max_iter", ctypes.c_int), ("max_subspace", ctypes.c_int), ("guess", ctypes.c_int),
-                ("tol", ctypes.c_double), ("switch_tol", ctypes.c_double)]
+                ("tol", ctypes.c_double), ("switch_tol", ctypes.c_double), ("extra_roots", ctypes.c_int)]
 
 
 class DavStats(ctypes.Structure):
@@ -219,10 +219,11 @@ class Hamiltonian:
 
     def davidson(self, nroots: int, prec: int = FP64, mode: int = STORED, tol: float = 1e-6,
                  switch_tol: float = 1e-4, max_iter: int = 200, max_subspace: int = 0, guess: int = 0,
-                 want_vectors: bool = True, guess_vectors=None):
+                 want_vectors: bool = True, guess_vectors=None, extra_roots: int = 0):
         """Block Davidson for the nroots lowest roots.  guess: 0 lowest diagonal,
-        1 first SEFs (the reference's), 2 guess_vectors ([n][nroots])."""
-        p = DavParams(nroots, prec, mode, max_iter, max_subspace, guess, tol, switch_tol)
+        1 first SEFs (the reference's), 2 guess_vectors ([n][nroots]);
+        extra_roots guard roots ride along in the block (not tested)."""
+        p = DavParams(nroots, prec, mode, max_iter, max_subspace, guess, tol, switch_tol, extra_roots)
         ev = np.zeros(nroots)
         vec = np.zeros((self.n, nroots)) if want_vectors else None
         if guess == 2:

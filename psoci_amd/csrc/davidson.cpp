@@ -93,8 +93,13 @@ struct Timer {
 int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, psoci_dav_stats* st) {
   Timer total;
   const int64_t n = c.sp.nsef_total;
-  const int k = p.nroots;
-  PSOCI_CHECK(k >= 1 && k <= 64 && k <= n, PSOCI_ERR_ARG, "nroots must be in 1..min(64, n)");
+  const int kw = p.nroots;  // roots wanted
+  PSOCI_CHECK(kw >= 1 && kw <= 64 && kw <= n, PSOCI_ERR_ARG, "nroots must be in 1..min(64, n)");
+  PSOCI_CHECK(p.extra_roots >= 0 && p.extra_roots <= 64, PSOCI_ERR_ARG, "extra_roots must be in 0..64");
+  PSOCI_CHECK(p.guess != 2 || p.extra_roots == 0, PSOCI_ERR_ARG, "guess 2 takes no extra roots");
+  // working block: the wanted roots plus guard roots, which speed up the
+  // convergence of the highest wanted ones; only the wanted are tested
+  const int k = (int)std::min<int64_t>((int64_t)kw + p.extra_roots, n);
   PSOCI_CHECK(p.prec == PSOCI_FP64 || p.prec == PSOCI_FP32 || p.prec == PSOCI_MIXED, PSOCI_ERR_ARG, "bad precision");
   PSOCI_CHECK(p.sigma_mode == PSOCI_SIGMA_STORED || p.sigma_mode == PSOCI_SIGMA_DIRECT, PSOCI_ERR_ARG, "bad sigma mode");
   PSOCI_CHECK(p.tol > 0, PSOCI_ERR_ARG, "tol must be positive");
@@ -263,12 +268,12 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     double maxr = 0.0;
     for (int j = 0; j < k; ++j) {
       rn[j] = std::sqrt(std::max(G[(size_t)j * k + j], 0.0));
-      maxr = std::max(maxr, rn[j]);
+      if (j < kw) maxr = std::max(maxr, rn[j]);
     }
     S.max_rnorm = maxr;
     if (trace)
       std::fprintf(stderr, "[psoci davidson] it %d m %d %s max|r| %.3e  E0 %.12f E%d %.12f\n", it, m,
-                   phase32 ? "fp32" : "fp64", maxr, w[0], k - 1, w[k - 1]);
+                   phase32 ? "fp32" : "fp64", maxr, w[0], kw - 1, w[kw - 1]);
     if (phase32 && p.prec == PSOCI_MIXED && maxr < switch_tol) {
       // switch: collapse to the Ritz vectors and recompute their sigma in FP64
       collapse(k);
@@ -313,11 +318,11 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
   S.iterations = std::min(it, maxit);
   S.converged = converged ? 1 : 0;
   solve(m);
-  for (int j = 0; j < k; ++j) evals[j] = w[j];
+  for (int j = 0; j < kw; ++j) evals[j] = w[j];
   if (evecs) {
-    cols_of(m, 0, k, Yk);
-    L.gemm_nn(n, m, k, V.p, mmax, Yk.data(), 0.0, R.p, k);
-    R.download(evecs, (size_t)n * k, c.stream);
+    cols_of(m, 0, kw, Yk);
+    L.gemm_nn(n, m, kw, V.p, mmax, Yk.data(), 0.0, R.p, kw);
+    R.download(evecs, (size_t)n * kw, c.stream);
     c.sync();
   }
   S.seconds = total.sec();

@@ -44,81 +44,136 @@ PSOCI_HD PairCoef pair_coef(int nopen_i, int nopen_j, double mult) {
   return c;
 }
 
-// Calls f(b, ab) for every determinant b of a configuration J != I (index
-// within J, alpha open shells ab) within two spin-orbital replacements of
-// determinant `aa` of I.  didxJ: determinant index by alpha pattern for J's
-// open-shell count.  The spins of the new open shells run over the subsets
-// of exactly the admissible size (next subset by Gosper's rule), and a
-// common open shell is flipped only where such a subset exists, so every
-// candidate visited is a connection.
-template <int EX = 0, class F>
-PSOCI_HD void for_each_conn(const ConfView& I, const ConfView& J, uint64_t aa, const int16_t* __restrict__ didxJ,
-                            F&& f) {
+// Resumable enumeration of the determinants b of a configuration J != I
+// (index within J, alpha open shells ab) within two spin-orbital
+// replacements of determinant `aa` of I.  The spins of the new open shells
+// run over the subsets of exactly the admissible size (next subset by
+// Gosper's rule), and a common open shell is flipped only where such a
+// subset exists, so every candidate visited is a connection.  ex: electrons
+// moved between the configurations (1 or 2).
+struct ConnIter {
+  uint64_t base, flips, flip;
+  uint32_t norb, npos, pbase, pflip, q, qend;
+  int nb, nn;
+};
+
+PSOCI_HD void conn_begin(ConnIter& it, const ConfView& I, const ConfView& J, uint64_t aa, int ex) {
   const uint64_t C = I.s & J.s, N = J.s & ~I.s;
-  // EX: the number of moved electrons when the caller knows it (1 or 2)
-  const int ex = EX ? EX : moved(I.d, I.s, J.d, J.s);
-  const int nb = popc(I.d) - popc(J.d) + popc(aa);
-  const uint64_t base = aa & C;
+  it.nb = popc(I.d) - popc(J.d) + popc(aa);
+  it.base = aa & C;
   // new open shells (at most four: each ends a moved electron), packed as
   // orbital (6 bits) and bit of J's pattern (4 bits) per entry
-  uint32_t norb = 0, npos = 0;
-  int nn = 0;
-  for (uint64_t r = N; r; r &= r - 1, ++nn) {
+  it.norb = it.npos = 0;
+  it.nn = 0;
+  for (uint64_t r = N; r; r &= r - 1, ++it.nn) {
     const int o = ctz(r);
-    norb |= (uint32_t)o << (6 * nn);
-    npos |= (uint32_t)popc(J.s & ((1ull << o) - 1)) << (4 * nn);
+    it.norb |= (uint32_t)o << (6 * it.nn);
+    it.npos |= (uint32_t)popc(J.s & ((1ull << o) - 1)) << (4 * it.nn);
   }
-  uint32_t pbase = 0;
-  for (uint64_t r = base; r; r &= r - 1) pbase |= 1u << popc(J.s & ((1ull << ctz(r)) - 1));
-  const int need0 = nb - popc(base);
+  it.pbase = 0;
+  for (uint64_t r = it.base; r; r &= r - 1) it.pbase |= 1u << popc(J.s & ((1ull << ctz(r)) - 1));
+  const int need0 = it.nb - popc(it.base);
   // one electron moved: a common open shell may change spin; alpha-to-beta
   // needs one more alpha among the new shells, beta-to-alpha one fewer
-  uint64_t flips = 0;
-  if (ex == 1) flips = (need0 + 1 <= nn ? base : 0ull) | (need0 >= 1 ? (C & ~base) : 0ull);
-  uint64_t flip = 0;
-  uint32_t pflip = 0;
-  int need = need0;
+  it.flips = 0;
+  if (ex == 1) it.flips = (need0 + 1 <= it.nn ? it.base : 0ull) | (need0 >= 1 ? (C & ~it.base) : 0ull);
+  it.flip = 0;
+  it.pflip = 0;
+  it.qend = 1u << it.nn;
+  it.q = (need0 >= 0 && need0 <= it.nn) ? (1u << need0) - 1u : it.qend;
+}
+
+// Next connection; false when the enumeration is exhausted.
+PSOCI_HD bool conn_next(ConnIter& it, const ConfView& J, const int16_t* __restrict__ didxJ, int& b, uint64_t& ab) {
   for (;;) {
-    if (need >= 0 && need <= nn) {
-      uint32_t q = (1u << need) - 1u;
-      const uint32_t qend = 1u << nn;
-      while (q < qend) {
-        uint64_t ab = base ^ flip;
-        uint32_t p = pbase ^ pflip;
-        for (uint32_t u = q; u; u &= u - 1) {
-          const int t = __builtin_ctz(u);
-          ab |= 1ull << ((norb >> (6 * t)) & 63u);
-          p |= 1u << ((npos >> (4 * t)) & 15u);
-        }
-        f((int)didxJ[p], ab);
-        if (!q) break;
-        const uint32_t c = q & (0u - q), r = q + c;
-        q = (((r ^ q) >> 2) >> __builtin_ctz(c)) | r;
+    if (it.q < it.qend) {
+      const uint32_t q = it.q;
+      ab = it.base ^ it.flip;
+      uint32_t p = it.pbase ^ it.pflip;
+      for (uint32_t u = q; u; u &= u - 1) {
+        const int t = __builtin_ctz(u);
+        ab |= 1ull << ((it.norb >> (6 * t)) & 63u);
+        p |= 1u << ((it.npos >> (4 * t)) & 15u);
       }
+      b = (int)didxJ[p];
+      if (!q) {
+        it.q = it.qend;  // the empty subset is the only one of its size
+      } else {
+        const uint32_t c = q & (0u - q), r = q + c;
+        it.q = (((r ^ q) >> 2) >> __builtin_ctz(c)) | r;
+      }
+      return true;
     }
-    if (!flips) break;
-    flip = flips & (~flips + 1);
-    flips &= flips - 1;
-    pflip = 1u << popc(J.s & (flip - 1));
-    need = nb - popc(base ^ flip);
+    if (!it.flips) return false;
+    it.flip = it.flips & (~it.flips + 1);
+    it.flips &= it.flips - 1;
+    it.pflip = 1u << popc(J.s & (it.flip - 1));
+    const int need = it.nb - popc(it.base ^ it.flip);
+    it.q = (need >= 0 && need <= it.nn) ? (1u << need) - 1u : it.qend;
   }
 }
 
-// The diagonal block (J = I): determinant aa itself and its alpha-beta spin
-// exchanges (one alpha and one beta open shell trade spins).
+// Calls f(b, ab) for every connection of determinant aa of I in J != I.
+template <int EX = 0, class F>
+PSOCI_HD void for_each_conn(const ConfView& I, const ConfView& J, uint64_t aa, const int16_t* __restrict__ didxJ,
+                            F&& f) {
+  ConnIter it;
+  conn_begin(it, I, J, aa, EX ? EX : moved(I.d, I.s, J.d, J.s));
+  int b;
+  uint64_t ab;
+  while (conn_next(it, J, didxJ, b, ab)) f(b, ab);
+}
+
+// The diagonal block (J = I): determinant aa itself, then its alpha-beta
+// spin exchanges (one alpha and one beta open shell trade spins).
+struct DiagIter {
+  uint64_t aa, remP, remQ, lp, allQ;
+  uint32_t pa, pp;
+  bool self;
+};
+
+PSOCI_HD void diag_begin(DiagIter& it, const ConfView& I, uint64_t aa) {
+  it.aa = aa;
+  it.pa = 0;
+  for (uint64_t r = aa; r; r &= r - 1) it.pa |= 1u << popc(I.s & ((1ull << ctz(r)) - 1));
+  it.self = true;
+  it.remP = aa;
+  it.allQ = I.s & ~aa;
+  it.remQ = 0;
+  it.lp = 0;
+  it.pp = 0;
+}
+
+PSOCI_HD bool diag_next(DiagIter& it, const ConfView& I, const int16_t* __restrict__ didxI, int& b, uint64_t& ab) {
+  if (it.self) {
+    it.self = false;
+    b = (int)didxI[it.pa];
+    ab = it.aa;
+    return true;
+  }
+  for (;;) {
+    if (it.remQ) {
+      const uint64_t lq = it.remQ & (~it.remQ + 1);
+      it.remQ &= it.remQ - 1;
+      b = (int)didxI[it.pp ^ (1u << popc(I.s & (lq - 1)))];
+      ab = it.aa ^ it.lp ^ lq;
+      return true;
+    }
+    if (!it.remP) return false;
+    it.lp = it.remP & (~it.remP + 1);
+    it.remP &= it.remP - 1;
+    it.pp = it.pa ^ (1u << popc(I.s & (it.lp - 1)));
+    it.remQ = it.allQ;
+  }
+}
+
 template <class F>
 PSOCI_HD void for_each_conn_diag(const ConfView& I, uint64_t aa, const int16_t* __restrict__ didxI, F&& f) {
-  uint32_t pa = 0;
-  for (uint64_t r = aa; r; r &= r - 1) pa |= 1u << popc(I.s & ((1ull << ctz(r)) - 1));
-  f((int)didxI[pa], aa);
-  for (uint64_t p = aa; p; p &= p - 1) {
-    const uint64_t lp = p & (~p + 1);
-    const uint32_t pp = pa ^ (1u << popc(I.s & (lp - 1)));
-    for (uint64_t q = I.s & ~aa; q; q &= q - 1) {
-      const uint64_t lq = q & (~q + 1);
-      f((int)didxI[pp ^ (1u << popc(I.s & (lq - 1)))], aa ^ lp ^ lq);
-    }
-  }
+  DiagIter it;
+  diag_begin(it, I, aa);
+  int b;
+  uint64_t ab;
+  while (diag_next(it, I, didxI, b, ab)) f(b, ab);
 }
 
 // Element D(a, b) between determinant aa of I (row) and ab of a different

@@ -288,6 +288,23 @@ __device__ __forceinline__ ConfView view_of(const ConfRec& r) {
   return v;
 }
 
+// The row configuration of this wavefront (-1: none).  Workgroups are dealt
+// to the 8 XCDs round-robin; with chunk C > 0 each XCD takes runs of C
+// consecutive workgroups of the row order in turn (the grid is a multiple of
+// 8C), so rows that share column configurations meet in the same L2 while
+// the XCDs stay evenly loaded.
+__device__ __forceinline__ int64_t row_of_wave(const DirectArgs& a) {
+  const int wave = threadIdx.x / kWave;
+  int64_t wg = blockIdx.x;
+  if (a.chunk > 0) {
+    const int64_t q = blockIdx.x / 8, x = blockIdx.x % 8;
+    wg = (q / a.chunk) * 8 * a.chunk + x * a.chunk + q % a.chunk;
+  }
+  const int64_t slotw = wg * (blockDim.x / kWave) + wave;
+  if (slotw >= a.nrows) return -1;
+  return a.order ? a.order[a.row0 + slotw] : a.row0 + slotw;
+}
+
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 template <typename T>
@@ -356,19 +373,8 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
   for (int t = threadIdx.x; t < a.ndidx; t += blockDim.x) s_didx[t] = a.didx[t];
   __syncthreads();
   const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
-  // workgroups are dealt to the 8 XCDs round-robin; with chunk C > 0 each
-  // XCD takes runs of C consecutive workgroups of the row order in turn (the
-  // grid is a multiple of 8C), so rows that share column configurations meet
-  // in the same L2 while the XCDs stay evenly loaded
-  int64_t wg = blockIdx.x;
-  if (a.chunk > 0) {
-    const int64_t q = blockIdx.x / 8, x = blockIdx.x % 8;
-    wg = (q / a.chunk) * 8 * a.chunk + x * a.chunk + q % a.chunk;
-  }
-  const int64_t slotw = wg * (blockDim.x / kWave) + wave;
-  if (slotw >= a.nrows) return;  // uniform per wavefront; no block-wide barrier below
-  const int64_t I = a.order ? a.order[a.row0 + slotw] : a.row0 + slotw;
+  const int64_t I = row_of_wave(a);
+  if (I < 0) return;  // uniform per wavefront; no block-wide barrier below
   const ConfRec ri = a.conf[I];
   const ConfView A = view_of(ri);
   const int nI = ri.nsef;
@@ -418,6 +424,7 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
   }
 }
 
+
 DirectArgs direct_args(Ctx& c) {
   DirectArgs a;
   a.ncfg = c.sp.ncfg;
@@ -460,8 +467,9 @@ void launch_direct(Ctx& c, const T* X, int ldx, T* Y, int ldy, int64_t row0 = 0,
 
 template <typename T>
 void direct_any_k(Ctx& c, int k, const T* X, int ldx, T* Y, int ldy) {
-  pad::blocks<T>(c, c.sp.nsef_total, k, 16, X, ldx, Y, ldy, [&](int K, const T* x, int lx, T* y, int ly) {
+  pad::blocks<T>(c, c.sp.nsef_total, k, 32, X, ldx, Y, ldy, [&](int K, const T* x, int lx, T* y, int ly) {
     switch (K) {
+      case 32: launch_direct<T, 32>(c, x, lx, y, ly); break;
       case 16: launch_direct<T, 16>(c, x, lx, y, ly); break;
       case 8: launch_direct<T, 8>(c, x, lx, y, ly); break;
       case 4: launch_direct<T, 4>(c, x, lx, y, ly); break;
