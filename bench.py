@@ -282,6 +282,8 @@ def main():
     if traffic:
         line["roofline"]["traffic_source"] = traffic["source"]
         line["roofline"]["traffic_gbs"] = round(traffic["bytes_per_launch"] / (ms32 * 1e-3) / 1e9, 1)
+    if not direct:
+        line["stored_shape"] = {"fp32": H.stored_shape(api.FP32), "fp64": H.stored_shape(api.FP64)}
     if direct:
         line["direct"] = {"ordered_pairs": npairs, "structural_elements_per_sigma": nelems,
                           "schedule": H.direct_schedule(),

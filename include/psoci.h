@@ -118,6 +118,13 @@ int psoci_direct_info(psoci_ctx* ctx, int64_t* npairs, int64_t* nelems);
  * NULL. */
 int psoci_direct_schedule(psoci_ctx* ctx, int* order, int* chunk, double* tune_ms);
 
+/* The stored sigma's kernel shape for precision prec (PSOCI_FP64 or
+ * PSOCI_FP32) as chosen by the autotuner after psoci_build_h: lanes per
+ * nonzero (1, 2, 4), rows per wavefront (1, 2, 4, 8) and the tuner's best
+ * 16-column time in ms (-1 when fixed by the PSOCI_SPMM_SHAPE environment,
+ * "<lanes>x<rows>").  No reference counterpart; any output may be NULL. */
+int psoci_stored_shape(psoci_ctx* ctx, int prec, int* lanes_per_nz, int* rows_per_wave, double* tune_ms);
+
 /* Diagonal of H in the SEF basis: computeOnlyDiagElements
  * (PsociGAhamiltonian.C:13171-13250).  diag[n] host. */
 int psoci_diag(psoci_ctx* ctx, double* diag);

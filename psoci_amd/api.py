@@ -91,6 +91,7 @@ def load(path: str | None = None):
         "psoci_project": (I, [V, I64, I, I, V, I, V, I, V]),
         "psoci_direct_info": (I, [V, V, V]),
         "psoci_direct_schedule": (I, [V, V, V, V]),
+        "psoci_stored_shape": (I, [V, I, V, V, V]),
         "psoci_sync": (I, [V]),
     }
     for name, (res, args) in sig.items():
@@ -256,6 +257,12 @@ class Hamiltonian:
         o, ch, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         self._check(self.L.psoci_direct_schedule(self._h, ctypes.byref(o), ctypes.byref(ch), ctypes.byref(ms)))
         return {"order": o.value, "chunk": ch.value, "tune_ms": ms.value}
+
+    def stored_shape(self, prec: int = FP32):
+        """The stored sigma's autotuned shape: dict(lanes_per_nz, rows_per_wave, tune_ms)."""
+        g, r, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._check(self.L.psoci_stored_shape(self._h, prec, ctypes.byref(g), ctypes.byref(r), ctypes.byref(ms)))
+        return {"lanes_per_nz": g.value, "rows_per_wave": r.value, "tune_ms": ms.value}
 
     def stream_handle(self) -> int:
         """The hipStream_t the library launches on (as an integer)."""

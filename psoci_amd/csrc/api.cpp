@@ -176,6 +176,18 @@ int psoci_direct_schedule(psoci_ctx* ctx, int* order, int* chunk, double* tune_m
   });
 }
 
+int psoci_stored_shape(psoci_ctx* ctx, int prec, int* lanes_per_nz, int* rows_per_wave, double* tune_ms) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_h, PSOCI_ERR_STATE, "stored sigma needs psoci_build_h first");
+    PSOCI_CHECK(prec == PSOCI_FP64 || prec == PSOCI_FP32, PSOCI_ERR_ARG, "prec must be PSOCI_FP64 or PSOCI_FP32");
+    const int pi = prec == PSOCI_FP32 ? 1 : 0;
+    if (lanes_per_nz) *lanes_per_nz = c.spmm_g[pi];
+    if (rows_per_wave) *rows_per_wave = c.spmm_r[pi];
+    if (tune_ms) *tune_ms = c.spmm_tune_ms[pi];
+    return PSOCI_OK;
+  });
+}
+
 int psoci_diag(psoci_ctx* ctx, double* diag) {
   return guarded(ctx, [&](Ctx& c) {
     PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");

@@ -34,6 +34,10 @@ struct Ctx {
   DevBuf<double> d_val;       // full FP64
   DevBuf<float> d_val32;      // full FP32
   DevBuf<int32_t> d_lowcnt;   // per row: entries of the lower part (incl. diagonal)
+  // stored sigma shape per precision ([0] FP64, [1] FP32): lanes per
+  // nonzero, rows per wavefront, and the tuner's best 16-column time (-1: fixed)
+  int spmm_g[2] = {1, 1}, spmm_r[2] = {1, 1};
+  double spmm_tune_ms[2] = {-1, -1};
   DevBuf<double> d_diag;
   bool have_diag = false;
 
@@ -75,6 +79,7 @@ struct Ctx {
 void upload_space(Ctx& c);
 void build_pairs(Ctx& c);               // adjacency + determinant masks (direct sigma, adjoint)
 void build_h(Ctx& c);                   // stored Hamiltonian
+void tune_stored(Ctx& c);               // stored sigma shape autotuner (after build_h)
 void compute_diag(Ctx& c);              // diagonal only
 void block_on_device(Ctx& c, int64_t I, int64_t J, double* host_out, int* status);
 void sigma_stored(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy);

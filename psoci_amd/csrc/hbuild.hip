@@ -454,6 +454,7 @@ void build_h(Ctx& c) {
   c.nnz_lower = (nnz + n) / 2;
   c.sync();
   c.have_h = true;
+  tune_stored(c);
   c.have_diag = true;
 }
 

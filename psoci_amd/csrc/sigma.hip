@@ -8,6 +8,10 @@
 // [n][k] row-major (the reference's GA vector layout), and one wavefront
 // reduces one row for all k vectors at once: the row's values and columns
 // stream from HBM once per launch, X rows are gathered from L2/MALL.
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+
 #include "ctx.hpp"
 #include "pad.hpp"
 
@@ -28,42 +32,86 @@ __device__ __forceinline__ float fmadd<float>(float a, float b, float c) {
   return __builtin_fmaf(a, b, c);
 }
 
-// one wavefront per row; K vectors; rows-per-block = blockDim / 64
-template <typename T, int K>
+// R rows per wavefront (64/R lanes per row: short rows share a wavefront)
+// and G lanes per nonzero (each of them multiplies K/G of the K columns, so
+// the G lanes of a nonzero read one contiguous piece of its x row).  Lanes
+// of a row stride over its nonzeros; partial sums are reduced across the
+// nonzero slots of the row with shuffles.  (R, G) = (1, 1) is one wavefront
+// per row with whole x rows per lane.
+template <typename T, int K, int G, int R>
 __global__ __launch_bounds__(256) void k_spmm(int64_t n, const int64_t* __restrict__ rowptr,
                                               const int32_t* __restrict__ col, const T* __restrict__ val,
                                               const T* __restrict__ X, int ldx, T* __restrict__ Y, int ldy) {
+  constexpr int LR = kWave / R;   // lanes per row
+  constexpr int NS = LR / G;      // nonzero slots per row
+  constexpr int C = K / G;        // columns per lane
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t r = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-  if (r >= n) return;
-  T acc[K];
+  const int64_t r = ((int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * R + lane / LR;
+  const int lr = lane % LR, slot = lr / G, cc = lr % G;
+  T acc[C];
 #pragma unroll
-  for (int v = 0; v < K; ++v) acc[v] = (T)0;
-  const int64_t lo = rowptr[r], hi = rowptr[r + 1];
-  for (int64_t q = lo + lane; q < hi; q += kWave) {
-    const T a = val[q];
-    const T* __restrict__ x = X + (int64_t)col[q] * ldx;
+  for (int v = 0; v < C; ++v) acc[v] = (T)0;
+  if (r < n) {
+    const int64_t lo = rowptr[r], hi = rowptr[r + 1];
+    for (int64_t q = lo + slot; q < hi; q += NS) {
+      const T a = val[q];
+      const T* __restrict__ x = X + (int64_t)col[q] * ldx + cc * C;
 #pragma unroll
-    for (int v = 0; v < K; ++v) acc[v] = fmadd(a, x[v], acc[v]);
+      for (int v = 0; v < C; ++v) acc[v] = fmadd(a, x[v], acc[v]);
+    }
   }
+  // reduce over the nonzero slots of the row (lane bits G .. LR-1)
 #pragma unroll
-  for (int v = 0; v < K; ++v) {
+  for (int off = G; off < LR; off <<= 1) {
 #pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) acc[v] += __shfl_xor(acc[v], off, kWave);
+    for (int v = 0; v < C; ++v) acc[v] += __shfl_xor(acc[v], off, kWave);
   }
-  T* y = Y + r * ldy;
+  if (r < n && slot == 0) {
+    T* y = Y + r * ldy + cc * C;
 #pragma unroll
-  for (int v = 0; v < K; ++v)
-    if (lane == v) y[v] = acc[v];
+    for (int v = 0; v < C; ++v) y[v] = acc[v];
+  }
 }
 
+template <typename T, int K, int G, int R>
+void launch_gr(Ctx& c, const T* val, const T* X, int ldx, T* Y, int ldy) {
+  const int64_t n = c.sp.nsef_total;
+  const int64_t rows_per_block = 4 * R;
+  hipLaunchKernelGGL((k_spmm<T, K, G, R>), dim3((unsigned)((n + rows_per_block - 1) / rows_per_block)), dim3(256), 0,
+                     c.stream, n, c.d_rowptr.p, c.d_col.p, val, X, ldx, Y, ldy);
+  PSOCI_HIP(hipGetLastError());
+}
+
+// the tuned shape of this precision, where the width allows it
 template <typename T, int K>
 void launch_spmm(Ctx& c, const T* val, const T* X, int ldx, T* Y, int ldy) {
-  const int64_t n = c.sp.nsef_total;
-  const int wpb = 4;
-  hipLaunchKernelGGL((k_spmm<T, K>), dim3((unsigned)((n + wpb - 1) / wpb)), dim3(256), 0, c.stream, n, c.d_rowptr.p,
-                     c.d_col.p, val, X, ldx, Y, ldy);
-  PSOCI_HIP(hipGetLastError());
+  const int pi = sizeof(T) == 4 ? 1 : 0;
+  const int g = K >= c.spmm_g[pi] ? c.spmm_g[pi] : 1;
+  const int key = g * 16 + c.spmm_r[pi];
+  switch (key) {
+#define PSOCI_SPMM_CASE(GG, RR)                                     \
+  case GG * 16 + RR:                                                \
+    if constexpr (K >= GG) {                                        \
+      launch_gr<T, K, (K >= GG ? GG : 1), RR>(c, val, X, ldx, Y, ldy); \
+      return;                                                       \
+    }                                                               \
+    break;
+    PSOCI_SPMM_CASE(1, 1)
+    PSOCI_SPMM_CASE(1, 2)
+    PSOCI_SPMM_CASE(1, 4)
+    PSOCI_SPMM_CASE(1, 8)
+    PSOCI_SPMM_CASE(2, 1)
+    PSOCI_SPMM_CASE(2, 2)
+    PSOCI_SPMM_CASE(2, 4)
+    PSOCI_SPMM_CASE(2, 8)
+    PSOCI_SPMM_CASE(4, 1)
+    PSOCI_SPMM_CASE(4, 2)
+    PSOCI_SPMM_CASE(4, 4)
+    PSOCI_SPMM_CASE(4, 8)
+#undef PSOCI_SPMM_CASE
+    default: break;
+  }
+  launch_gr<T, K, 1, 1>(c, val, X, ldx, Y, ldy);
 }
 
 template <typename T>
@@ -81,6 +129,67 @@ void spmm_any_k(Ctx& c, const T* val, int k, const T* X, int ldx, T* Y, int ldy)
 }
 
 }  // namespace
+
+// Shape autotuner of the stored sigma: for each precision, every (lanes per
+// nonzero, rows per wavefront) pair times the 16-column product on the built
+// matrix (best of two after a warm-up) and the fastest is kept for all
+// widths.  PSOCI_SPMM_SHAPE=<G>x<R> fixes it instead.
+template <typename T>
+double tune_prec(Ctx& c, const T* val, hipEvent_t e0, hipEvent_t e1) {
+  const int pi = sizeof(T) == 4 ? 1 : 0;
+  const int64_t n = c.sp.nsef_total;
+  const int K = 16;
+  DevBuf<T> X, Y;
+  X.alloc((size_t)n * K);
+  Y.alloc((size_t)n * K);
+  PSOCI_HIP(hipMemsetAsync(X.p, 0, (size_t)n * K * sizeof(T), c.stream));
+  double best = 1e300;
+  int bg = 1, br = 1;
+  for (int g : {1, 2, 4})
+    for (int r : {1, 2, 4, 8}) {
+      c.spmm_g[pi] = g;
+      c.spmm_r[pi] = r;
+      double t = 1e300;
+      for (int rep = 0; rep < 3; ++rep) {
+        PSOCI_HIP(hipEventRecord(e0, c.stream));
+        launch_spmm<T, 16>(c, val, X.p, K, Y.p, K);
+        PSOCI_HIP(hipEventRecord(e1, c.stream));
+        PSOCI_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PSOCI_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (rep > 0) t = std::min(t, (double)ms);
+      }
+      if (t < best) {
+        best = t;
+        bg = g;
+        br = r;
+      }
+    }
+  c.spmm_g[pi] = bg;
+  c.spmm_r[pi] = br;
+  return best;
+}
+
+void tune_stored(Ctx& c) {
+  const char* env = std::getenv("PSOCI_SPMM_SHAPE");
+  if (env) {
+    int g = 1, r = 1;
+    if (std::sscanf(env, "%dx%d", &g, &r) == 2 && (g == 1 || g == 2 || g == 4) &&
+        (r == 1 || r == 2 || r == 4 || r == 8)) {
+      c.spmm_g[0] = c.spmm_g[1] = g;
+      c.spmm_r[0] = c.spmm_r[1] = r;
+    }
+    c.spmm_tune_ms[0] = c.spmm_tune_ms[1] = -1.0;
+    return;
+  }
+  hipEvent_t e0, e1;
+  PSOCI_HIP(hipEventCreate(&e0));
+  PSOCI_HIP(hipEventCreate(&e1));
+  c.spmm_tune_ms[1] = tune_prec<float>(c, c.d_val32.p, e0, e1);
+  c.spmm_tune_ms[0] = tune_prec<double>(c, c.d_val.p, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+}
 
 void sigma_stored(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy) {
   PSOCI_CHECK(c.have_h, PSOCI_ERR_STATE, "stored sigma needs psoci_build_h first");
