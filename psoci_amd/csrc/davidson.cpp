@@ -28,58 +28,9 @@
 
 #include "ctx.hpp"
 #include "linalg.hpp"
+#include "smalleig.hpp"
 
 namespace psoci {
-
-bool sym_eig(int m, double* a, double* w) {
-  // cyclic Jacobi with threshold; a (m x m row-major) -> eigenvectors in columns
-  std::vector<double> v((size_t)m * m, 0.0), s(a, a + (size_t)m * m);
-  for (int i = 0; i < m; ++i) v[(size_t)i * m + i] = 1.0;
-  for (int sweep = 0; sweep < 60; ++sweep) {
-    double off = 0.0, tot = 0.0;
-    for (int p = 0; p < m; ++p)
-      for (int q = 0; q < m; ++q) {
-        double x = s[(size_t)p * m + q] * s[(size_t)p * m + q];
-        tot += x;
-        if (p != q) off += x;
-      }
-    if (off <= 1e-32 * (tot > 0 ? tot : 1.0)) break;
-    for (int p = 0; p < m - 1; ++p)
-      for (int q = p + 1; q < m; ++q) {
-        double apq = s[(size_t)p * m + q];
-        if (apq == 0.0) continue;
-        double app = s[(size_t)p * m + p], aqq = s[(size_t)q * m + q];
-        double tau = (aqq - app) / (2.0 * apq);
-        double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
-        double cs = 1.0 / std::sqrt(1.0 + t * t), sn = t * cs;
-        for (int r = 0; r < m; ++r) {
-          double srp = s[(size_t)r * m + p], srq = s[(size_t)r * m + q];
-          s[(size_t)r * m + p] = cs * srp - sn * srq;
-          s[(size_t)r * m + q] = sn * srp + cs * srq;
-        }
-        for (int r = 0; r < m; ++r) {
-          double spr = s[(size_t)p * m + r], sqr = s[(size_t)q * m + r];
-          s[(size_t)p * m + r] = cs * spr - sn * sqr;
-          s[(size_t)q * m + r] = sn * spr + cs * sqr;
-        }
-        for (int r = 0; r < m; ++r) {
-          double vrp = v[(size_t)r * m + p], vrq = v[(size_t)r * m + q];
-          v[(size_t)r * m + p] = cs * vrp - sn * vrq;
-          v[(size_t)r * m + q] = sn * vrp + cs * vrq;
-        }
-      }
-  }
-  std::vector<int> ord(m);
-  std::iota(ord.begin(), ord.end(), 0);
-  std::sort(ord.begin(), ord.end(), [&](int x, int y) { return s[(size_t)x * m + x] < s[(size_t)y * m + y]; });
-  for (int c = 0; c < m; ++c) {
-    w[c] = s[(size_t)ord[c] * m + ord[c]];
-    for (int r = 0; r < m; ++r) a[(size_t)r * m + c] = v[(size_t)r * m + ord[c]];
-  }
-  for (int c = 0; c < m; ++c)
-    if (!std::isfinite(w[c])) return false;
-  return true;
-}
 
 namespace {
 

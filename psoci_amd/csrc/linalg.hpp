@@ -29,6 +29,5 @@ struct Linalg {
 // Symmetric eigen-decomposition of a small dense matrix (host, FP64):
 // Householder tridiagonalisation + implicit QL.  a is m x m row-major and is
 // overwritten by the eigenvectors (column c = eigenvector c); w ascending.
-bool sym_eig(int m, double* a, double* w);
 
 }  // namespace psoci

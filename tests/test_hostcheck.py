@@ -20,7 +20,8 @@ OUT = os.path.join(ROOT, "tests", "hostcheck", "_build", "libhostcheck.so")
 @pytest.fixture(scope="module")
 def hc():
     deps = [SRC, os.path.join(ROOT, "psoci_amd", "csrc", "space.cpp"), os.path.join(ROOT, "psoci_amd", "csrc", "elem.hpp"),
-            os.path.join(ROOT, "psoci_amd", "csrc", "common.hpp"), os.path.join(ROOT, "psoci_amd", "csrc", "detdrive.hpp")]
+            os.path.join(ROOT, "psoci_amd", "csrc", "common.hpp"), os.path.join(ROOT, "psoci_amd", "csrc", "detdrive.hpp"),
+            os.path.join(ROOT, "psoci_amd", "csrc", "smalleig.hpp")]
     if not os.path.exists(OUT) or os.path.getmtime(OUT) < max(os.path.getmtime(d) for d in deps):
         os.makedirs(os.path.dirname(OUT), exist_ok=True)
         if not os.path.exists("/opt/rocm/bin/hipcc"):
@@ -138,3 +139,30 @@ def test_determinant_driven_rows_fci10_mrci(hc, fci10):
     cfg = inputs.mrcisd_configs(4, 6, 4)
     ints = inputs.make_integrals(10, 99, n_internal=4)
     _rows_agree(hc, cfg, ints, np.random.default_rng(4).choice(cfg.ncfg, 30, replace=False))
+
+
+def test_subspace_eigensolver_vs_lapack(hc):
+    """The Davidson subspace eigensolver (Householder + implicit QR) against
+    LAPACK, including degenerate and clustered spectra."""
+    hc.hc_sym_eig.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    rng = np.random.default_rng(8)
+    cases = []
+    for m in (1, 2, 3, 7, 16, 48, 80, 160):
+        A = rng.standard_normal((m, m))
+        cases.append((A + A.T) / 2)
+    # degenerate pairs and a cluster
+    Q, _ = np.linalg.qr(rng.standard_normal((40, 40)))
+    d = np.concatenate([np.repeat([1.0, 2.0, 3.0], 2), 5 + 1e-9 * np.arange(6), np.linspace(6, 9, 28)])
+    cases.append(Q @ np.diag(d) @ Q.T)
+    cases.append(np.diag(rng.standard_normal(12)))
+    for A in cases:
+        m = A.shape[0]
+        a = np.ascontiguousarray(A.copy())
+        w = np.zeros(m)
+        assert hc.hc_sym_eig(m, _p(a), _p(w)) == 1
+        ref = np.linalg.eigvalsh(A)
+        scale = max(1.0, np.abs(ref).max())
+        assert np.abs(w - ref).max() <= 1e-13 * scale * m, m
+        V = a
+        assert np.abs(V.T @ V - np.eye(m)).max() < 1e-12 * m
+        assert np.abs(A @ V - V * w).max() < 1e-12 * scale * m
