@@ -104,3 +104,30 @@ def test_direct_mixed_davidson_mrci8x24():
     R = H.sigma(V, api.DIRECT, api.FP64) - V * a["evals"]
     assert np.linalg.norm(R, axis=0).max() < 1e-4
     H.close()
+
+
+def test_direct_full_size_mrci10x24():
+    """The 47.6M-determinant workload at full size: sampled rows of the FP64
+    and FP32 direct sigma against the oracle's blocks, and the symmetry of
+    the operator (u^T H v = v^T H u) over all 11,894,370 SEFs."""
+    cfg, ints, _ = inputs.workload("mrci10x24")
+    H = api.Hamiltonian(cfg, ints)
+    sp = oracle.OracleSpace(cfg, ints)
+    rng = np.random.default_rng(17)
+    X = rng.standard_normal((H.n, 4))
+    Y = H.sigma(X, api.DIRECT, api.FP64)
+    Y32 = H.sigma(X.astype(np.float32), api.DIRECT, api.FP32)
+    nopen = (cfg.occ == 1).sum(1)
+    rows = [int(r) for r in rng.choice(cfg.ncfg, 5, replace=False)] + [int(np.where(nopen == 8)[0][7])]
+    rows += [0, cfg.ncfg - 1]
+    for I in rows:
+        ref = oracle_rows(sp, cfg, I, X)
+        o = int(sp.sef_off[I])
+        got = Y[o:o + int(sp.nsef[I])]
+        scale = max(1.0, np.abs(ref).max())
+        assert np.abs(got - ref).max() <= 1e-12 * scale, I
+        assert np.abs(Y32[o:o + int(sp.nsef[I])] - ref).max() <= 5e-5 * scale, I
+    u = X[:, 0].copy()
+    v = X[:, 1].copy()
+    assert abs(v @ Y[:, 0] - u @ Y[:, 1]) <= 1e-11 * abs(v @ Y[:, 0])
+    H.close()
