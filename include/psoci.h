@@ -43,6 +43,8 @@ enum { PSOCI_SIGMA_STORED = 0, PSOCI_SIGMA_DIRECT = 1 };
  * drivers (driver.PSOCI.splittest.C:280-760). */
 int psoci_ctx_create(int device, psoci_ctx** out);
 int psoci_ctx_destroy(psoci_ctx* ctx);
+/* ctx NULL: the message of the last failed context-free call
+ * (psoci_sifs_read / psoci_sifs_write) on this thread. */
 const char* psoci_last_error(const psoci_ctx* ctx);
 
 /* Integrals after the SIFS read: replaces PsociIntegrals::fetchAndProcessIntegrals
@@ -52,6 +54,28 @@ const char* psoci_last_error(const psoci_ctx* ctx);
  * valtwo[NNDAF(NNDAF(nbf))]: (pq|rs) at the canonical 8-fold index of
  * PsociIntegrals::canonicalIndex (PsociIntegrals.C:892-901) minus one. */
 int psoci_set_integrals(psoci_ctx* ctx, int nbf, const double* valone, const double* valtwo, double ecore);
+
+/* SIFS-style integral files (the reference reads the Columbus moints file:
+ * PsociIntegrals::fetchAndProcessIntegrals, PsociIntegrals.C:914-961, with
+ * sifrh1/sifrh2/sifrd1/sifrd2).  The record layout is this library's own (the
+ * Columbus library is not available as source); the header fields and the
+ * processing follow the reference: 1-e records of itypea 0, itypeb != 0 are
+ * summed (T + V + ...; the overlap is skipped), itypea 2 (spin-orbit x, y,
+ * z) overwrite with the antisymmetric sign, 2-e values go to their canonical
+ * slot (last write wins), the core energy is energy[0].  Context free; on
+ * failure psoci_last_error(NULL) has the message.
+ *
+ * psoci_sifs_read: nbf always; valone [NNDAF(nbf)], valtwo
+ * [NNDAF(NNDAF(nbf))] and ecore when not NULL (pass NULLs to query nbf).
+ * psoci_sifs_write: s1 (overlap), t1, v1 (packed like valone; s1 and one of
+ * t1 / v1 may be NULL), so [3][NNDAF(nbf)] or NULL (only p > q entries are
+ * written), valtwo, ecore, title (NULL: a default). */
+int psoci_sifs_read(const char* path, int* nbf, double* valone, double* valtwo, double* ecore);
+int psoci_sifs_write(const char* path, int nbf, const double* s1, const double* t1, const double* v1,
+                     const double* so, const double* valtwo, double ecore, const char* title);
+
+/* psoci_sifs_read + psoci_set_integrals. */
+int psoci_set_integrals_sifs(psoci_ctx* ctx, const char* path);
 
 /* Configuration list in file order: replaces PsociConfigs::readConfigsToGA
  * (PsociConfigs.C:336-495) plus the determinant/SEF generation of

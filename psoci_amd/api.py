@@ -17,6 +17,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
+from .inputs import Integrals
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpsoci_amd.so")
 HEADER = os.path.join(HERE, "..", "include", "psoci.h")
@@ -77,6 +79,9 @@ def load(path: str | None = None):
         "psoci_ctx_destroy": (I, [V]),
         "psoci_last_error": (ctypes.c_char_p, [V]),
         "psoci_set_integrals": (I, [V, I, V, V, D]),
+        "psoci_set_integrals_sifs": (I, [V, ctypes.c_char_p]),
+        "psoci_sifs_read": (I, [ctypes.c_char_p, V, V, V, V]),
+        "psoci_sifs_write": (I, [ctypes.c_char_p, I, V, V, V, V, V, D, ctypes.c_char_p]),
         "psoci_set_configs": (I, [V, I, I, I, I64, V, V]),
         "psoci_space_info": (I, [V, V, V, V, V]),
         "psoci_dets": (I, [V, I64, V, V, V, V]),
@@ -159,6 +164,13 @@ class Hamiltonian:
         two = np.ascontiguousarray(ints.valtwo, np.float64)
         self._check(self.L.psoci_set_integrals(self._h, ints.nbf, _p(one), _p(two), float(ints.ecore)))
         self.ints = ints
+        self.nnz_lower = None
+
+    def set_integrals_sifs(self, path: str):
+        """Integrals from a SIFS-style file (psoci_set_integrals_sifs)."""
+        self._check(self.L.psoci_set_integrals_sifs(self._h, os.fsencode(path)))
+        nbf, one, two, ecore = sifs_read(path)
+        self.ints = Integrals(nbf=nbf, valone=one, valtwo=two, ecore=ecore)
         self.nnz_lower = None
 
     def dets(self, I: int):
@@ -284,3 +296,37 @@ class Hamiltonian:
         gso = np.zeros(n1)
         self._check(self.L.psoci_rdm(self._h, k, _p(C), _p(w), _p(g1), _p(g2), _p(gso)))
         return g1, g2, gso
+
+
+def _ctxfree(rc: int):
+    if rc != OK:
+        msg = load().psoci_last_error(None)
+        raise PsociError(rc, msg.decode() if msg else "")
+
+
+def sifs_read(path: str):
+    """(nbf, valone, valtwo, ecore) from a SIFS-style integral file, processed
+    as the reference does (T + V summed, overlap skipped, spin-orbit records
+    overwrite antisymmetrically, core energy = energy[0])."""
+    L = load()
+    nbf = ctypes.c_int()
+    _ctxfree(L.psoci_sifs_read(os.fsencode(path), ctypes.byref(nbf), None, None, None))
+    n = nbf.value
+    one = np.zeros(n * (n + 1) // 2)
+    two = np.zeros(len(one) * (len(one) + 1) // 2)
+    ecore = ctypes.c_double()
+    _ctxfree(L.psoci_sifs_read(os.fsencode(path), ctypes.byref(nbf), _p(one), _p(two), ctypes.byref(ecore)))
+    return n, one, two, ecore.value
+
+
+def sifs_write(path: str, nbf: int, valtwo, t1=None, v1=None, s1=None, so=None, ecore: float = 0.0,
+               title: str | None = None):
+    """Write a SIFS-style integral file: overlap s1, kinetic t1 and potential
+    v1 one-electron blocks (packed like valone; the reader sums t1 + v1),
+    optional spin-orbit so [3][NNDAF(nbf)], two-electron valtwo."""
+    L = load()
+    arr = lambda a: None if a is None else np.ascontiguousarray(a, np.float64)  # noqa: E731
+    t1, v1, s1, so, two = arr(t1), arr(v1), arr(s1), arr(so), arr(valtwo)
+    ptr = lambda a: None if a is None else _p(a)  # noqa: E731
+    _ctxfree(L.psoci_sifs_write(os.fsencode(path), nbf, ptr(s1), ptr(t1), ptr(v1), ptr(so), _p(two), float(ecore),
+                                None if title is None else title.encode()))

@@ -61,7 +61,21 @@ int psoci_ctx_destroy(psoci_ctx* ctx) {
   return PSOCI_OK;
 }
 
-const char* psoci_last_error(const psoci_ctx* ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+const char* psoci_last_error(const psoci_ctx* ctx) {
+  return ctx ? ctx->c.err.c_str() : psoci::ctxfree_error().c_str();
+}
+
+int psoci_set_integrals_sifs(psoci_ctx* ctx, const char* path) {
+  return guarded(ctx, [&](Ctx& c) {
+    int nbf = 0;
+    psoci::read_sifs(path, &nbf, nullptr, nullptr, nullptr);
+    std::vector<double> one(nndaf(nbf)), two(nndaf(nndaf(nbf)));
+    double ecore = 0.0;
+    psoci::read_sifs(path, &nbf, one.data(), two.data(), &ecore);
+    (void)c;
+    return psoci_set_integrals(ctx, nbf, one.data(), two.data(), ecore);
+  });
+}
 
 int psoci_set_integrals(psoci_ctx* ctx, int nbf, const double* valone, const double* valtwo, double ecore) {
   return guarded(ctx, [&](Ctx& c) {

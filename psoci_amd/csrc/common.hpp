@@ -91,6 +91,11 @@ struct Space {
 // Builds Space from a configuration list; throws Error on unsupported input.
 void build_space(Space& sp, int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, const int32_t* jsym);
 
+// SIFS-style integral files (sifs.cpp); errors of the context-free entry
+// points are kept per thread for psoci_last_error(NULL).
+void read_sifs(const char* path, int* nbf, double* one, double* two, double* ecore);
+const std::string& ctxfree_error();
+
 // Library-wide kernel launch helpers (defined in the .hip files).
 void exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t n, hipStream_t s, void*& tmp, size_t& tmp_bytes);
 

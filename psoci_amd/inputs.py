@@ -285,3 +285,23 @@ def count_space(cfg: Configs):
 __all__ = ["Integrals", "Configs", "make_integrals", "fci_configs", "mrcisd_configs", "write_configs",
            "read_configs", "ConfigFileError", "workload", "WORKLOADS", "count_space", "nndaf", "pair_index",
            "canonical_index", "comb"]
+
+def write_sifs(path: str, ints: Integrals, title: str = "synthetic integrals") -> None:
+    """Write integrals as a SIFS-style file through the library: an overlap
+    record (identity; the reader skips it), the one-electron integrals as the
+    kinetic block with an all-zero potential block (the reader sums them back
+    to valone exactly), no spin-orbit records (with one configuration symmetry
+    the reference model never uses them), and the two-electron integrals."""
+    from .api import sifs_write
+    n1 = nndaf(ints.nbf)
+    s1 = np.zeros(n1)
+    for p in range(ints.nbf):
+        s1[pair_index(p, p)] = 1.0
+    sifs_write(path, ints.nbf, ints.valtwo, t1=ints.valone, v1=np.zeros(n1), s1=s1, ecore=ints.ecore, title=title)
+
+
+def read_sifs(path: str) -> Integrals:
+    """Integrals from a SIFS-style file, processed as the reference does."""
+    from .api import sifs_read
+    nbf, one, two, ecore = sifs_read(path)
+    return Integrals(nbf=nbf, valone=one, valtwo=two, ecore=ecore)

@@ -59,3 +59,21 @@ def test_dets_match_fixture():
         assert [int(x) for x in d["beta"]] == want_b, s
         assert [int(x) for x in d["phase"]] == rec["phase"], s
         H.close()
+
+
+def test_sifs_integrals_path(fci6, tmp_path):
+    """Integrals through a SIFS-style file give the same Hamiltonian as the
+    arrays (psoci_set_integrals_sifs vs psoci_set_integrals)."""
+    cfg, ints, _ = fci6
+    path = str(tmp_path / "fci6.sifs")
+    inputs.write_sifs(path, ints)
+    A = api.Hamiltonian(cfg, ints)
+    B = api.Hamiltonian(cfg, ints)
+    B.set_integrals_sifs(path)
+    A.build()
+    B.build()
+    ra, ca, va, da = A.get_h()
+    rb, cb, vb, db = B.get_h()
+    assert np.array_equal(ra, rb) and np.array_equal(ca, cb) and np.array_equal(va, vb) and np.array_equal(da, db)
+    A.close()
+    B.close()
