@@ -43,8 +43,9 @@ enum { PSOCI_SIGMA_STORED = 0, PSOCI_SIGMA_DIRECT = 1 };
  * drivers (driver.PSOCI.splittest.C:280-760). */
 int psoci_ctx_create(int device, psoci_ctx** out);
 int psoci_ctx_destroy(psoci_ctx* ctx);
-/* ctx NULL: the message of the last failed context-free call
- * (psoci_sifs_read / psoci_sifs_write) on this thread. */
+/* ctx NULL: the message of the last failure on this thread of a
+ * context-free call (psoci_sifs_read / psoci_sifs_write) or of a call made
+ * with a NULL context ("null context"). */
 const char* psoci_last_error(const psoci_ctx* ctx);
 
 /* Integrals after the SIFS read: replaces PsociIntegrals::fetchAndProcessIntegrals

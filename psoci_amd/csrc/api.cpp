@@ -17,7 +17,10 @@ namespace {
 
 template <class F>
 int guarded(psoci_ctx* ctx, F&& f) {
-  if (!ctx) return PSOCI_ERR_ARG;
+  if (!ctx) {
+    psoci::set_ctxfree_error("null context");
+    return PSOCI_ERR_ARG;
+  }
   try {
     PSOCI_HIP(hipSetDevice(ctx->c.device));
     ctx->c.err.clear();

@@ -95,6 +95,7 @@ void build_space(Space& sp, int nbf, int nelec, int ksym, int64_t ncfg, const ui
 // points are kept per thread for psoci_last_error(NULL).
 void read_sifs(const char* path, int* nbf, double* one, double* two, double* ecore);
 const std::string& ctxfree_error();
+void set_ctxfree_error(const std::string& m);
 
 // Library-wide kernel launch helpers (defined in the .hip files).
 void exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t n, hipStream_t s, void*& tmp, size_t& tmp_bytes);

@@ -289,6 +289,7 @@ int ctxfree(F&& f) {
 }  // namespace
 
 const std::string& ctxfree_error() { return g_err; }
+void set_ctxfree_error(const std::string& m) { g_err = m; }
 
 void read_sifs(const char* path, int* nbf, double* one, double* two, double* ecore) {
   read_file(path, nbf, one, two, ecore);
