@@ -120,6 +120,19 @@ int psoci_build_h(psoci_ctx* ctx, int64_t* nnz_lower);
  * fetchLocalHblock (PsociGAhamiltonian.C:11791-11846) and fetchDIAG_SEF. */
 int psoci_get_h(psoci_ctx* ctx, int64_t* rowptr, int32_t* col, double* val, double* diag);
 
+/* Stored-Hamiltonian checkpoint: replaces the DRA dump and restart of the
+ * ELL arrays cimat / icicol / number / diag (PsociConstructHamiltonian.C:
+ * 56-700, driver.PSOCI.splittest.C:454-457, 690-760, 848-858,
+ * --hamiltonian_restart).  The
+ * file holds the lower triangle + diagonal in the reference's row layout
+ * (as psoci_get_h returns it), fingerprints of the configuration list and
+ * the integrals, the tuned stored-sigma shapes and a payload checksum.
+ * psoci_h_load needs psoci_set_configs and psoci_set_integrals with the same
+ * inputs as the writer (PSOCI_ERR_ARG otherwise, and for a truncated or
+ * corrupted file; such failures leave the context as it was) and then
+ * stands in for psoci_build_h: *nnz_lower as there. */
+int psoci_h_save(psoci_ctx* ctx, const char* path);
+int psoci_h_load(psoci_ctx* ctx, const char* path, int64_t* nnz_lower);
 /* Sigma Y = H X for k vectors: matrixVectorProductsSplitChunkMultipleJchunks
  * (PsociGAhamiltonian.C:6921-7269; mode STORED) and the direct
  * ...DirectFetch path (:7549-7823; mode DIRECT, elements regenerated on the
@@ -190,6 +203,26 @@ typedef struct {
 int psoci_davidson(psoci_ctx* ctx, const psoci_dav_params* params, double* evals, double* evecs,
                    psoci_dav_stats* stats);
 
+/* CI-vector restart files in the reference's binary vector format
+ * (PsociVector.C:530-553, 627-643, 685-693; written by
+ * PsociGAvectors::dumpVectorScalable, PsociGAvectors.C:207-325, at the end
+ * of the driver, driver.PSOCI.splittest.C:1354-1358; read by
+ * readVectorScalable + specifyDiagonalGuessVectors, PsociGAvectors.C:539-659,
+ * 687-726, for --vector_restart, driver.PSOCI.splittest.C:962-971):
+ * int64 nbasis, int64 nroots, int32 width, int32 prec, then per root an
+ * int64 root number and nbasis doubles, then the title's bytes.  Context
+ * free; on failure psoci_last_error(NULL) has the message.
+ *
+ * psoci_civec_write: C [n][k] row-major (k roots), title NULL: "default".
+ * psoci_civec_info: nbasis, nroots and the title (NUL-terminated, at most
+ * title_cap - 1 bytes) of a file; any output may be NULL.
+ * psoci_civec_read: n must equal the file's nbasis; fills C [n][k] with the
+ * first min(nroots, k) roots and completes columns j >= *nread with unit
+ * vectors e_j (requires k <= n).  The result is a guess for psoci_davidson
+ * with guess = 2. */
+int psoci_civec_write(const char* path, int64_t n, int k, const double* C, const char* title);
+int psoci_civec_info(const char* path, int64_t* nbasis, int64_t* nroots, char* title, int64_t title_cap);
+int psoci_civec_read(const char* path, int64_t n, int k, double* C, int* nread);
 /* Subspace projection out = A^T B over n rows on FP64 MFMA: the VtV / VtHV
  * dot products of PsociGAsubspace::generateIncrementalSubspace
  * (PsociGAsubspace.C:273-362) and the MGS overlaps of

@@ -98,6 +98,11 @@ def load(path: str | None = None):
         "psoci_direct_schedule": (I, [V, V, V, V]),
         "psoci_stored_shape": (I, [V, I, V, V, V]),
         "psoci_sync": (I, [V]),
+        "psoci_h_save": (I, [V, ctypes.c_char_p]),
+        "psoci_h_load": (I, [V, ctypes.c_char_p, V]),
+        "psoci_civec_write": (I, [ctypes.c_char_p, I64, I, V, ctypes.c_char_p]),
+        "psoci_civec_info": (I, [ctypes.c_char_p, V, V, ctypes.c_char_p, I64]),
+        "psoci_civec_read": (I, [ctypes.c_char_p, I64, I, V, V]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -199,6 +204,18 @@ class Hamiltonian:
         self.nnz_lower = nnz.value
         return nnz.value
 
+    def save_h(self, path: str):
+        """Checkpoint the stored Hamiltonian (the reference's DRA dump)."""
+        self._check(self.L.psoci_h_save(self._h, os.fsencode(path)))
+
+    def load_h(self, path: str) -> int:
+        """Load a checkpoint written for the same configurations and integrals
+        in place of build() (the reference's --hamiltonian_restart)."""
+        nnz = ctypes.c_int64()
+        self._check(self.L.psoci_h_load(self._h, os.fsencode(path), ctypes.byref(nnz)))
+        self.nnz_lower = nnz.value
+        return nnz.value
+
     def get_h(self):
         n, nnz = self.n, self.nnz_lower
         rowptr = np.zeros(n + 1, np.int64)
@@ -232,10 +249,16 @@ class Hamiltonian:
 
     def davidson(self, nroots: int, prec: int = FP64, mode: int = STORED, tol: float = 1e-6,
                  switch_tol: float = 1e-4, max_iter: int = 200, max_subspace: int = 0, guess: int = 0,
-                 want_vectors: bool = True, guess_vectors=None, extra_roots: int = 0):
+                 want_vectors: bool = True, guess_vectors=None, extra_roots: int = 0,
+                 restart: str | None = None):
         """Block Davidson for the nroots lowest roots.  guess: 0 lowest diagonal,
         1 first SEFs (the reference's), 2 guess_vectors ([n][nroots]);
+        restart: a CI-vector file (civec_write) read as guess_vectors, missing
+        roots completed with unit vectors (the reference's --vector_restart);
         extra_roots guard roots ride along in the block (not tested)."""
+        if restart is not None:
+            guess_vectors, _ = civec_read(restart, self.n, nroots)
+            guess = 2
         p = DavParams(nroots, prec, mode, max_iter, max_subspace, guess, tol, switch_tol, extra_roots)
         ev = np.zeros(nroots)
         vec = np.zeros((self.n, nroots)) if want_vectors else None
@@ -330,3 +353,38 @@ def sifs_write(path: str, nbf: int, valtwo, t1=None, v1=None, s1=None, so=None, 
     ptr = lambda a: None if a is None else _p(a)  # noqa: E731
     _ctxfree(L.psoci_sifs_write(os.fsencode(path), nbf, ptr(s1), ptr(t1), ptr(v1), ptr(so), _p(two), float(ecore),
                                 None if title is None else title.encode()))
+
+
+def civec_write(path: str, C, title: str | None = None):
+    """Write CI vectors C [n][k] in the reference's binary restart format."""
+    C = np.ascontiguousarray(C, np.float64)
+    if C.ndim == 1:
+        C = C[:, None]
+    _ctxfree(load().psoci_civec_write(os.fsencode(path), C.shape[0], C.shape[1], _p(C),
+                                      None if title is None else title.encode()))
+
+
+def civec_info(path: str):
+    """(nbasis, nroots, title) of a CI-vector restart file."""
+    nb, nr = ctypes.c_int64(), ctypes.c_int64()
+    buf = ctypes.create_string_buffer(4096)
+    _ctxfree(load().psoci_civec_info(os.fsencode(path), ctypes.byref(nb), ctypes.byref(nr), buf, len(buf)))
+    return nb.value, nr.value, buf.value.decode(errors="replace")
+
+
+def civec_read(path: str, n: int, k: int):
+    """(C [n][k], roots read): the first min(nroots, k) roots of the file and
+    unit vectors e_j in the remaining columns j, as the reference's
+    readVectorScalable + specifyDiagonalGuessVectors do."""
+    C = np.zeros((n, k))
+    got = ctypes.c_int()
+    _ctxfree(load().psoci_civec_read(os.fsencode(path), n, k, _p(C), ctypes.byref(got)))
+    return C, got.value
+
+
+def restart_filename(name: str, rank: int = 0) -> str:
+    """The name PsociVector gives a file for rank `rank` (PsociVector.C:185-208):
+    "-<rank>" inserted before the last "." of the string, else appended."""
+    i = name.rfind(".")
+    tag = f"-{rank}"
+    return name[:i] + tag + name[i:] if i >= 0 else name + tag

@@ -87,6 +87,8 @@ int psoci_set_integrals(psoci_ctx* ctx, int nbf, const double* valone, const dou
     c.d_one.upload(valone, nndaf(nbf), c.stream);
     c.d_two.upload(valtwo, nndaf(nndaf(nbf)), c.stream);
     c.sync();
+    c.ints_hash = psoci::hash_bytes(valtwo, nndaf(nndaf(nbf)) * sizeof(double),
+                                    psoci::hash_bytes(valone, nndaf(nbf) * sizeof(double), (uint64_t)nbf));
     c.nbf_ints = nbf;
     c.ecore = ecore;
     c.have_ints = true;
@@ -103,6 +105,9 @@ int psoci_set_configs(psoci_ctx* ctx, int nbf, int nelec, int ksym, int64_t ncfg
     c.have_space = false;
     psoci::build_space(c.sp, nbf, nelec, ksym, ncfg, occ, jsym);
     psoci::upload_space(c);
+    const int64_t dims[4] = {nbf, nelec, ksym, ncfg};
+    c.space_hash = psoci::hash_bytes(jsym, ncfg * sizeof(int32_t),
+                                     psoci::hash_bytes(occ, ncfg * nbf, psoci::hash_bytes(dims, sizeof(dims))));
     c.have_space = true;
     c.have_h = c.have_diag = c.have_pairs = false;
     return PSOCI_OK;
@@ -159,6 +164,21 @@ int psoci_build_h(psoci_ctx* ctx, int64_t* nnz_lower) {
   return guarded(ctx, [&](Ctx& c) {
     PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
     psoci::build_h(c);
+    if (nnz_lower) *nnz_lower = c.nnz_lower;
+    return PSOCI_OK;
+  });
+}
+
+int psoci_h_save(psoci_ctx* ctx, const char* path) {
+  return guarded(ctx, [&](Ctx& c) {
+    psoci::save_h(c, path);
+    return PSOCI_OK;
+  });
+}
+
+int psoci_h_load(psoci_ctx* ctx, const char* path, int64_t* nnz_lower) {
+  return guarded(ctx, [&](Ctx& c) {
+    psoci::load_h(c, path);
     if (nnz_lower) *nnz_lower = c.nnz_lower;
     return PSOCI_OK;
   });

@@ -97,6 +97,49 @@ void read_sifs(const char* path, int* nbf, double* one, double* two, double* eco
 const std::string& ctxfree_error();
 void set_ctxfree_error(const std::string& m);
 
+// Runs a context-free entry point: status code out, message kept per thread.
+template <class F>
+int ctxfree(F&& f) {
+  try {
+    set_ctxfree_error("");
+    f();
+    return PSOCI_OK;
+  } catch (const Error& e) {
+    set_ctxfree_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_ctxfree_error(e.what());
+    return PSOCI_ERR_NOMEM;
+  }
+}
+
+// 64-bit fingerprint of a byte range (8-byte words, multiply-rotate mixing):
+// detects a checkpoint written for other inputs and corrupted payloads; not
+// a cryptographic hash.
+inline uint64_t hash_bytes(const void* p, size_t nbytes, uint64_t h = 0x9e3779b97f4a7c15ull) {
+  const unsigned char* b = static_cast<const unsigned char*>(p);
+  auto mix = [&h](uint64_t w) {
+    w *= 0x87c37b91114253d5ull;
+    w = (w << 31) | (w >> 33);
+    h ^= w * 0x4cf5ad432745937full;
+    h = ((h << 27) | (h >> 37)) * 5 + 0x52dce729;
+  };
+  size_t i = 0;
+  for (; i + 8 <= nbytes; i += 8) {
+    uint64_t w;
+    std::memcpy(&w, b + i, 8);
+    mix(w);
+  }
+  uint64_t tail = 0;
+  std::memcpy(&tail, b + i, nbytes - i);
+  mix(tail ^ ((uint64_t)(nbytes - i) << 56));
+  h ^= nbytes;
+  h ^= h >> 33;
+  h *= 0xff51afd7ed558ccdull;
+  h ^= h >> 33;
+  return h;
+}
+
 // Library-wide kernel launch helpers (defined in the .hip files).
 void exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t n, hipStream_t s, void*& tmp, size_t& tmp_bytes);
 

@@ -15,6 +15,9 @@ struct Ctx {
   int nbf_ints = 0;
   double ecore = 0.0;
   bool have_ints = false;
+  // fingerprints of the inputs (configuration list, integrals), written into
+  // a stored-Hamiltonian checkpoint and checked when one is loaded
+  uint64_t space_hash = 0, ints_hash = 0;
 
   // space on the device
   DevBuf<uint64_t> d_dm, d_sm;
@@ -85,6 +88,12 @@ void block_on_device(Ctx& c, int64_t I, int64_t J, double* host_out, int* status
 void sigma_stored(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy);
 void sigma_direct(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy);
 void get_h_lower(Ctx& c, int64_t* rowptr, int32_t* col, double* val, double* diag);
+// installs a stored Hamiltonian given as the full symmetric CSR (lower part
+// first in each row, lowcnt entries) and its diagonal (restart.cpp -> hbuild.hip)
+void install_h(Ctx& c, const std::vector<int64_t>& rowptr, const std::vector<int32_t>& col,
+               const std::vector<double>& val, const std::vector<int32_t>& lowcnt, const std::vector<double>& diag);
+void save_h(Ctx& c, const char* path);  // stored-Hamiltonian checkpoint (restart.cpp)
+void load_h(Ctx& c, const char* path);
 void rdm_grad(Ctx& c, int k, const double* dC, const double* w, double* d_g1, double* d_g2);
 
 struct DavParams;

@@ -520,4 +520,24 @@ void get_h_lower(Ctx& c, int64_t* rowptr, int32_t* col, double* val, double* dia
   }
 }
 
+void install_h(Ctx& c, const std::vector<int64_t>& rowptr, const std::vector<int32_t>& col,
+               const std::vector<double>& val, const std::vector<int32_t>& lowcnt, const std::vector<double>& diag) {
+  const int64_t n = c.sp.nsef_total, nnz = rowptr[n];
+  c.have_h = false;
+  c.d_rowptr.upload(rowptr, c.stream);
+  c.d_col.upload(col, c.stream);
+  c.d_val.upload(val, c.stream);
+  c.d_lowcnt.upload(lowcnt, c.stream);
+  c.d_diag.upload(diag, c.stream);
+  c.d_val32.alloc(nnz);
+  if (nnz > 0)
+    hipLaunchKernelGGL(k_to_f32, dim3(blocks_for(nnz, 256)), dim3(256), 0, c.stream, nnz, c.d_val.p, c.d_val32.p);
+  PSOCI_HIP(hipGetLastError());
+  c.nnz_full = nnz;
+  c.nnz_lower = (nnz + n) / 2;
+  c.sync();
+  c.have_h = true;
+  c.have_diag = true;
+}
+
 }  // namespace psoci

@@ -271,21 +271,6 @@ void read_file(const char* path, int* nbf_out, double* one, double* two, double*
   std::fclose(r.f);
 }
 
-template <class F>
-int ctxfree(F&& f) {
-  try {
-    g_err.clear();
-    f();
-    return PSOCI_OK;
-  } catch (const Error& e) {
-    g_err = e.what();
-    return e.code;
-  } catch (const std::exception& e) {
-    g_err = e.what();
-    return PSOCI_ERR_NOMEM;
-  }
-}
-
 }  // namespace
 
 const std::string& ctxfree_error() { return g_err; }
