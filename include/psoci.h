@@ -75,6 +75,17 @@ int psoci_sifs_read(const char* path, int* nbf, double* valone, double* valtwo, 
 int psoci_sifs_write(const char* path, int nbf, const double* s1, const double* t1, const double* v1,
                      const double* so, const double* valtwo, double ecore, const char* title);
 
+/* AO-basis files for the Mulliken analysis (the reference's "aoints":
+ * PsociIntegrals::fetchAndProcessOverlapIntegrals / fetchOverlapInts /
+ * fetchBasisLabels, PsociIntegrals.C:90-93, 967-1110).  Same record layout;
+ * psoci_sifs_read_ao sums the 1-e records with itypea = itypeb = 0 into
+ * overlap [NNDAF(nbf)] (packed like valone) and copies the basis labels
+ * into labels [nbf][8] (blank padded, not NUL-terminated); NULL outputs
+ * are skipped (pass NULLs to query nbf).  psoci_sifs_write_ao writes the
+ * overlap, the labels ([nbf][8]; NULL: "mo1", "mo2", ...) and an empty
+ * 2-e part. */
+int psoci_sifs_read_ao(const char* path, int* nbf, double* overlap, char* labels);
+int psoci_sifs_write_ao(const char* path, int nbf, const double* overlap, const char* labels, const char* title);
 /* psoci_sifs_read + psoci_set_integrals. */
 int psoci_set_integrals_sifs(psoci_ctx* ctx, const char* path);
 
@@ -243,6 +254,47 @@ int psoci_project(psoci_ctx* ctx, int64_t n, int m, int b, const double* A, int 
 int psoci_rdm(psoci_ctx* ctx, int k, const double* C, const double* w, double* grad_one, double* grad_two,
               double* grad_so);
 
+/* CI energy analysis: analyzeEnergyContributions (PsociCIAnalysis.C:42-267,
+ * the per-configuration sums of :126-142) on the GPU.  For k roots C [n][k]
+ * (row-major, host) with eigenvalues evals[k] (without ecore), deltae
+ * [ncfg][k] receives, per configuration I and root r, the sum over the SEFs i
+ * of I of -c_ir^2 (H_ii - E_r) / (1 - c_ir^2) in the reference's order and
+ * rounding (an SEF with c_ir^2 == 1 resets the sum to E_r).  Sorting by
+ * |deltae| and the threshold of the report are the caller's (psoci_amd.api
+ * analyze_energy_contributions mirrors the reference's report). */
+int psoci_energy_contrib(psoci_ctx* ctx, int k, const double* C, const double* evals, double* deltae);
+/* Natural orbitals from a CI one-particle density (context free, host):
+ * constructNaturalOrbitals with diagonalizeNOinMObasis, sortNOinMObasis,
+ * transformNOstoAObasis and computeMOdensity (PsociNOpopulations.C:173-500,
+ * 569-729).  rdm1 [NNDAF(nbf)] is the reference's packed CI density (the
+ * off-diagonal counted once per pair: psoci_rdm's grad_one with the
+ * off-diagonal halved).  occ [nbf]: occupations, largest first; no_mo
+ * [nbf][nbf]: row i = natural orbital i over the MOs (sign: largest-magnitude
+ * coefficient positive).  With mocoef [nbf MOs][nbf AOs] (else NULL):
+ * no_ao [nbf][nbf] (row i over the AOs) and dens_ao [NNDAF(nbf)], the AO
+ * density C^T D C packed.  Any output may be NULL. */
+int psoci_natural_orbitals(int nbf, const double* rdm1, const double* mocoef, double* occ, double* no_mo,
+                           double* no_ao, double* dens_ao);
+/* Mulliken analysis (context free, host): constructMullikenPopulationDensAO
+ * (PsociNOpopulations.C:1084-1206).  population [nbf][nbf] = P_{mu nu}
+ * S_{mu nu} from the packed AO density and overlap; *total = its sum (the
+ * electron count for an S-orthonormal MO set). */
+int psoci_mulliken(int nbf, const double* dens_ao, const double* overlap, double* population, double* total);
+/* Basis-function groups for condensed populations: clusterBasisLabels
+ * (PsociNOpopulations.C:1451-1510).  labels [nbf][8]; mode 0 atomic (leading
+ * digits and non-alphanumerics dropped, cut at the first '_'), 1 subshell
+ * (leading digits and non-alphanumerics dropped), 2 the labels as they are.
+ * group [nbf] receives the group of each function, numbered in the sorted
+ * order of the condensed labels; *ngroups their count. */
+int psoci_basis_groups(int nbf, const char* labels, int mode, int32_t* group, int32_t* ngroups);
+/* Condensed Mulliken populations: atomicPopulations / subshellPopulations
+ * (gross [ngroups]: sum over the functions of a group of their population
+ * columns) and atomicOverlapPopulations (bond [ngroups][ngroups], g <= h:
+ * the block sum, doubled for g != h; entries below the diagonal are 0)
+ * (PsociNOpopulations.C:1513-1790), summed in the reference's order.  Either
+ * output may be NULL. */
+int psoci_group_populations(int nbf, const double* population, const int32_t* group, int ngroups, double* gross,
+                            double* bond);
 /* Device handles of the context (for zero-copy callers and for timing with
  * HIP events on the stream the library launches on): which = 0: diag
  * (double[n]); which = 1: the context's hipStream_t. */

@@ -25,6 +25,10 @@ Parity status of each oracle function (what pins it):
                                   (its iteration trajectory: parity unpinned)
   * CIDEN 1-RDM                -- brute-force <Psi|E_pq|Psi> from the
                                   second-quantised determinant expansion
+  * CI energy contributions    -- closed forms: a 2x2 eigenproblem (each
+                                  configuration's contribution is E minus the
+                                  other's diagonal), the c^2 == 1 reset, zero
+                                  coefficients contribute nothing
 """
 from __future__ import annotations
 
@@ -75,6 +79,7 @@ def lib():
         L.orc_davidson.restype = ctypes.c_int
         L.orc_davidson.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, P, P, P, P]
         L.orc_rdm1.argtypes = [P, P, P]
+        L.orc_energy_contrib.argtypes = [ctypes.c_long, P, ctypes.c_long, ctypes.c_int, P, P, P, P]
         L.orc_direct_rows.restype = ctypes.c_int
         L.orc_direct_rows.argtypes = [P, ctypes.c_long, P, ctypes.c_int, P, P, ctypes.c_int]
         L.orc_two_e_index.restype = ctypes.c_int
@@ -200,3 +205,17 @@ class OracleSpace:
         D = np.zeros(self.nbf * (self.nbf + 1) // 2)
         lib().orc_rdm1(self._h, _p(c), _p(D))
         return D
+
+
+def energy_contrib(sef_off, diag, C, evals):
+    """deltae [ncfg][k] of analyzeEnergyContributions for C [n][k]."""
+    sef_off = np.ascontiguousarray(sef_off, np.int64)
+    diag = np.ascontiguousarray(diag, np.float64)
+    C = np.ascontiguousarray(C, np.float64)
+    if C.ndim == 1:
+        C = C[:, None]
+    evals = np.ascontiguousarray(evals, np.float64)
+    n, k = C.shape
+    out = np.zeros((len(sef_off), k))
+    lib().orc_energy_contrib(len(sef_off), _p(sef_off), n, k, _p(C), _p(diag), _p(evals), _p(out))
+    return out

@@ -1353,6 +1353,32 @@ int orc_direct_rows(void* h, long nrows, const long* rows, int k, const double* 
   return bad ? -2 : 0;
 }
 
+// CI energy analysis, analyzeEnergyContributions (PsociCIAnalysis.C:42-267),
+// the per-configuration sums of :126-142 for every root: the reference reads
+// root iroot of the GA vector into vect (dims[0] = maxsef long), takes its
+// l_nsef offsets as each configuration's first SEF (the last configuration
+// ending at maxsef) and accumulates in SEF order.  C is [n][k] row-major.
+void orc_energy_contrib(long ncfg, const long* sef_off, long n, int k, const double* C, const double* diags,
+                        const double* evals, double* deltae) {
+  std::vector<double> vect(n);
+  for (int iroot = 0; iroot < k; ++iroot) {
+    for (long i = 0; i < n; ++i) vect[i] = C[i * k + iroot];
+    for (long itotfg = 0; itotfg < ncfg; ++itotfg) {
+      double acc = 0.0;
+      long isefstart = sef_off[itotfg];
+      long isefstop = itotfg + 1 < ncfg ? sef_off[itotfg + 1] : n;
+      for (long i = isefstart; i < isefstop; ++i) {
+        double vect2 = vect[i] * vect[i];
+        if (vect2 == 1.0)
+          acc = evals[iroot];
+        else
+          acc -= vect2 * (diags[i] - evals[iroot]) / (1.0 - vect2);
+      }
+      deltae[itotfg * k + iroot] = acc;
+    }
+  }
+}
+
 // Canonical packed indices, exported so tests can check them against brute
 // force permutation symmetry.
 int orc_two_e_index(int i, int j) { return two_e_index(i, j); }

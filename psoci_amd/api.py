@@ -103,6 +103,13 @@ def load(path: str | None = None):
         "psoci_civec_write": (I, [ctypes.c_char_p, I64, I, V, ctypes.c_char_p]),
         "psoci_civec_info": (I, [ctypes.c_char_p, V, V, ctypes.c_char_p, I64]),
         "psoci_civec_read": (I, [ctypes.c_char_p, I64, I, V, V]),
+        "psoci_sifs_read_ao": (I, [ctypes.c_char_p, V, V, V]),
+        "psoci_sifs_write_ao": (I, [ctypes.c_char_p, I, V, ctypes.c_char_p, ctypes.c_char_p]),
+        "psoci_energy_contrib": (I, [V, I, V, V, V]),
+        "psoci_natural_orbitals": (I, [I, V, V, V, V, V, V]),
+        "psoci_mulliken": (I, [I, V, V, V, V]),
+        "psoci_basis_groups": (I, [I, ctypes.c_char_p, I, V, V]),
+        "psoci_group_populations": (I, [I, V, V, I, V, V]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -272,6 +279,18 @@ class Hamiltonian:
             self._check(rc)
         return dict(evals=ev, evecs=vec, stats=st.as_dict(), converged=rc == OK)
 
+    def energy_contrib(self, C: np.ndarray, evals) -> np.ndarray:
+        """deltae [ncfg][k]: each configuration's energy contribution to each
+        root (analyzeEnergyContributions, PsociCIAnalysis.C:126-142)."""
+        C = np.ascontiguousarray(C, np.float64)
+        if C.ndim == 1:
+            C = C[:, None]
+        k = C.shape[1]
+        ev = np.ascontiguousarray(evals, np.float64).reshape(k)
+        out = np.zeros((self.cfg.ncfg, k))
+        self._check(self.L.psoci_energy_contrib(self._h, k, _p(C), _p(ev), _p(out)))
+        return out
+
     def project(self, A, B):
         """A^T B over the rows of two GPU tensors (FP64 MFMA)."""
         n, m = A.shape
@@ -388,3 +407,144 @@ def restart_filename(name: str, rank: int = 0) -> str:
     i = name.rfind(".")
     tag = f"-{rank}"
     return name[:i] + tag + name[i:] if i >= 0 else name + tag
+
+
+AU_TO_EV = 27.21138505  # PsociCIAnalysis.hpp:42
+
+
+def analyze_energy_contributions(H: "Hamiltonian", C, evals, thresh: float = 1e-4, ecore: float = 0.0):
+    """The reference's CI energy analysis (analyzeEnergyContributions,
+    PsociCIAnalysis.C:42-267) for the roots C [n][k]: per root the energies
+    (valence, core, total, E(i) - E(1) in eV) and the configurations with
+    |deltae| >= thresh, largest first (stable for ties, as the reference's
+    list sort), each with its SEFs' coefficients and per-SEF contributions."""
+    C = np.ascontiguousarray(C, np.float64)
+    if C.ndim == 1:
+        C = C[:, None]
+    evals = np.asarray(evals, np.float64)
+    de = H.energy_contrib(C, evals)
+    diag = H.diag()
+    off = H.info.sef_offset
+    nsef = H.info.nsef
+    roots = []
+    for r in range(C.shape[1]):
+        order = np.argsort(-np.abs(de[:, r]), kind="stable")
+        keep = order[np.abs(de[order, r]) >= thresh]
+        confs = []
+        for I in keep:
+            I = int(I)
+            i0, i1 = int(off[I]), int(off[I]) + int(nsef[I])
+            v = C[i0:i1, r]
+            v2 = v * v
+            with np.errstate(divide="ignore", invalid="ignore"):
+                dl = np.where(v2 == 1.0, evals[r], v2 * (evals[r] - diag[i0:i1]) / (1.0 - v2))
+            confs.append(dict(config=I, deltae=float(de[I, r]), occ="".join(str(int(x)) for x in H.cfg.occ[I]),
+                              sefs=list(range(i0, i1)), coef=v.copy(), del_sef=dl))
+        roots.append(dict(valence=float(evals[r]), core=float(ecore), total=float(evals[r] + ecore),
+                          excitation_ev=float((evals[r] - evals[0]) * AU_TO_EV), configs=confs,
+                          distribution=energy_distribution(de[:, r])))
+    return roots
+
+
+def energy_distribution(deltae, bins=(1e-8, 1e-7, 1e-6, 1e-5, 1e-4, 1e-3, 1e-2, 1e-1, 1.0, 10.0)):
+    """Count and sum of the configuration contributions per |deltae| decade
+    (the bins of energyDistributionSpatial, PsociCIAnalysis.C:267-335):
+    entry b covers bins[b-1] <= |deltae| < bins[b] (b = 0: below bins[0],
+    the last entry: at or above bins[-1])."""
+    a = np.abs(np.asarray(deltae))
+    edges = np.concatenate([[0.0], bins, [np.inf]])
+    out = []
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        m = (a >= lo) & (a < hi)
+        out.append(dict(lo=float(lo), hi=float(hi), count=int(m.sum()), sum=float(np.asarray(deltae)[m].sum())))
+    return out
+
+
+def rdm1_from_grad(grad_one):
+    """The reference's packed CI density (off-diagonal once per orbital pair)
+    from psoci_rdm's dE/d(valone): the off-diagonal halved."""
+    g = np.asarray(grad_one, np.float64).copy()
+    n1 = len(g)
+    nbf = int((np.sqrt(8 * n1 + 1) - 1) / 2)
+    for p in range(nbf):
+        for q in range(p):
+            g[p * (p + 1) // 2 + q] *= 0.5
+    return g
+
+
+def natural_orbitals(rdm1, mocoef=None):
+    """Natural orbitals of a packed CI density: dict(occ, no_mo[, no_ao,
+    dens_ao]) (constructNaturalOrbitals, PsociNOpopulations.C:569-729);
+    mocoef [MO][AO] adds the AO-basis NOs and the packed AO density."""
+    rdm1 = np.ascontiguousarray(rdm1, np.float64)
+    nbf = int((np.sqrt(8 * len(rdm1) + 1) - 1) / 2)
+    occ = np.zeros(nbf)
+    no_mo = np.zeros((nbf, nbf))
+    out = dict(occ=occ, no_mo=no_mo)
+    mo = no_ao = dens = None
+    if mocoef is not None:
+        mo = np.ascontiguousarray(mocoef, np.float64).reshape(nbf, nbf)
+        no_ao = np.zeros((nbf, nbf))
+        dens = np.zeros(len(rdm1))
+        out.update(no_ao=no_ao, dens_ao=dens)
+    _ctxfree(load().psoci_natural_orbitals(nbf, _p(rdm1), _p(mo), _p(occ), _p(no_mo), _p(no_ao), _p(dens)))
+    return out
+
+
+def mulliken(dens_ao, overlap):
+    """(population [nbf][nbf], total) of constructMullikenPopulationDensAO."""
+    d = np.ascontiguousarray(dens_ao, np.float64)
+    s = np.ascontiguousarray(overlap, np.float64)
+    nbf = int((np.sqrt(8 * len(d) + 1) - 1) / 2)
+    pop = np.zeros((nbf, nbf))
+    tot = ctypes.c_double()
+    _ctxfree(load().psoci_mulliken(nbf, _p(d), _p(s), _p(pop), ctypes.byref(tot)))
+    return pop, tot.value
+
+
+ATOMIC, SUBSHELL, NONE = 0, 1, 2
+
+
+def _labels_bytes(labels):
+    return b"".join(lab.encode()[:8].ljust(8) for lab in labels)
+
+
+def basis_groups(labels, mode: int = ATOMIC):
+    """(group per basis function, number of groups) of clusterBasisLabels."""
+    nbf = len(labels)
+    grp = np.zeros(nbf, np.int32)
+    ng = ctypes.c_int32()
+    _ctxfree(load().psoci_basis_groups(nbf, _labels_bytes(labels), mode, _p(grp), ctypes.byref(ng)))
+    return grp, ng.value
+
+
+def group_populations(population, group, ngroups: int):
+    """(gross [ngroups], bond [ngroups][ngroups]) condensed Mulliken populations."""
+    pop = np.ascontiguousarray(population, np.float64)
+    grp = np.ascontiguousarray(group, np.int32)
+    gross = np.zeros(ngroups)
+    bond = np.zeros((ngroups, ngroups))
+    _ctxfree(load().psoci_group_populations(len(grp), _p(pop), _p(grp), ngroups, _p(gross), _p(bond)))
+    return gross, bond
+
+
+def sifs_write_ao(path: str, overlap, labels=None, title: str | None = None):
+    """Write an AO-basis file: packed overlap and 8-character basis labels."""
+    s = np.ascontiguousarray(overlap, np.float64)
+    nbf = int((np.sqrt(8 * len(s) + 1) - 1) / 2)
+    _ctxfree(load().psoci_sifs_write_ao(os.fsencode(path), nbf, _p(s),
+                                        None if labels is None else _labels_bytes(labels),
+                                        None if title is None else title.encode()))
+
+
+def sifs_read_ao(path: str):
+    """(packed overlap, basis labels) of an AO-basis file."""
+    L = load()
+    nbf = ctypes.c_int()
+    _ctxfree(L.psoci_sifs_read_ao(os.fsencode(path), ctypes.byref(nbf), None, None))
+    n = nbf.value
+    s = np.zeros(n * (n + 1) // 2)
+    lab = ctypes.create_string_buffer(8 * n)
+    _ctxfree(L.psoci_sifs_read_ao(os.fsencode(path), ctypes.byref(nbf), _p(s), lab))
+    raw = lab.raw
+    return s, [raw[8 * i:8 * i + 8].decode() for i in range(n)]

@@ -293,6 +293,23 @@ int psoci_rdm(psoci_ctx* ctx, int k, const double* C, const double* w, double* g
   });
 }
 
+int psoci_energy_contrib(psoci_ctx* ctx, int k, const double* C, const double* evals, double* deltae) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
+    PSOCI_CHECK(k >= 1 && C && evals && deltae, PSOCI_ERR_ARG, "bad energy-contribution arguments");
+    if (!c.have_diag) psoci::compute_diag(c);
+    const int64_t n = c.sp.nsef_total, ncfg = c.sp.ncfg;
+    psoci::DevBuf<double> dC, dE, dD;
+    dC.upload(C, (size_t)n * k, c.stream);
+    dE.upload(evals, (size_t)k, c.stream);
+    dD.alloc((size_t)ncfg * k);
+    psoci::energy_contrib(c, k, dC.p, dE.p, dD.p);
+    dD.download(deltae, (size_t)ncfg * k, c.stream);
+    c.sync();
+    return PSOCI_OK;
+  });
+}
+
 int psoci_project(psoci_ctx* ctx, int64_t n, int m, int b, const double* A, int lda, const double* B, int ldb,
                   double* out) {
   return guarded(ctx, [&](Ctx& c) {

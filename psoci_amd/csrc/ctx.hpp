@@ -92,6 +92,8 @@ void get_h_lower(Ctx& c, int64_t* rowptr, int32_t* col, double* val, double* dia
 // first in each row, lowcnt entries) and its diagonal (restart.cpp -> hbuild.hip)
 void install_h(Ctx& c, const std::vector<int64_t>& rowptr, const std::vector<int32_t>& col,
                const std::vector<double>& val, const std::vector<int32_t>& lowcnt, const std::vector<double>& diag);
+// per-configuration energy contributions of k roots (analysis.hip)
+void energy_contrib(Ctx& c, int k, const double* dC, const double* d_evals, double* d_deltae);
 void save_h(Ctx& c, const char* path);  // stored-Hamiltonian checkpoint (restart.cpp)
 void load_h(Ctx& c, const char* path);
 void rdm_grad(Ctx& c, int k, const double* dC, const double* w, double* d_g1, double* d_g2);

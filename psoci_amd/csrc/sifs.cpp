@@ -123,10 +123,13 @@ void write_block(Writer& w, int nipv, int nmax, int itypea, int itypeb, const st
   } while (done < n);
 }
 
+// labels: [nbf][8] basis-function labels (NULL: "mo1", "mo2", ...); two NULL
+// (AO overlap files): an empty 2-e part
 void write_file(const char* path, int nbf, const double* s1, const double* t1, const double* v1, const double* so,
-                const double* two, double ecore, const char* title) {
-  PSOCI_CHECK(path && nbf > 0 && nbf <= 64 && two, PSOCI_ERR_ARG, "bad SIFS write arguments");
-  PSOCI_CHECK(t1 || v1, PSOCI_ERR_ARG, "at least one one-electron integral type is needed");
+                const double* two, double ecore, const char* title, const char* labels = nullptr) {
+  PSOCI_CHECK(path && nbf > 0 && nbf <= 64 && (two || (s1 && !t1 && !v1 && !so)), PSOCI_ERR_ARG,
+              "bad SIFS write arguments");
+  PSOCI_CHECK(t1 || v1 || !two, PSOCI_ERR_ARG, "at least one one-electron integral type is needed");
   Writer w;
   w.f = std::fopen(path, "wb");
   PSOCI_CHECK(w.f, PSOCI_ERR_ARG, std::string("cannot create ") + path);
@@ -140,7 +143,8 @@ void write_file(const char* path, int nbf, const double* s1, const double* t1, c
     w.put_str("a", kSlabel);
     const int32_t info[5] = {1, kN1max, kN1max, kN2max, kN2max};
     w.put(info, 5);
-    for (int i = 0; i < nbf; ++i) w.put_str("mo" + std::to_string(i + 1), kBfnlab);
+    for (int i = 0; i < nbf; ++i)
+      w.put_str(labels ? std::string(labels + (size_t)kBfnlab * i, kBfnlab) : "mo" + std::to_string(i + 1), kBfnlab);
     const int32_t ietype = -1;  // the first core energy is the nuclear repulsion
     w.put(&ietype, 1);
     w.put(&ecore, 1);
@@ -178,7 +182,7 @@ void write_file(const char* path, int nbf, const double* s1, const double* t1, c
     // 2-e part: every canonical (ij|kl), i >= j, k >= l, ij >= kl
     std::vector<double> vals;
     std::vector<int32_t> labs;
-    for (int i = 1; i <= nbf; ++i)
+    for (int i = 1; two && i <= nbf; ++i)
       for (int j = 1; j <= i; ++j)
         for (int k = 1; k <= i; ++k)
           for (int l = 1; l <= k; ++l) {
@@ -192,6 +196,65 @@ void write_file(const char* path, int nbf, const double* s1, const double* t1, c
     throw;
   }
   PSOCI_CHECK(std::fclose(w.f) == 0, PSOCI_ERR_ARG, std::string("cannot write ") + path);
+}
+
+// AO-basis file (the reference's "aoints" for the Mulliken analysis,
+// PsociIntegrals::fetchAndProcessOverlapIntegrals / fetchOverlapInts /
+// fetchBasisLabels, PsociIntegrals.C:90-93, 967-1110): the 1-e records with
+// itypea = 0 and itypeb = 0 summed into the packed overlap, the basis labels
+// as stored (8 characters each); the 2-e part is not read.
+void read_ao_file(const char* path, int* nbf_out, double* overlap, char* labels) {
+  PSOCI_CHECK(path, PSOCI_ERR_ARG, "null path");
+  Reader r;
+  r.path = path;
+  r.f = std::fopen(path, "rb");
+  PSOCI_CHECK(r.f, PSOCI_ERR_ARG, std::string("cannot open ") + path);
+  try {
+    int32_t h1[6];
+    r.next();
+    r.get(h1, 6);
+    const int ntitle = h1[0], nsym = h1[1], nbft = h1[2], ninfo = h1[3], nenrgy = h1[4], nmap = h1[5];
+    PSOCI_CHECK(ntitle >= 0 && nsym >= 1 && nbft >= 1 && nbft <= 64 && ninfo >= 5 && nenrgy >= 1 && nmap >= 0,
+                PSOCI_ERR_ARG, r.path + ": bad header 1");
+    if (nbf_out) *nbf_out = nbft;
+    if (overlap || labels) {
+      r.next();
+      r.skip((size_t)ntitle * kTitle);
+      std::vector<int32_t> nbpsy(nsym), info(ninfo);
+      r.get(nbpsy.data(), nsym);
+      r.skip((size_t)nsym * kSlabel);
+      r.get(info.data(), ninfo);
+      std::vector<char> lab((size_t)nbft * kBfnlab);
+      r.get(lab.data(), lab.size());
+      if (labels) std::memcpy(labels, lab.data(), lab.size());
+      const int n1max = info[2];
+      PSOCI_CHECK(n1max > 0, PSOCI_ERR_ARG, r.path + ": bad info record");
+      std::vector<double> s(nndaf(nbft), 0.0), vals;
+      std::vector<int32_t> labs;
+      int32_t hdr[6];
+      do {
+        r.next();
+        r.get(hdr, 6);
+        double fcore;
+        r.get(&fcore, 1);
+        PSOCI_CHECK(hdr[0] >= 0 && hdr[0] <= n1max, PSOCI_ERR_ARG, r.path + ": record exceeds its declared capacity");
+        vals.resize(hdr[0]);
+        labs.resize((size_t)hdr[0] * 2);
+        r.get(vals.data(), vals.size());
+        r.get(labs.data(), labs.size());
+        for (int32_t i = 0; i < hdr[0]; ++i) {
+          const int64_t a = labs[2 * i], b = labs[2 * i + 1];
+          PSOCI_CHECK(a >= 1 && a <= nbft && b >= 1 && b <= nbft, PSOCI_ERR_ARG, r.path + ": orbital label out of range");
+          if (hdr[2] == 0 && hdr[3] == 0) s[nndxf(std::max(a, b)) + std::min(a, b) - 1] += vals[i];
+        }
+      } while (hdr[1] != 2);
+      if (overlap) std::memcpy(overlap, s.data(), s.size() * sizeof(double));
+    }
+  } catch (...) {
+    std::fclose(r.f);
+    throw;
+  }
+  std::fclose(r.f);
 }
 
 void read_file(const char* path, int* nbf_out, double* one, double* two, double* ecore) {
@@ -291,6 +354,17 @@ int psoci_sifs_write(const char* path, int nbf, const double* s1, const double* 
 
 int psoci_sifs_read(const char* path, int* nbf, double* valone, double* valtwo, double* ecore) {
   return psoci::ctxfree([&] { psoci::read_file(path, nbf, valone, valtwo, ecore); });
+}
+
+int psoci_sifs_write_ao(const char* path, int nbf, const double* overlap, const char* labels, const char* title) {
+  return psoci::ctxfree([&] {
+    PSOCI_CHECK(overlap, PSOCI_ERR_ARG, "null overlap");
+    psoci::write_file(path, nbf, overlap, nullptr, nullptr, nullptr, nullptr, 0.0, title, labels);
+  });
+}
+
+int psoci_sifs_read_ao(const char* path, int* nbf, double* overlap, char* labels) {
+  return psoci::ctxfree([&] { psoci::read_ao_file(path, nbf, overlap, labels); });
 }
 
 }  // extern "C"
