@@ -14,10 +14,11 @@
 //
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fopenmp).
 //
-// Scope: EVEN-class determinants (even electron count, configuration
-// symmetry class equal to the state class), which is what a C1 space uses.
-// ODD-class and odd-electron (fractional) generators and the spin-orbit
-// block (socig) are not restated; the oracle reports an error for them.
+// Scope: even electron counts, EVEN-class determinants (configuration
+// symmetry class equal to the state class, what a C1 space uses) and
+// ODD-class ones (the classes differ).  The odd-electron (fractional)
+// generators and the spin-orbit block (socig) are not restated; the oracle
+// reports an error for them.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -244,6 +245,89 @@ bool even_determinants(Joutfg& c, int msbas, std::string& err) {
   for (auto& v : c.alpha_spins) c.nalpha.push_back((int)v.size());
   for (auto& v : c.beta_spins) c.nbeta.push_back((int)v.size());
   return true;
+}
+
+// generateODDSpinDeterminants and its per-open-shell-count cases
+// (PsociDeterminants.C:1613-1658, 3634-4353): the determinants of odd spin
+// projection (Ms = +-1, +-3, ...), each alpha list followed by its flip; a
+// closed shell keeps its single Ms = 0 determinant.
+bool odd_determinants(Joutfg& c, int msbas, std::string& err) {
+  int s = c.num_singly;
+  int ndet = c.ndeti;
+  GenState g;
+  g.c = &c;
+  g.msbas = msbas;
+  g.ifiga.assign(ndet, {});
+  g.ifigb.assign(ndet, {});
+  g.ms.assign(ndet, 0);
+  std::vector<int> all(s);
+  for (int t = 0; t < s; ++t) all[t] = t + 1;
+  switch (s) {
+    case 0:  // :3634-3693
+      g.ms[0] = msbas;
+      g.ifiga[0] = c.doubly;
+      g.ifigb[0] = c.doubly;
+      break;
+    case 2:  // :3695-3766 (jdbl written past size(), within reserve())
+      g.ms[0] = msbas + 1;
+      g.ms[1] = msbas - 1;
+      g.ifiga[0] = g.list_of(all);
+      g.ifigb[1] = g.list_of(all);
+      g.ifiga[1] = c.doubly;
+      g.ifigb[0] = c.doubly;
+      break;
+    case 4:  // :3768-3857
+      g.ascending(4, 3, 1, 1);
+      g.descending(4, 1, 8);
+      break;
+    case 6:  // :3859-4009
+      g.ascending(6, 4, 1, 1);
+      g.descending(6, 2, 30);
+      g.ms[30] = msbas + 3;
+      g.ms[31] = msbas - 3;
+      g.ifiga[31] = c.doubly;
+      g.ifigb[30] = c.doubly;
+      g.ifiga[30] = g.list_of(all);
+      g.ifigb[31] = g.list_of(all);
+      break;
+    case 8:  // :4011-4158
+      g.ascending(8, 5, 1, 1);
+      g.descending(8, 3, 112);
+      g.ascending(8, 7, 113, 3);
+      g.descending(8, 1, 128);
+      break;
+    case 10:  // :4160-4353
+      g.ascending(10, 6, 1, 1);
+      g.descending(10, 4, 420);
+      g.ascending(10, 8, 421, 3);
+      g.descending(10, 2, 510);
+      g.ms[510] = msbas + 5;
+      g.ms[511] = msbas - 5;
+      g.ifiga[511] = c.doubly;
+      g.ifigb[510] = c.doubly;
+      g.ifiga[510] = g.list_of(all);
+      g.ifigb[511] = g.list_of(all);
+      break;
+    default:
+      err = "ODD class supports 0,2,4,6,8,10 open shells";
+      return false;
+  }
+  c.alpha_spins = g.ifiga;
+  c.beta_spins = g.ifigb;
+  c.spin_projection = g.ms;
+  c.nalpha.clear();
+  c.nbeta.clear();
+  for (auto& v : c.alpha_spins) c.nalpha.push_back((int)v.size());
+  for (auto& v : c.beta_spins) c.nbeta.push_back((int)v.size());
+  return true;
+}
+
+// generateDeterminants (PsociDeterminants.C:1467-1503): the EVEN generators
+// when the configuration's symmetry class (jsym + 1) / 2 equals the state's,
+// else the ODD ones.
+bool class_determinants(Joutfg& c, int ksym, int msbas, std::string& err) {
+  if ((c.jsym + 1) / 2 == (ksym + 1) / 2) return even_determinants(c, msbas, err);
+  return odd_determinants(c, msbas, err);
 }
 
 // computePhaseAndSort (PsociDeterminants.C:4355-4469): per determinant, the
@@ -668,7 +752,7 @@ void make_conf(const Space& sp, long c, Joutfg& j) {
   j.nsefi = (1 << h) * (1 << h);
   j.ndeti = 1 << std::max(j.num_singly - 1, 0);
   std::string e;
-  even_determinants(j, sp.mxopen / 2, e);
+  class_determinants(j, sp.ksym, sp.mxopen / 2, e);
   phase_and_sort(j, sp.nbf);
 }
 
@@ -865,18 +949,13 @@ void* orc_space_new(int nbf, int nelec, int ksym, long ncfg, const unsigned char
     int h = std::max(j.num_singly - 1, 0) / 2;
     j.nsefi = (1 << h) * (1 << h);
     j.ndeti = 1 << std::max(j.num_singly - 1, 0);
-    if ((j.jsym + 1) / 2 != (ksym + 1) / 2) {
-      set_err(err, errlen, "ODD-class configurations are not restated");
-      delete sp;
-      return nullptr;
-    }
     if (j.num_singly > 10) {
-      set_err(err, errlen, "EVEN class supports 0,2,4,6,8,10 open shells");
+      set_err(err, errlen, "the EVEN and ODD classes support 0,2,4,6,8,10 open shells");
       delete sp;
       return nullptr;
     }
     if (sp->prebuilt) {
-      if (!even_determinants(j, msbas, e2)) {
+      if (!class_determinants(j, ksym, msbas, e2)) {
         set_err(err, errlen, e2);
         delete sp;
         return nullptr;
@@ -928,7 +1007,7 @@ int orc_dets(void* h, long I, int raw, int* alpha, int* beta, int* ms, int* phas
     r.num_doubly = c.num_doubly;
     r.ndeti = c.ndeti;
     r.nsefi = c.nsefi;
-    even_determinants(r, msbas, e);
+    class_determinants(r, sp->ksym, msbas, e);
     c.alpha_spins = r.alpha_spins;
     c.beta_spins = r.beta_spins;
   }

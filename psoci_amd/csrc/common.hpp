@@ -74,6 +74,7 @@ struct Space {
   int nbf = 0, nelec = 0, ksym = 1, jsym0 = 1;
   int64_t ncfg = 0;
   double mult = 1.0;  // SEF combination sign: -1 when jsym and ksym parities differ
+  bool odd = false;   // ODD class (jsym class != ksym class): determinants of odd Ms
   std::vector<uint64_t> dm, sm;   // doubly / singly occupied orbital masks
   std::vector<int8_t> nopen;
   std::vector<int32_t> nsef, ndet;

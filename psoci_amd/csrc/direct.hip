@@ -152,9 +152,13 @@ __global__ void k_cnt64(int64_t n, const int32_t* __restrict__ a, int64_t* __res
 // the rest (pairs the reference skips, and the diagonal block); within each
 // group by the number of open shells new in J and by J's open-shell count, so
 // that the records the sigma kernel runs side by side enumerate alike.
+// In an ODD-class space a closed shell keeps its Ms = 0 determinant while
+// open-shell configurations have odd Ms only, so a closed-shell / open-shell
+// pair has no connected determinants: such records go to the rest.
 constexpr int kNClass = 90;
-__device__ __forceinline__ int rec_class(const PairInfo& P, uint64_t sI, uint64_t sJ, int nJ) {
-  const int g = P.kind == 3 ? 0 : (P.kind == 2 ? 1 : 2);
+__device__ __forceinline__ int rec_class(const PairInfo& P, uint64_t sI, uint64_t sJ, int nJ, int odd) {
+  const bool none = odd && ((sI == 0) != (sJ == 0));
+  const int g = none ? 2 : (P.kind == 3 ? 0 : (P.kind == 2 ? 1 : 2));
   return g * 30 + popc(sJ & ~sI) * 6 + (nJ >> 1);
 }
 
@@ -174,7 +178,7 @@ __global__ __launch_bounds__(256) void k_records(int64_t ncfg, const int64_t* __
                                                  const uint64_t* __restrict__ sm, const int8_t* __restrict__ nopen,
                                                  const int32_t* __restrict__ nsef, int nbf,
                                                  const double* __restrict__ one, const double* __restrict__ two,
-                                                 PairRec* __restrict__ rec, double* __restrict__ cdiag,
+                                                 int odd, PairRec* __restrict__ rec, double* __restrict__ cdiag,
                                                  int32_t* __restrict__ ndbl, int32_t* __restrict__ nsgl,
                                                  unsigned long long* __restrict__ nelems) {
   __shared__ int s_base[4][kNClass];
@@ -199,7 +203,7 @@ __global__ __launch_bounds__(256) void k_records(int64_t ncfg, const int64_t* __
     PairInfo P;
     bool rowI;
     setup(J, P, rowI);
-    atomicAdd(&base[rec_class(P, sI, sm[J], nopen[J])], 1);
+    atomicAdd(&base[rec_class(P, sI, sm[J], nopen[J], odd)], 1);
   }
   wave_sync_lds();
   if (lane == 0) {
@@ -224,7 +228,7 @@ __global__ __launch_bounds__(256) void k_records(int64_t ncfg, const int64_t* __
     if (act) {
       J = adj[t];
       setup(J, P, rowI);
-      cls = rec_class(P, sI, sm[J], nopen[J]);
+      cls = rec_class(P, sI, sm[J], nopen[J], odd);
       ne += (unsigned long long)nsef[J];
     }
     uint64_t pend = __ballot(act);
@@ -606,8 +610,8 @@ void build_pairs(Ctx& c) {
   ne.alloc(1);
   PSOCI_HIP(hipMemsetAsync(ne.p, 0, sizeof(unsigned long long), c.stream));
   hipLaunchKernelGGL(k_records, dim3(nblk(ncfg, 4)), dim3(256), 0, c.stream, ncfg, c.d_adjoff.p, adj.p, c.d_dm.p,
-                     c.d_sm.p, c.d_nopen.p, c.d_nsef.p, nbf, c.d_one.p, c.d_two.p, c.d_rec.p, c.d_cdiag.p, c.d_ndbl.p,
-                     c.d_nsgl.p, ne.p);
+                     c.d_sm.p, c.d_nopen.p, c.d_nsef.p, nbf, c.d_one.p, c.d_two.p, sp.odd ? 1 : 0, c.d_rec.p,
+                     c.d_cdiag.p, c.d_ndbl.p, c.d_nsgl.p, ne.p);
   PSOCI_HIP(hipGetLastError());
   unsigned long long nelems = 0;
   ne.download(&nelems, 1, c.stream);

@@ -1,7 +1,9 @@
 // Determinant/SEF space on the host: configuration masks, determinant
 // patterns and phases.
 //
-// The reference's EVEN-class generators (PsociDeterminants.C:2848-3633)
+// The reference's EVEN-class generators (PsociDeterminants.C:2848-3633),
+// used when a configuration's symmetry class (jsym + 1) / 2 equals the
+// state's (generateDeterminants, PsociDeterminants.C:1467-1503),
 // order the determinants of a configuration with s open shells as follows:
 // first the Ms = 0 determinants, in pairs (2k, 2k+1) where 2k puts the k-th
 // (s/2)-subset of the open shells -- subsets taken in colexicographic order,
@@ -13,6 +15,12 @@
 // (PsociDeterminants.C:4355-4469) gives every determinant of a configuration
 // the same phase: the parity of the number of (singly, doubly) orbital pairs
 // with the doubly occupied orbital above the singly occupied one.
+//
+// The ODD-class generators (PsociDeterminants.C:3634-4353), used when the
+// classes differ, list the determinants of odd Ms instead: for Ms = 1, 3, ...
+// <= s/2 the (s/2 + Ms)-subsets in ascending mask order, each followed by
+// its spin flip; a closed shell keeps its one determinant.  The lists have
+// the same structure, hence the same phase rule.
 #include <algorithm>
 #include <cstdlib>
 
@@ -20,7 +28,7 @@
 
 namespace psoci {
 
-static void patterns_for(int s, std::vector<uint16_t>& out) {
+static void patterns_for(int s, bool odd, std::vector<uint16_t>& out) {
   if (s == 0) {
     out.push_back(0);
     return;
@@ -33,6 +41,16 @@ static void patterns_for(int s, std::vector<uint16_t>& out) {
     for (uint32_t x = 0; x <= full; ++x)
       if (__builtin_popcount(x) == m) subs.push_back(x);
   };
+  if (odd) {
+    for (int ms = 1; ms <= h; ms += 2) {
+      collect(h + ms);
+      for (uint32_t x : subs) {
+        out.push_back((uint16_t)x);
+        out.push_back((uint16_t)(full ^ x));
+      }
+    }
+    return;
+  }
   collect(h);
   for (size_t k = 0; k < subs.size() / 2; ++k) {
     out.push_back((uint16_t)subs[k]);
@@ -63,12 +81,11 @@ void build_space(Space& sp, int nbf, int nelec, int ksym, int64_t ncfg, const ui
     PSOCI_CHECK(jsym[c] == sp.jsym0, PSOCI_ERR_UNSUPPORTED,
                 "configurations of different spatial symmetry (spin-orbit blocks) are not implemented");
   }
-  PSOCI_CHECK((sp.jsym0 + 1) / 2 == (ksym + 1) / 2, PSOCI_ERR_UNSUPPORTED,
-              "ODD-class configurations (jsym class != ksym class) are not implemented");
+  sp.odd = (sp.jsym0 + 1) / 2 != (ksym + 1) / 2;
   sp.mult = (sp.jsym0 % 2 != ksym % 2) ? -1.0 : 1.0;
   for (int s = 0; s <= 10; s += 2) {
     sp.pat_off[s] = (int)sp.pat.size();
-    patterns_for(s, sp.pat);
+    patterns_for(s, sp.odd, sp.pat);
   }
   sp.pat_off[11] = (int)sp.pat.size();
   sp.didx.assign(2047, (int16_t)-1);

@@ -115,6 +115,9 @@ int hc_row(void* h, const double* one, const double* two, int64_t I, int k, cons
           for (int c = 0; c < k; ++c) y[i * k + c] += v * xJ[j * k + c];
         }
     } else {
+      // the direct sigma's record rule: in an ODD-class space a closed shell
+      // (Ms = 0) and an open-shell configuration (odd Ms) never connect
+      if (sp.odd && ((A.nopen == 0) != (B.nopen == 0))) continue;
       const double* xrow = xt.data() + ((int64_t)(P.kind == 2 ? P.e0 : 0) * sp.nbf + (P.kind == 2 ? P.f0 : 0)) * sp.nbf;
       const PairCoef pc = pair_coef(A.nopen, B.nopen, sp.mult);
       const int16_t* dj = sp.didx.data() + (1 << B.nopen) - 1;

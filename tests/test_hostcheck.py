@@ -130,6 +130,18 @@ def test_determinant_driven_rows_opposite_sef_sign(hc, fci6):
     _rows_agree(hc, cfg, ints, range(cfg.ncfg))
 
 
+@pytest.mark.parametrize("ksym", [3, 4])
+def test_odd_class_blocks_and_rows(hc, fci6, ksym):
+    """ODD-class spaces (state symmetry class differs from the
+    configurations'): odd-Ms determinants, SEF sign + (ksym 3) or - (ksym 4);
+    blocks bit-identical to the oracle, determinant-driven rows equal to
+    SEF-pair rows."""
+    cfg, ints, _ = fci6
+    cfg = inputs.fci_configs(cfg.nbf, cfg.nelec, jsym=1, ksym=ksym)
+    _compare(hc, cfg, ints, range(cfg.ncfg))
+    _rows_agree(hc, cfg, ints, range(cfg.ncfg))
+
+
 def test_determinant_driven_rows_fci10_mrci(hc, fci10):
     cfg, ints, _ = fci10
     nopen = (cfg.occ == 1).sum(1)

@@ -37,7 +37,7 @@ def test_packed_indices_bruteforce():
             assert L.orc_two_e_index(p, q) == max(p, q) * (max(p, q) + 1) // 2 + min(p, q) + 1
 
 
-def _single_config_space(nopen, ndoub=2, nbf=None):
+def _single_config_space(nopen, ndoub=2, nbf=None, ksym=1):
     nbf = nbf or (nopen + 2 * ndoub + 1)
     occ = np.zeros((1, nbf), np.uint8)
     # interleave doubly and singly orbitals so phases are non-trivial
@@ -46,7 +46,7 @@ def _single_config_space(nopen, ndoub=2, nbf=None):
     rest = [o for o in order if o not in dbl][:nopen]
     occ[0, dbl] = 2
     occ[0, rest] = 1
-    cfg = inputs.Configs(nbf=nbf, nelec=2 * ndoub + nopen, ksym=1, occ=occ, jsym=np.ones(1, np.int32))
+    cfg = inputs.Configs(nbf=nbf, nelec=2 * ndoub + nopen, ksym=ksym, occ=occ, jsym=np.ones(1, np.int32))
     return cfg, oracle.OracleSpace(cfg), sorted(rest), sorted(dbl)
 
 
@@ -88,9 +88,50 @@ def test_det_invariants(nopen):
             assert d["alpha"][2 * j + 1] == d["beta"][2 * j] and d["beta"][2 * j + 1] == d["alpha"][2 * j]
 
 
+@pytest.mark.parametrize("nopen", [2, 4, 6])
+def test_odd_det_tables_match_reference_comments(nopen):
+    """ODD class (state symmetry 3 over configuration symmetry 1)."""
+    tab = json.load(open(os.path.join(HERE, "golden", "det_tables_odd.json")))
+    rows = tab[str(nopen)]
+    cfg, sp, singly, dbl = _single_config_space(nopen, ksym=3)
+    d = sp.dets(0, raw=True)
+    assert len(d["alpha"]) == len(rows)
+    for (ta, tb), a, b in zip(rows, d["alpha"], d["beta"]):
+        assert a == dbl + [singly[t - 1] for t in ta]
+        assert b == dbl + [singly[t - 1] for t in tb]
+
+
+@pytest.mark.parametrize("nopen", [0, 2, 4, 6, 8, 10])
+def test_odd_det_invariants(nopen):
+    cfg, sp, singly, dbl = _single_config_space(nopen, ndoub=2, ksym=3)
+    d = sp.dets(0)
+    nd = 1 << max(nopen - 1, 0)
+    assert len(d["alpha"]) == nd
+    seen = set()
+    for k, (a, b) in enumerate(zip(d["alpha"], d["beta"])):
+        assert a == sorted(a) and b == sorted(b)
+        assert sorted(set(a) & set(b)) == dbl
+        assert sorted((set(a) | set(b)) - set(dbl)) == singly
+        ms = (len(a) - len(b)) // 2
+        assert d["ms"][k] == ms
+        # odd spin projection; a closed shell keeps its Ms = 0 determinant
+        assert (ms % 2 == 1) if nopen else ms == 0
+        seen.add((tuple(a), tuple(b)))
+    assert len(seen) == nd
+    if nopen:
+        for j in range(nd // 2):
+            assert d["alpha"][2 * j + 1] == d["beta"][2 * j] and d["beta"][2 * j + 1] == d["alpha"][2 * j]
+    # together with the EVEN class: every determinant of the configuration once
+    _, sp_even, _, _ = _single_config_space(nopen, ndoub=2)
+    de = sp_even.dets(0)
+    both = seen | {(tuple(a), tuple(b)) for a, b in zip(de["alpha"], de["beta"])}
+    assert len(both) == (2 ** nopen if nopen else 1)
+
+
 @pytest.mark.parametrize("nopen", [2, 4, 6, 8])
-def test_phases_by_explicit_inversion_count(nopen):
-    cfg, sp, singly, dbl = _single_config_space(nopen, ndoub=3, nbf=nopen + 5)
+@pytest.mark.parametrize("ksym", [1, 3])
+def test_phases_by_explicit_inversion_count(nopen, ksym):
+    cfg, sp, singly, dbl = _single_config_space(nopen, ndoub=3, nbf=nopen + 5, ksym=ksym)
     raw = sp.dets(0, raw=True)
     d = sp.dets(0)
     for k in range(len(raw["alpha"])):
@@ -129,6 +170,25 @@ def test_blocks_match_bruteforce_fci6(fci6):
     assert nblocks == 4656
 
 
+@pytest.mark.parametrize("ksym", [3, 4])
+def test_odd_class_blocks_match_bruteforce(fci6, ksym):
+    cfg, ints, _ = fci6
+    cfg = inputs.fci_configs(cfg.nbf, cfg.nelec, jsym=1, ksym=ksym)
+    sp = oracle.OracleSpace(cfg, ints)
+    h, g = ints.dense()
+    B = bf.Brute(h, g)
+    ncp = 2 if ksym % 2 != 1 else 1
+    sefs = [bf.sef_vectors(sp.dets(c), cfg.nbf, int(sp.nopen[c]), ncp) for c in range(cfg.ncfg)]
+    for I in range(cfg.ncfg):
+        for J in range(I + 1):
+            st, blk = sp.block(I, J)
+            ref = bf.block(B, sefs[I], sefs[J])
+            if st:
+                np.testing.assert_allclose(blk, ref, rtol=0, atol=1e-12)
+            else:
+                assert np.abs(ref).max() < 1e-12
+
+
 def test_blocks_match_bruteforce_fci10_sample(fci10):
     """Sampled blocks of the 31,650-SEF space, including 8-open-shell ones."""
     cfg, ints, _ = fci10
@@ -160,6 +220,31 @@ def test_spectrum_in_fci_spectrum(fci6, oracle_fci6):
     w = np.linalg.eigvalsh(oracle_fci6.dense_h())
     assert len(w) == 246
     assert max(np.abs(wf - x).min() for x in w) < 1e-10
+
+
+def test_odd_class_spectrum_closed_form(fci6):
+    """ODD class: a Kramers SEF (D+ + m D-)/sqrt(2) of odd-Ms determinants
+    has the element of its positive-Ms determinant (spin-free H conserves Ms
+    and is invariant under flipping all spins), and closed shells keep their
+    Ms = 0 determinant, decoupled from the odd-Ms ones.  So the spectrum is
+    that of the FCI Hamiltonian in the sectors Ms = 1 and 3 plus that of H
+    over the closed-shell determinants -- for either SEF sign."""
+    cfg, ints, _ = fci6
+    h, g = ints.dense()
+    H, dets = bf.fci_matrix(bf.Brute(h, g), cfg.nelec)
+    nbf = cfg.nbf
+    amask = (1 << nbf) - 1
+    two_ms = np.array([bin(d & amask).count("1") - bin(d >> nbf).count("1") for d in dets])
+    closed = np.array([(d & amask) == (d >> nbf) for d in dets])
+    parts = [np.linalg.eigvalsh(H[np.ix_(m, m)]) for m in (two_ms == 2, two_ms == 6, closed)]
+    ref = np.sort(np.concatenate(parts))
+    for ksym in (3, 4):
+        c = inputs.fci_configs(nbf, cfg.nelec, jsym=1, ksym=ksym)
+        sp = oracle.OracleSpace(c, ints)
+        sp.build_h(1)
+        ev = np.linalg.eigvalsh(sp.dense_h())
+        assert len(ev) == len(ref) == 246
+        np.testing.assert_allclose(ev, ref, atol=1e-10)
 
 
 def test_stored_matrix_properties(oracle_fci6):
