@@ -94,13 +94,15 @@ int psoci_set_integrals_sifs(psoci_ctx* ctx, const char* path);
  * PsociDeterminants::computeConfigs (PsociDeterminants.C:1083-1191) and
  * computeNsefEntryPoints (PsociGAhamiltonian.C:4022-4048).
  * occ[ncfg][nbf] holds 0/1/2 occupations, jsym[ncfg] the spatial symmetry.
- * Implemented: even electron count and one common jsym (a C1 double
- * group), in the state's symmetry class ((jsym + 1) / 2 == (ksym + 1) / 2:
+ * Implemented: one common jsym (a C1 double group).  Even electron
+ * counts in the state's symmetry class ((jsym + 1) / 2 == (ksym + 1) / 2:
  * the EVEN determinants, PsociDeterminants.C:1563-1610) or not (the ODD
- * determinants of odd Ms, :1613-1658); the SEF sign follows the parities
- * of jsym and ksym (PsociHamiltonian.C:1487-1495).  Odd electron counts and
- * mixed jsym return PSOCI_ERR_UNSUPPORTED. nbf <= 64, at most 10 open
- * shells. */
+ * determinants of odd Ms, :1613-1658), the SEF sign following the parities
+ * of jsym and ksym (PsociHamiltonian.C:1487-1495); odd electron counts with
+ * one determinant per SEF (the fractional determinants, :1508-1561, alpha
+ * and beta swapped when (jsym + 1) / 2 is even).  Mixed jsym returns
+ * PSOCI_ERR_UNSUPPORTED. nbf <= 64, at most 10 open shells (8 for the
+ * direct sigma and the adjoint). */
 int psoci_set_configs(psoci_ctx* ctx, int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ,
                       const int32_t* jsym);
 

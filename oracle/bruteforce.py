@@ -117,13 +117,16 @@ def det_bits(alpha, beta, nbf):
     return b
 
 
-def sef_vectors(dets: dict, nbf: int, nopen: int, ncp: int = 1):
-    """SEF k = (D_2k +/- D_2k+1)/sqrt(2) (D_0 for closed shells), each reference
+def sef_vectors(dets: dict, nbf: int, nopen: int, ncp: int = 1, fractional: bool = False):
+    """SEF k = (D_2k +/- D_2k+1)/sqrt(2) (D_0 for closed shells; every
+    determinant by itself for odd electron counts), each reference
     determinant being (-1)^phase times its sorted bit string.  Returns a list
     of {bitstring: coefficient} dicts."""
     terms = []
     for a, b_, ph in zip(dets["alpha"], dets["beta"], dets["phase"]):
         terms.append((det_bits(a, b_, nbf), -1.0 if ph % 2 else 1.0))
+    if fractional:
+        return [{d: c} for d, c in terms]
     if nopen == 0:
         return [{terms[0][0]: terms[0][1]}]
     m = -1.0 if ncp == 2 else 1.0

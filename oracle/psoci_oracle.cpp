@@ -14,11 +14,11 @@
 //
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fopenmp).
 //
-// Scope: even electron counts, EVEN-class determinants (configuration
-// symmetry class equal to the state class, what a C1 space uses) and
-// ODD-class ones (the classes differ).  The odd-electron (fractional)
-// generators and the spin-orbit block (socig) are not restated; the oracle
-// reports an error for them.
+// Scope: EVEN-class determinants (even electron count, configuration
+// symmetry class equal to the state class, what a C1 space uses), ODD-class
+// ones (the classes differ) and the fractional class (odd electron count).
+// The spin-orbit block (socig) is not restated; configurations of different
+// symmetry never meet in the spaces the oracle is used on.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -322,10 +322,108 @@ bool odd_determinants(Joutfg& c, int msbas, std::string& err) {
   return true;
 }
 
-// generateDeterminants (PsociDeterminants.C:1467-1503): the EVEN generators
-// when the configuration's symmetry class (jsym + 1) / 2 equals the state's,
-// else the ODD ones.
-bool class_determinants(Joutfg& c, int ksym, int msbas, std::string& err) {
+// generateSpinFractionalDeterminants and its per-open-shell-count cases
+// (PsociDeterminants.C:1508-1561, 1680-2443), odd electron counts: one
+// determinant per double-group function, Ms = 1/2, -3/2, 5/2, ...  Each
+// block lists the subsets of its majority spin in colexicographic order at
+// ascending determinant indices and the minority spin's (complementary)
+// subsets at descending ones; pushBackToJoutfg with isgn = -1 swaps the
+// alpha and beta lists (:2833-2843).
+bool fractional_determinants(Joutfg& c, int msbas, int isgn, std::string& err) {
+  const int s = c.num_singly;
+  const int ndet = c.ndeti;
+  GenState g;
+  g.c = &c;
+  g.msbas = msbas;
+  g.ifiga.assign(ndet, {});
+  g.ifigb.assign(ndet, {});
+  g.ms.assign(ndet, 0);
+  // majority block: list[ndeta-1] = subset, ms label, ++ndeta
+  auto major = [&](bool alpha, int m, int start, int mslab) {
+    std::vector<std::vector<int>> subs;
+    colex_subsets(s, m, subs);
+    int ndeta = start;
+    for (auto& sub : subs) {
+      g.ms[ndeta - 1] = mslab;
+      (alpha ? g.ifiga : g.ifigb)[ndeta - 1] = g.list_of(sub);
+      ++ndeta;
+    }
+  };
+  // minority block: list[ndeta-1] = subset, --ndeta
+  auto minor = [&](bool alpha, int m, int start) {
+    std::vector<std::vector<int>> subs;
+    colex_subsets(s, m, subs);
+    int ndeta = start;
+    for (auto& sub : subs) {
+      (alpha ? g.ifiga : g.ifigb)[ndeta - 1] = g.list_of(sub);
+      --ndeta;
+    }
+  };
+  std::vector<int> all(s);
+  for (int t = 0; t < s; ++t) all[t] = t + 1;
+  auto last = [&](int d, bool alpha_full, int mslab) {
+    g.ms[d] = mslab;
+    (alpha_full ? g.ifiga : g.ifigb)[d] = g.list_of(all);
+    (alpha_full ? g.ifigb : g.ifiga)[d] = c.doubly;
+  };
+  switch (s) {
+    case 1:  // :1680-1747
+      g.ms[0] = msbas + (1 + isgn) / 2;
+      g.ifigb[0] = c.doubly;
+      g.ifiga[0] = g.list_of(all);
+      break;
+    case 3:  // :1749-1858
+      major(true, 2, 1, msbas + (1 + isgn) / 2);
+      minor(false, 1, 3);
+      last(3, false, msbas + (1 - 3 * isgn) / 2);
+      break;
+    case 5:  // :1860-2009
+      major(true, 3, 1, msbas + (1 + isgn) / 2);
+      minor(false, 2, 10);
+      major(false, 4, 11, msbas + (1 - 3 * isgn) / 2);
+      minor(true, 1, 15);
+      last(15, true, msbas + (1 + 5 * isgn) / 2);
+      break;
+    case 7:  // :2011-2193
+      major(true, 4, 1, msbas + (1 + isgn) / 2);
+      minor(false, 3, 35);
+      major(false, 5, 36, msbas + (1 - 3 * isgn) / 2);
+      minor(true, 2, 56);
+      major(true, 6, 57, msbas + (1 + 5 * isgn) / 2);
+      minor(false, 1, 63);
+      last(63, false, msbas + (1 - 7 * isgn) / 2);
+      break;
+    case 9:  // :2195-2443
+      major(true, 5, 1, msbas + (1 + isgn) / 2);
+      minor(false, 4, 126);
+      major(false, 6, 127, msbas + (1 - 3 * isgn) / 2);
+      minor(true, 3, 210);
+      major(true, 7, 211, msbas + (1 + 5 * isgn) / 2);
+      minor(false, 2, 246);
+      major(false, 8, 247, msbas + (1 - 7 * isgn) / 2);
+      minor(true, 1, 255);
+      last(255, true, msbas + (1 + 9 * isgn) / 2);
+      break;
+    default:
+      err = "the fractional class supports 1,3,5,7,9 open shells";
+      return false;
+  }
+  c.alpha_spins = isgn == 1 ? g.ifiga : g.ifigb;
+  c.beta_spins = isgn == 1 ? g.ifigb : g.ifiga;
+  c.spin_projection = g.ms;
+  c.nalpha.clear();
+  c.nbeta.clear();
+  for (auto& v : c.alpha_spins) c.nalpha.push_back((int)v.size());
+  for (auto& v : c.beta_spins) c.nbeta.push_back((int)v.size());
+  return true;
+}
+
+// generateDeterminants (PsociDeterminants.C:1467-1503): odd electron counts
+// use the fractional generators with isgn = -1 when (jsym + 1) / 2 is even;
+// otherwise the EVEN generators when the configuration's symmetry class
+// (jsym + 1) / 2 equals the state's, else the ODD ones.
+bool class_determinants(Joutfg& c, int ksym, int nelec, int msbas, std::string& err) {
+  if (nelec % 2 == 1) return fractional_determinants(c, msbas, ((c.jsym + 1) / 2) % 2 == 0 ? -1 : 1, err);
   if ((c.jsym + 1) / 2 == (ksym + 1) / 2) return even_determinants(c, msbas, err);
   return odd_determinants(c, msbas, err);
 }
@@ -752,7 +850,7 @@ void make_conf(const Space& sp, long c, Joutfg& j) {
   j.nsefi = (1 << h) * (1 << h);
   j.ndeti = 1 << std::max(j.num_singly - 1, 0);
   std::string e;
-  class_determinants(j, sp.ksym, sp.mxopen / 2, e);
+  class_determinants(j, sp.ksym, sp.nelec, sp.mxopen / 2, e);
   phase_and_sort(j, sp.nbf);
 }
 
@@ -901,11 +999,6 @@ void* orc_space_new(int nbf, int nelec, int ksym, long ncfg, const unsigned char
   sp->nelec = nelec;
   sp->ksym = ksym;
   sp->ncfg = ncfg;
-  if (nelec % 2 == 1) {
-    set_err(err, errlen, "odd electron counts (fractional class) are not restated");
-    delete sp;
-    return nullptr;
-  }
   sp->occ.assign(ncfg, std::vector<int>(nbf));
   sp->jsym.assign(jsym, jsym + ncfg);
   sp->nopen.resize(ncfg);
@@ -950,12 +1043,12 @@ void* orc_space_new(int nbf, int nelec, int ksym, long ncfg, const unsigned char
     j.nsefi = (1 << h) * (1 << h);
     j.ndeti = 1 << std::max(j.num_singly - 1, 0);
     if (j.num_singly > 10) {
-      set_err(err, errlen, "the EVEN and ODD classes support 0,2,4,6,8,10 open shells");
+      set_err(err, errlen, "at most 10 open shells");
       delete sp;
       return nullptr;
     }
     if (sp->prebuilt) {
-      if (!class_determinants(j, ksym, msbas, e2)) {
+      if (!class_determinants(j, ksym, nelec, msbas, e2)) {
         set_err(err, errlen, e2);
         delete sp;
         return nullptr;
@@ -1007,7 +1100,7 @@ int orc_dets(void* h, long I, int raw, int* alpha, int* beta, int* ms, int* phas
     r.num_doubly = c.num_doubly;
     r.ndeti = c.ndeti;
     r.nsefi = c.nsefi;
-    class_determinants(r, sp->ksym, msbas, e);
+    class_determinants(r, sp->ksym, sp->nelec, msbas, e);
     c.alpha_spins = r.alpha_spins;
     c.beta_spins = r.beta_spins;
   }

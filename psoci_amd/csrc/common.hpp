@@ -75,6 +75,7 @@ struct Space {
   int64_t ncfg = 0;
   double mult = 1.0;  // SEF combination sign: -1 when jsym and ksym parities differ
   bool odd = false;   // ODD class (jsym class != ksym class): determinants of odd Ms
+  bool frac = false;  // odd electron count: one determinant per SEF (no pairing)
   std::vector<uint64_t> dm, sm;   // doubly / singly occupied orbital masks
   std::vector<int8_t> nopen;
   std::vector<int32_t> nsef, ndet;

@@ -15,7 +15,8 @@
 // throughout (spin projection is conserved).
 //
 // SEF assembly.  SEF k of a configuration is (D_2k + mult D_2k+1)/sqrt(2)
-// (determinant 2k+1 is the spin flip of 2k), and every determinant element
+// (determinant 2k+1 is the spin flip of 2k; with an odd electron count SEF k
+// is determinant k itself, and H_ij = D(i, j)), and every determinant element
 // is invariant under flipping both determinants (the phases of detval_ao()
 // and detval_x() depend only on spatial occupations).  The reference's
 // transformCDCt (PsociHamiltonian.C:1350-1454) reduces to
@@ -27,14 +28,17 @@
 
 namespace psoci {
 
-// Coefficients of D(2i, b) in H_ij for b even (ce) and odd (co).
+// Coefficients of D(2i, b) in H_ij for b even (ce) and odd (co); with one
+// determinant per SEF (frac, odd electron count) H_ij = D(i, j).
 struct PairCoef {
   double ce, co;
 };
 
-PSOCI_HD PairCoef pair_coef(int nopen_i, int nopen_j, double mult) {
+PSOCI_HD PairCoef pair_coef(int nopen_i, int nopen_j, double mult, bool frac = false) {
   PairCoef c;
-  if (nopen_i) {
+  if (frac) {
+    c.ce = c.co = 1.0;
+  } else if (nopen_i) {
     c.ce = nopen_j ? 1.0 : kRthlf * (1.0 + mult);
     c.co = mult;
   } else {

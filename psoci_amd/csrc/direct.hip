@@ -277,6 +277,7 @@ struct DirectArgs {
   const double* cdiag;
   const int32_t* ndbl;
   const int32_t* nsgl;
+  int frac;              // one determinant per SEF (odd electron count)
   const int32_t* order;  // row processing order (null: natural)
   int chunk;             // XCD run length in workgroups (0: plain round-robin)
   int64_t row0, nrows;   // the slice of the row order to process (all rows but when tuning)
@@ -354,13 +355,14 @@ __device__ __forceinline__ void pair_rows(const DirectArgs& a, const PairRec& r,
   const ConfRec rj = a.conf[J];
   const ConfView B = view_of(rj);
   const double* __restrict__ xrow = KIND == 2 ? a.xt + ((int64_t)P.e0 * a.nbf + P.f0) * a.nbf : nullptr;
-  const PairCoef pc = pair_coef(A.nopen, B.nopen, a.mult);
+  const PairCoef pc = pair_coef(A.nopen, B.nopen, a.mult, a.frac != 0);
+  const int sh = a.frac ? 0 : 1;  // determinant b -> SEF b >> sh
   const T* __restrict__ xJ = X + (int64_t)rj.sefoff * ldx;
   for_each_conn<KIND == 3 ? 2 : 1>(A, B, aa, s_didx + (1 << B.nopen) - 1, [&](int b, uint64_t ab) {
     const double v = conn_value<KIND>(P, xrow, rowI, A, B, aa, ab, a.two);
     if (v == 0.0) return;
     const T hv = (T)(((b & 1) ? pc.co : pc.ce) * v);
-    axpy_row<T, K, VEC>(hv, xJ + (int64_t)(b >> 1) * ldx, acc);
+    axpy_row<T, K, VEC>(hv, xJ + (int64_t)(b >> sh) * ldx, acc);
   });
 }
 
@@ -384,7 +386,8 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
   const int nI = ri.nsef;
   const int nslot = kWave / nI;
   const int slot = lane / nI, i = lane - slot * nI;
-  const uint64_t aa = A.nopen ? a.detao[ri.detoff + 2 * i] : 0ull;
+  const int sh = a.frac ? 0 : 1;  // SEF i <- determinant i << sh
+  const uint64_t aa = A.nopen ? a.detao[ri.detoff + (i << sh)] : 0ull;
   T acc[K];
 #pragma unroll
   for (int v = 0; v < K; ++v) acc[v] = (T)0;
@@ -397,13 +400,13 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
     P.e0 = P.e1 = P.f0 = P.f1 = -1;
     P.v0 = a.cdiag[I];
     P.v1 = 0.0;
-    const PairCoef pc = pair_coef(A.nopen, A.nopen, a.mult);
+    const PairCoef pc = pair_coef(A.nopen, A.nopen, a.mult, a.frac != 0);
     const T* __restrict__ xI = X + (int64_t)ri.sefoff * ldx;
     for_each_conn_diag(A, aa, s_didx + (1 << A.nopen) - 1, [&](int b, uint64_t ab) {
       const double v = detval_ao(P, A, A, aa, ab, a.two);
       if (v == 0.0) return;
       const T hv = (T)(((b & 1) ? pc.co : pc.ce) * v);
-      axpy_row<T, K, VEC>(hv, xI + (int64_t)(b >> 1) * ldx, acc);
+      axpy_row<T, K, VEC>(hv, xI + (int64_t)(b >> sh) * ldx, acc);
     });
   }
   // double excitations, then single excitations (the segment's order)
@@ -439,6 +442,7 @@ DirectArgs direct_args(Ctx& c) {
   a.xt = c.d_xt.p;
   a.nbf = c.sp.nbf;
   a.mult = c.sp.mult;
+  a.frac = c.sp.frac ? 1 : 0;
   a.two = c.d_two.p;
   a.didx = c.d_didx.p;
   a.ndidx = (2 << c.sp.max_open) - 1;

@@ -302,8 +302,13 @@ PSOCI_HD double detval(const PairInfo& P, const ConfView& A, const ConfView& B, 
 // state's (ciout, :1487-1495).  aI0/aI1: alpha open shells of determinants
 // 2i and 2i+1 of the row configuration (aI1 unused without open shells),
 // aJ0/aJ1 likewise for 2j, 2j+1 of the column configuration.
+//
+// frac: odd electron count, one determinant per SEF (transformCDCt returns
+// detdet unchanged, :1386-1391): the element is D(aI0, aJ0).
 PSOCI_HD double sef_elem_ao(const PairInfo& P, const ConfView& A, const ConfView& B, uint64_t aI0, uint64_t aI1,
-                            uint64_t aJ0, uint64_t aJ1, double multi, double multj, const double* __restrict__ two) {
+                            uint64_t aJ0, uint64_t aJ1, double multi, double multj, const double* __restrict__ two,
+                            bool frac = false) {
+  if (frac) return detval_ao(P, A, B, aI0, aJ0, two);
   double ds0, ds1 = 0.0;
   if (B.nopen == 0) {
     ds0 = detval_ao(P, A, B, aI0, aJ0, two);
@@ -401,7 +406,8 @@ PSOCI_HD double detval_x(const PairInfo& P, const double* __restrict__ xrow, con
 PSOCI_HD double sef_elem_x(const PairInfo& P, const double* __restrict__ xrow, const ConfView& A, const ConfView& B,
                            uint64_t aI0,
                            uint64_t aI1, uint64_t aJ0, uint64_t aJ1, double multi, double multj,
-                           const double* __restrict__ two) {
+                           const double* __restrict__ two, bool frac = false) {
+  if (frac) return detval_x(P, xrow, A, B, aI0, aJ0, two);
   double ds0, ds1 = 0.0;
   if (B.nopen == 0) {
     ds0 = detval_x(P, xrow, A, B, aI0, aJ0, two);
@@ -423,7 +429,8 @@ PSOCI_HD double sef_elem_x(const PairInfo& P, const double* __restrict__ xrow, c
 }
 
 PSOCI_HD double sef_elem(const PairInfo& P, const ConfView& A, const ConfView& B, int i, int j, double multi,
-                         double multj, const double* __restrict__ two) {
+                         double multj, const double* __restrict__ two, bool frac = false) {
+  if (frac) return detval_ao(P, A, B, deposit(A.pat[i], A.s), deposit(B.pat[j], B.s), two);
   const uint64_t aI0 = deposit(A.pat[A.nopen ? 2 * i : 0], A.s);
   const uint64_t aI1 = A.nopen ? deposit(A.pat[2 * i + 1], A.s) : 0;
   const uint64_t aJ0 = deposit(B.pat[B.nopen ? 2 * j : 0], B.s);

@@ -111,7 +111,7 @@ __global__ void k_block_elems(int64_t total, int64_t npairs, const int64_t* __re
                               const uint64_t* __restrict__ sm, const int8_t* __restrict__ nopen,
                               const uint8_t* __restrict__ cph, const uint16_t* __restrict__ pat,
                               const int32_t* __restrict__ patoff, const int32_t* __restrict__ nsef, double mult,
-                              const double* __restrict__ two, double* __restrict__ staging) {
+                              int frac, const double* __restrict__ two, double* __restrict__ staging) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= total) return;
   int64_t p = upper_bound_i64(blk_off, npairs, e) - 1;
@@ -123,7 +123,7 @@ __global__ void k_block_elems(int64_t total, int64_t npairs, const int64_t* __re
     int i = (int)(loc / nj), j = (int)(loc - (int64_t)i * nj);
     ConfView A = conf_view(P.I, dm, sm, nopen, cph, pat, patoff);
     ConfView B = conf_view(P.J, dm, sm, nopen, cph, pat, patoff);
-    v = sef_elem(P, A, B, i, j, mult, mult, two);
+    v = sef_elem(P, A, B, i, j, mult, mult, two, frac != 0);
   }
   staging[e] = v;
 }
@@ -258,7 +258,8 @@ __global__ void k_diag_only(int64_t ncfg, const uint64_t* __restrict__ dm, const
                             const int8_t* __restrict__ nopen, const uint8_t* __restrict__ cph,
                             const uint16_t* __restrict__ pat, const int32_t* __restrict__ patoff,
                             const int32_t* __restrict__ nsef, const int64_t* __restrict__ sefoff, int nbf, double mult,
-                            const double* __restrict__ one, const double* __restrict__ two, double* __restrict__ diag) {
+                            int frac, const double* __restrict__ one, const double* __restrict__ two,
+                            double* __restrict__ diag) {
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (I >= ncfg) return;
@@ -268,14 +269,14 @@ __global__ void k_diag_only(int64_t ncfg, const uint64_t* __restrict__ dm, const
   P.kind = 1;
   P.I = P.J = (int)I;
   ConfView A = conf_view(I, dm, sm, nopen, cph, pat, patoff);
-  for (int i = lane; i < nsef[I]; i += kWave) diag[sefoff[I] + i] = sef_elem(P, A, A, i, i, mult, mult, two);
+  for (int i = lane; i < nsef[I]; i += kWave) diag[sefoff[I] + i] = sef_elem(P, A, A, i, i, mult, mult, two, frac != 0);
 }
 
 __global__ void k_single_block(PairInfo* __restrict__ Pout, int I, int J, const uint64_t* __restrict__ dm,
                                const uint64_t* __restrict__ sm, const int8_t* __restrict__ nopen,
                                const uint8_t* __restrict__ cph, const uint16_t* __restrict__ pat,
                                const int32_t* __restrict__ patoff, const int32_t* __restrict__ nsef, int nbf,
-                               double mult, const double* __restrict__ one, const double* __restrict__ two,
+                               double mult, int frac, const double* __restrict__ one, const double* __restrict__ two,
                                double* __restrict__ out) {
   __shared__ PairInfo P;
   if (threadIdx.x == 0) {
@@ -288,7 +289,7 @@ __global__ void k_single_block(PairInfo* __restrict__ Pout, int I, int J, const 
   int ni = nsef[I], nj = nsef[J];
   for (int e = threadIdx.x; e < ni * nj; e += blockDim.x) {
     int i = e / nj, j = e - i * nj;
-    out[e] = (P.kind != 0) ? sef_elem(P, A, B, i, j, mult, mult, two) : 0.0;
+    out[e] = (P.kind != 0) ? sef_elem(P, A, B, i, j, mult, mult, two, frac != 0) : 0.0;
   }
 }
 
@@ -406,7 +407,7 @@ void build_h(Ctx& c) {
   staging.alloc(nstage);
   hipLaunchKernelGGL(k_block_elems, dim3(blocks_for(nstage, 256)), dim3(256), 0, c.stream, nstage, np, boff.p,
                      L.info.p, c.d_dm.p, c.d_sm.p, c.d_nopen.p, c.d_cph.p, c.d_pat.p, c.d_patoff.p, c.d_nsef.p,
-                     sp.mult, c.d_two.p, staging.p);
+                     sp.mult, sp.frac ? 1 : 0, c.d_two.p, staging.p);
   PSOCI_HIP(hipGetLastError());
   // slots: per row of J, nl(J) lower segments, the diagonal block's upper
   // triangle, then nu(J) transposed segments
@@ -463,7 +464,7 @@ void compute_diag(Ctx& c) {
   c.d_diag.alloc(sp.nsef_total);
   hipLaunchKernelGGL(k_diag_only, dim3(blocks_for(sp.ncfg, 4)), dim3(256), 0, c.stream, sp.ncfg, c.d_dm.p, c.d_sm.p,
                      c.d_nopen.p, c.d_cph.p, c.d_pat.p, c.d_patoff.p, c.d_nsef.p, c.d_sefoff.p, sp.nbf, sp.mult,
-                     c.d_one.p, c.d_two.p, c.d_diag.p);
+                     sp.frac ? 1 : 0, c.d_one.p, c.d_two.p, c.d_diag.p);
   PSOCI_HIP(hipGetLastError());
   c.sync();
   c.have_diag = true;
@@ -477,8 +478,8 @@ void block_on_device(Ctx& c, int64_t I, int64_t J, double* host_out, int* status
   out.alloc((size_t)ni * nj);
   P.alloc(1);
   hipLaunchKernelGGL(k_single_block, dim3(1), dim3(256), 0, c.stream, P.p, (int)I, (int)J, c.d_dm.p, c.d_sm.p,
-                     c.d_nopen.p, c.d_cph.p, c.d_pat.p, c.d_patoff.p, c.d_nsef.p, sp.nbf, sp.mult, c.d_one.p,
-                     c.d_two.p, out.p);
+                     c.d_nopen.p, c.d_cph.p, c.d_pat.p, c.d_patoff.p, c.d_nsef.p, sp.nbf, sp.mult, sp.frac ? 1 : 0,
+                     c.d_one.p, c.d_two.p, out.p);
   PSOCI_HIP(hipGetLastError());
   PairInfo hp;
   out.download(host_out, (size_t)ni * nj, c.stream);

@@ -112,11 +112,12 @@ __device__ __forceinline__ void det_grad(const PairInfo& P, const ConfView& A, c
 
 // Adjoint of sef_elem: the element's four determinant pairs with the
 // transformCDCt coefficients.
+// With one determinant per SEF (frac) the element is D(aI0, aJ0) itself.
 __device__ __forceinline__ void sef_grad(const PairInfo& P, const ConfView& A, const ConfView& B, uint64_t aI0,
-                                         uint64_t aI1, uint64_t aJ0, uint64_t aJ1, double mult, double W, GradAcc& acc,
-                                         double* __restrict__ g2buf) {
+                                         uint64_t aI1, uint64_t aJ0, uint64_t aJ1, double mult, bool frac, double W,
+                                         GradAcc& acc, double* __restrict__ g2buf) {
   const double r2 = 0.5, r1 = kRthlf;
-  if (A.nopen == 0 && B.nopen == 0) {
+  if (frac || (A.nopen == 0 && B.nopen == 0)) {
     det_grad(P, A, B, aI0, aJ0, W, acc, g2buf);
   } else if (B.nopen == 0) {
     det_grad(P, A, B, aI0, aJ0, W * r1, acc, g2buf);
@@ -202,6 +203,7 @@ struct RdmArgs {
   const ConfRec* conf;
   const uint64_t* detao;
   double mult;
+  int frac;          // one determinant per SEF (odd electron count)
   int k;
   const double* C;   // [n][k]
   const double* w;   // [k]
@@ -247,11 +249,12 @@ __global__ __launch_bounds__(256) void k_rdm(RdmArgs a) {
       double W = 0.0;
       for (int v = 0; v < a.k; ++v) W += (a.w ? a.w[v] : 1.0) * ci[v] * cj[v];
       W *= fac;
-      const uint64_t aI0 = a.detao[ri.detoff + (A.nopen ? 2 * i : 0)];
-      const uint64_t aI1 = A.nopen ? a.detao[ri.detoff + 2 * i + 1] : 0ull;
-      const uint64_t aJ0 = a.detao[rj.detoff + (B.nopen ? 2 * j : 0)];
-      const uint64_t aJ1 = B.nopen ? a.detao[rj.detoff + 2 * j + 1] : 0ull;
-      sef_grad(P, A, B, aI0, aI1, aJ0, aJ1, a.mult, W, acc, a.g2);
+      const bool fr = a.frac != 0;
+      const uint64_t aI0 = a.detao[ri.detoff + (fr ? i : (A.nopen ? 2 * i : 0))];
+      const uint64_t aI1 = (!fr && A.nopen) ? a.detao[ri.detoff + 2 * i + 1] : 0ull;
+      const uint64_t aJ0 = a.detao[rj.detoff + (fr ? j : (B.nopen ? 2 * j : 0))];
+      const uint64_t aJ1 = (!fr && B.nopen) ? a.detao[rj.detoff + 2 * j + 1] : 0ull;
+      sef_grad(P, A, B, aI0, aI1, aJ0, aJ1, a.mult, fr, W, acc, a.g2);
     }
     // reduce the pair's weights over the wavefront, then spread them
     for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -285,6 +288,7 @@ void rdm_grad(Ctx& c, int k, const double* dC, const double* w, double* d_g1, do
   a.conf = c.d_conf.p;
   a.detao = c.d_detao.p;
   a.mult = c.sp.mult;
+  a.frac = c.sp.frac ? 1 : 0;
   a.k = k;
   a.C = dC;
   a.w = w ? dw.p : nullptr;

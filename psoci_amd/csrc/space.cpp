@@ -21,6 +21,13 @@
 // <= s/2 the (s/2 + Ms)-subsets in ascending mask order, each followed by
 // its spin flip; a closed shell keeps its one determinant.  The lists have
 // the same structure, hence the same phase rule.
+//
+// With an odd electron count the fractional generators (PsociDeterminants.C:
+// 1508-1561, 1680-2443) give one determinant per SEF, in blocks of
+// 2 Ms = 1, -3, 5, -7, 9: each block's majority-spin subsets in ascending
+// mask order (so for a negative block the alpha masks, their complements,
+// descend); when (jsym + 1) / 2 is even, alpha and beta are swapped
+// (pushBackToJoutfg with isgn = -1, :2833-2843).
 #include <algorithm>
 #include <cstdlib>
 
@@ -65,12 +72,23 @@ static void patterns_for(int s, bool odd, std::vector<uint16_t>& out) {
   }
 }
 
+static void patterns_frac(int s, bool swap, std::vector<uint16_t>& out) {
+  const uint32_t full = (1u << s) - 1u;
+  int sign = 1;
+  for (int t = 1; t <= s; t += 2, sign = -sign) {
+    const int m = (s + t) / 2;  // the block's majority-spin count
+    for (uint32_t x = 0; x <= full; ++x) {
+      if (__builtin_popcount(x) != m) continue;
+      const uint32_t alpha = sign > 0 ? x : (full ^ x);
+      out.push_back((uint16_t)(swap ? (full ^ alpha) : alpha));
+    }
+  }
+}
+
 void build_space(Space& sp, int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, const int32_t* jsym) {
   PSOCI_CHECK(nbf > 0 && nbf <= 64, PSOCI_ERR_UNSUPPORTED, "nbf must be in 1..64");
   PSOCI_CHECK(ncfg > 0, PSOCI_ERR_ARG, "empty configuration list");
   PSOCI_CHECK(ncfg < (int64_t)1 << 31, PSOCI_ERR_UNSUPPORTED, "more than 2^31 configurations");
-  PSOCI_CHECK(nelec % 2 == 0, PSOCI_ERR_UNSUPPORTED,
-              "odd electron counts (fractional determinant class) are not implemented");
   sp = Space();
   sp.nbf = nbf;
   sp.nelec = nelec;
@@ -81,16 +99,21 @@ void build_space(Space& sp, int nbf, int nelec, int ksym, int64_t ncfg, const ui
     PSOCI_CHECK(jsym[c] == sp.jsym0, PSOCI_ERR_UNSUPPORTED,
                 "configurations of different spatial symmetry (spin-orbit blocks) are not implemented");
   }
-  sp.odd = (sp.jsym0 + 1) / 2 != (ksym + 1) / 2;
+  sp.frac = nelec % 2 == 1;
+  sp.odd = !sp.frac && (sp.jsym0 + 1) / 2 != (ksym + 1) / 2;
   sp.mult = (sp.jsym0 % 2 != ksym % 2) ? -1.0 : 1.0;
-  for (int s = 0; s <= 10; s += 2) {
+  const bool swap = ((sp.jsym0 + 1) / 2) % 2 == 0;  // isgn = -1
+  for (int s = 0; s <= 10; ++s) {
     sp.pat_off[s] = (int)sp.pat.size();
-    patterns_for(s, sp.odd, sp.pat);
+    if (sp.frac && s % 2 == 1)
+      patterns_frac(s, swap, sp.pat);
+    else if (!sp.frac && s % 2 == 0)
+      patterns_for(s, sp.odd, sp.pat);
   }
   sp.pat_off[11] = (int)sp.pat.size();
   sp.didx.assign(2047, (int16_t)-1);
-  for (int s = 0; s <= 10; s += 2) {
-    const int nd = 1 << (s > 0 ? s - 1 : 0);
+  for (int s = 0; s <= 10; ++s) {
+    const int nd = sp.pat_off[s + 1] - sp.pat_off[s];
     for (int d = 0; d < nd; ++d) sp.didx[(1 << s) - 1 + sp.pat[sp.pat_off[s] + d]] = (int16_t)d;
   }
   sp.dm.resize(ncfg);
@@ -120,7 +143,7 @@ void build_space(Space& sp, int nbf, int nelec, int ksym, int64_t ncfg, const ui
     sp.sm[c] = s;
     sp.nopen[c] = (int8_t)n;
     int h = (n > 0 ? n - 1 : 0) / 2;
-    sp.nsef[c] = (1 << h) * (1 << h);
+    sp.nsef[c] = (1 << h) * (1 << h);  // findParams: 2^(s-2) pairs, or 2^(s-1) determinants for odd s
     sp.ndet[c] = 1 << (n > 0 ? n - 1 : 0);
     // phase: pairs (singly o, doubly d) with d above o
     int par = 0;

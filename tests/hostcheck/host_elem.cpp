@@ -42,7 +42,8 @@ int hc_block(void* h, const double* one, const double* two, int64_t I, int64_t J
   ConfView A = view(sp, I), B = view(sp, J);
   int ni = sp.nsef[I], nj = sp.nsef[J];
   for (int i = 0; i < ni; ++i)
-    for (int j = 0; j < nj; ++j) out[i * nj + j] = P.kind ? sef_elem(P, A, B, i, j, sp.mult, sp.mult, two) : 0.0;
+    for (int j = 0; j < nj; ++j)
+      out[i * nj + j] = P.kind ? sef_elem(P, A, B, i, j, sp.mult, sp.mult, two, sp.frac) : 0.0;
   return P.kind ? 1 : 0;
 }
 
@@ -62,9 +63,10 @@ int hc_block_x(void* h, const double* one, const double* two, int64_t I, int64_t
     for (int j = 0; j < nj; ++j) {
       double v = 0.0;
       if (P.kind) {
-        uint64_t aI0 = deposit(A.pat[A.nopen ? 2 * i : 0], A.s), aI1 = A.nopen ? deposit(A.pat[2 * i + 1], A.s) : 0;
-        uint64_t aJ0 = deposit(B.pat[B.nopen ? 2 * j : 0], B.s), aJ1 = B.nopen ? deposit(B.pat[2 * j + 1], B.s) : 0;
-        v = sef_elem_x(P, xrow, A, B, aI0, aI1, aJ0, aJ1, sp.mult, sp.mult, two);
+        const int si = sp.frac ? i : (A.nopen ? 2 * i : 0), sj = sp.frac ? j : (B.nopen ? 2 * j : 0);
+        uint64_t aI0 = deposit(A.pat[si], A.s), aI1 = (!sp.frac && A.nopen) ? deposit(A.pat[2 * i + 1], A.s) : 0;
+        uint64_t aJ0 = deposit(B.pat[sj], B.s), aJ1 = (!sp.frac && B.nopen) ? deposit(B.pat[2 * j + 1], B.s) : 0;
+        v = sef_elem_x(P, xrow, A, B, aI0, aI1, aJ0, aJ1, sp.mult, sp.mult, two, sp.frac);
       }
       out[i * nj + j] = v;
     }
@@ -111,7 +113,8 @@ int hc_row(void* h, const double* one, const double* two, int64_t I, int k, cons
       const ConfView RA = view(sp, R), CB = view(sp, Cc);
       for (int i = 0; i < ni; ++i)
         for (int j = 0; j < nj; ++j) {
-          double v = rowI ? sef_elem(P, RA, CB, i, j, sp.mult, sp.mult, two) : sef_elem(P, RA, CB, j, i, sp.mult, sp.mult, two);
+          double v = rowI ? sef_elem(P, RA, CB, i, j, sp.mult, sp.mult, two, sp.frac)
+                          : sef_elem(P, RA, CB, j, i, sp.mult, sp.mult, two, sp.frac);
           for (int c = 0; c < k; ++c) y[i * k + c] += v * xJ[j * k + c];
         }
     } else {
@@ -119,16 +122,17 @@ int hc_row(void* h, const double* one, const double* two, int64_t I, int k, cons
       // (Ms = 0) and an open-shell configuration (odd Ms) never connect
       if (sp.odd && ((A.nopen == 0) != (B.nopen == 0))) continue;
       const double* xrow = xt.data() + ((int64_t)(P.kind == 2 ? P.e0 : 0) * sp.nbf + (P.kind == 2 ? P.f0 : 0)) * sp.nbf;
-      const PairCoef pc = pair_coef(A.nopen, B.nopen, sp.mult);
+      const PairCoef pc = pair_coef(A.nopen, B.nopen, sp.mult, sp.frac);
       const int16_t* dj = sp.didx.data() + (1 << B.nopen) - 1;
+      const int sh = sp.frac ? 0 : 1;
       for (int i = 0; i < ni; ++i) {
-        const uint64_t aa = A.nopen ? deposit(A.pat[2 * i], A.s) : 0ull;
+        const uint64_t aa = A.nopen ? deposit(A.pat[i << sh], A.s) : 0ull;
         auto add = [&](int b, uint64_t ab) {
           if (b < 0) throw Error(PSOCI_ERR_NUMERIC, "determinant outside the class");
           const double v = P.kind == 1 ? detval_ao(P, A, A, aa, ab, two) : conn_value(P, xrow, rowI, A, B, aa, ab, two);
           const double cf = (b & 1) ? pc.co : pc.ce;
           const double hv = cf * v;
-          for (int c = 0; c < k; ++c) y[i * k + c] += hv * xJ[(b >> 1) * k + c];
+          for (int c = 0; c < k; ++c) y[i * k + c] += hv * xJ[(b >> sh) * k + c];
         };
         if (I == J)
           for_each_conn_diag(A, aa, dj, add);

@@ -121,9 +121,11 @@ def test_error_codes_not_faults(gpu_fci6):
     with pytest.raises(api.PsociError) as e:
         H.block(10**6, 0)
     assert e.value.code == api.ERR_ARG
-    odd = inputs.fci_configs(4, 3)
+    # configurations of different spatial symmetry (spin-orbit blocks) are outside the model
+    mixed = inputs.fci_configs(4, 4)
+    mixed.jsym[1] = 2
     with pytest.raises(api.PsociError) as e:
-        api.Hamiltonian(odd, inputs.make_integrals(4, 1))
+        api.Hamiltonian(mixed, inputs.make_integrals(4, 1))
     assert e.value.code == api.ERR_UNSUPPORTED
     cfg, ints, _ = inputs.workload("fci6")
     H2 = api.Hamiltonian(cfg, ints)

@@ -142,6 +142,22 @@ def test_odd_class_blocks_and_rows(hc, fci6, ksym):
     _rows_agree(hc, cfg, ints, range(cfg.ncfg))
 
 
+@pytest.mark.parametrize("jsym", [1, 3])
+def test_fractional_blocks_and_rows(hc, jsym):
+    """Odd electron count (one determinant per SEF; jsym 3 swaps alpha and
+    beta): blocks bit-identical to the oracle, determinant-driven rows equal
+    to SEF-pair rows."""
+    cfg = inputs.fci_configs(6, 5, jsym=jsym, ksym=jsym)
+    ints = inputs.make_integrals(6, 424242)
+    _compare(hc, cfg, ints, range(cfg.ncfg))
+    _rows_agree(hc, cfg, ints, range(cfg.ncfg))
+    cfg = inputs.mrcisd_configs(4, 6, 5, jsym=jsym, ksym=jsym)
+    ints = inputs.make_integrals(10, 77, n_internal=4)
+    rows = np.random.default_rng(jsym).choice(cfg.ncfg, 30, replace=False)
+    _compare(hc, cfg, ints, rows)
+    _rows_agree(hc, cfg, ints, rows)
+
+
 def test_determinant_driven_rows_fci10_mrci(hc, fci10):
     cfg, ints, _ = fci10
     nopen = (cfg.occ == 1).sum(1)

@@ -37,7 +37,7 @@ def test_packed_indices_bruteforce():
             assert L.orc_two_e_index(p, q) == max(p, q) * (max(p, q) + 1) // 2 + min(p, q) + 1
 
 
-def _single_config_space(nopen, ndoub=2, nbf=None, ksym=1):
+def _single_config_space(nopen, ndoub=2, nbf=None, ksym=1, jsym=1):
     nbf = nbf or (nopen + 2 * ndoub + 1)
     occ = np.zeros((1, nbf), np.uint8)
     # interleave doubly and singly orbitals so phases are non-trivial
@@ -46,7 +46,7 @@ def _single_config_space(nopen, ndoub=2, nbf=None, ksym=1):
     rest = [o for o in order if o not in dbl][:nopen]
     occ[0, dbl] = 2
     occ[0, rest] = 1
-    cfg = inputs.Configs(nbf=nbf, nelec=2 * ndoub + nopen, ksym=ksym, occ=occ, jsym=np.ones(1, np.int32))
+    cfg = inputs.Configs(nbf=nbf, nelec=2 * ndoub + nopen, ksym=ksym, occ=occ, jsym=np.full(1, jsym, np.int32))
     return cfg, oracle.OracleSpace(cfg), sorted(rest), sorted(dbl)
 
 
@@ -126,6 +126,47 @@ def test_odd_det_invariants(nopen):
     de = sp_even.dets(0)
     both = seen | {(tuple(a), tuple(b)) for a, b in zip(de["alpha"], de["beta"])}
     assert len(both) == (2 ** nopen if nopen else 1)
+
+
+@pytest.mark.parametrize("nopen", [1, 3, 5])
+def test_fractional_det_tables_match_reference_comments(nopen):
+    """Odd electron count, configuration symmetry 1 (isgn = +1), and with
+    symmetry 3 (isgn = -1) the same lists with alpha and beta swapped."""
+    tab = json.load(open(os.path.join(HERE, "golden", "det_tables_frac.json")))
+    rows = tab[str(nopen)]
+    for jsym, swap in ((1, False), (3, True)):
+        cfg, sp, singly, dbl = _single_config_space(nopen, jsym=jsym, ksym=jsym)
+        d = sp.dets(0, raw=True)
+        assert len(d["alpha"]) == len(rows)
+        for (ta, tb), a, b in zip(rows, d["alpha"], d["beta"]):
+            if swap:
+                ta, tb = tb, ta
+            assert a == dbl + [singly[t - 1] for t in ta]
+            assert b == dbl + [singly[t - 1] for t in tb]
+
+
+@pytest.mark.parametrize("nopen", [1, 3, 5, 7, 9])
+def test_fractional_det_invariants(nopen):
+    cfg, sp, singly, dbl = _single_config_space(nopen, ndoub=2)
+    d = sp.dets(0)
+    nd = 1 << (nopen - 1)
+    assert len(d["alpha"]) == nd
+    seen = set()
+    two_ms = []
+    for k, (a, b) in enumerate(zip(d["alpha"], d["beta"])):
+        assert a == sorted(a) and b == sorted(b)
+        assert sorted(set(a) & set(b)) == dbl
+        assert sorted((set(a) | set(b)) - set(dbl)) == singly
+        two_ms.append(len(a) - len(b))
+        seen.add((tuple(a), tuple(b)))
+    assert len(seen) == nd
+    # blocks of 2 Ms = 1, -3, 5, -7, 9 in that order, each complete
+    blocks = [2 * Ms for Ms in [0.5, -1.5, 2.5, -3.5, 4.5] if abs(2 * Ms) <= nopen]
+    from math import comb
+    expect = []
+    for t in blocks:
+        expect += [int(t)] * comb(nopen, (nopen + int(t)) // 2)
+    assert two_ms == expect
 
 
 @pytest.mark.parametrize("nopen", [2, 4, 6, 8])
@@ -245,6 +286,39 @@ def test_odd_class_spectrum_closed_form(fci6):
         ev = np.linalg.eigvalsh(sp.dense_h())
         assert len(ev) == len(ref) == 246
         np.testing.assert_allclose(ev, ref, atol=1e-10)
+
+
+@pytest.mark.parametrize("jsym", [1, 3])
+def test_fractional_spectrum_closed_form(jsym):
+    """Odd electron count: one SEF per determinant, Ms = +-1/2, -+3/2,
+    +-5/2 by blocks; spin-free H conserves Ms, so the spectrum is the FCI
+    spectrum of the sectors 2 Ms = 1, 3, 5 (a sector and its spin flip have
+    the same spectrum)."""
+    nbf, nelec = 6, 5
+    cfg = inputs.fci_configs(nbf, nelec, jsym=jsym, ksym=jsym)
+    ints = inputs.make_integrals(nbf, 424242)
+    h, g = ints.dense()
+    H, dets = bf.fci_matrix(bf.Brute(h, g), nelec)
+    amask = (1 << nbf) - 1
+    two_ms = np.array([bin(d & amask).count("1") - bin(d >> nbf).count("1") for d in dets])
+    ref = np.sort(np.concatenate([np.linalg.eigvalsh(H[np.ix_(two_ms == t, two_ms == t)]) for t in (1, 3, 5)]))
+    sp = oracle.OracleSpace(cfg, ints)
+    sp.build_h(1)
+    ev = np.linalg.eigvalsh(sp.dense_h())
+    assert len(ev) == len(ref) == 396
+    np.testing.assert_allclose(ev, ref, atol=1e-10)
+    # and element by element against the brute force
+    B = bf.Brute(h, g)
+    sefs = [bf.sef_vectors(sp.dets(c), nbf, int(sp.nopen[c]), fractional=True) for c in range(cfg.ncfg)]
+    rng = np.random.default_rng(jsym)
+    for I in rng.choice(cfg.ncfg, 25, replace=False):
+        for J in range(int(I) + 1):
+            st, blk = sp.block(int(I), J)
+            ref_b = bf.block(B, sefs[int(I)], sefs[J])
+            if st:
+                np.testing.assert_allclose(blk, ref_b, rtol=0, atol=1e-12)
+            else:
+                assert np.abs(ref_b).max() < 1e-12
 
 
 def test_stored_matrix_properties(oracle_fci6):
