@@ -234,7 +234,7 @@ def main():
         n2 = (nbf * (nbf + 1) // 2) * (nbf * (nbf + 1) // 2 + 1) // 2
         bytes32 = k * n * (4 + 4) + 16 * cfg.ncfg + 8 * n2
         bytes64 = k * n * (8 + 8) + 16 * cfg.ncfg + 8 * n2
-        kernel = f"k_direct<float,{min(k, 16)},true> (direct FP32 sigma, {k} vectors)"
+        kernel = f"k_direct<float,{min(k, 16)},true,false> (direct FP32 sigma, {k} vectors)"
     else:
         bytes32 = nnz * (4 + 4) + (n + 1) * 8 + k * n * (4 + 4)
         bytes64 = nnz * (8 + 4) + (n + 1) * 8 + k * n * (8 + 8)

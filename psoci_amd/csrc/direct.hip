@@ -344,7 +344,7 @@ __device__ __forceinline__ void axpy_row(T hv, const T* __restrict__ x, T (&acc)
 
 // Contribution of one pair record (kind KIND) to the lane's row, determinant
 // aa of I: coef * D(aa, b) * x_J[b / 2] for every connected determinant b.
-template <typename T, int K, bool VEC, int KIND>
+template <typename T, int K, bool VEC, int KIND, bool FRAC>
 __device__ __forceinline__ void pair_rows(const DirectArgs& a, const PairRec& r, const ConfView& A, uint64_t aa,
                                           const int16_t* __restrict__ s_didx, const T* __restrict__ X, int ldx,
                                           T (&acc)[K]) {
@@ -355,8 +355,8 @@ __device__ __forceinline__ void pair_rows(const DirectArgs& a, const PairRec& r,
   const ConfRec rj = a.conf[J];
   const ConfView B = view_of(rj);
   const double* __restrict__ xrow = KIND == 2 ? a.xt + ((int64_t)P.e0 * a.nbf + P.f0) * a.nbf : nullptr;
-  const PairCoef pc = pair_coef(A.nopen, B.nopen, a.mult, a.frac != 0);
-  const int sh = a.frac ? 0 : 1;  // determinant b -> SEF b >> sh
+  const PairCoef pc = pair_coef(A.nopen, B.nopen, a.mult, FRAC);
+  constexpr int sh = FRAC ? 0 : 1;  // determinant b -> SEF b >> sh
   const T* __restrict__ xJ = X + (int64_t)rj.sefoff * ldx;
   for_each_conn<KIND == 3 ? 2 : 1>(A, B, aa, s_didx + (1 << B.nopen) - 1, [&](int b, uint64_t ab) {
     const double v = conn_value<KIND>(P, xrow, rowI, A, B, aa, ab, a.two);
@@ -372,7 +372,7 @@ __device__ __forceinline__ void pair_rows(const DirectArgs& a, const PairRec& r,
 // determinants of J it connects to (for_each_conn), adding
 // coef * D(2i, b) * x_J[b / 2].  Lanes of one i are summed across slots at
 // the end; the order is fixed, so results are deterministic.
-template <typename T, int K, bool VEC>
+template <typename T, int K, bool VEC, bool FRAC>
 __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restrict__ X, int ldx, T* __restrict__ Y,
                                                 int ldy) {
   __shared__ int16_t s_didx[2047];
@@ -386,7 +386,7 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
   const int nI = ri.nsef;
   const int nslot = kWave / nI;
   const int slot = lane / nI, i = lane - slot * nI;
-  const int sh = a.frac ? 0 : 1;  // SEF i <- determinant i << sh
+  constexpr int sh = FRAC ? 0 : 1;  // SEF i <- determinant i << sh
   const uint64_t aa = A.nopen ? a.detao[ri.detoff + (i << sh)] : 0ull;
   T acc[K];
 #pragma unroll
@@ -400,7 +400,7 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
     P.e0 = P.e1 = P.f0 = P.f1 = -1;
     P.v0 = a.cdiag[I];
     P.v1 = 0.0;
-    const PairCoef pc = pair_coef(A.nopen, A.nopen, a.mult, a.frac != 0);
+    const PairCoef pc = pair_coef(A.nopen, A.nopen, a.mult, FRAC);
     const T* __restrict__ xI = X + (int64_t)ri.sefoff * ldx;
     for_each_conn_diag(A, aa, s_didx + (1 << A.nopen) - 1, [&](int b, uint64_t ab) {
       const double v = detval_ao(P, A, A, aa, ab, a.two);
@@ -414,11 +414,11 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
   const int64_t r1 = r0 + a.ndbl[I], r2 = r1 + a.nsgl[I];
   for (int64_t t0 = r0; t0 < r1; t0 += nslot) {
     const int64_t t = t0 + slot;
-    if (t < r1) pair_rows<T, K, VEC, 3>(a, a.rec[t], A, aa, s_didx, X, ldx, acc);
+    if (t < r1) pair_rows<T, K, VEC, 3, FRAC>(a, a.rec[t], A, aa, s_didx, X, ldx, acc);
   }
   for (int64_t t0 = r1; t0 < r2; t0 += nslot) {
     const int64_t t = t0 + slot;
-    if (t < r2) pair_rows<T, K, VEC, 2>(a, a.rec[t], A, aa, s_didx, X, ldx, acc);
+    if (t < r2) pair_rows<T, K, VEC, 2, FRAC>(a, a.rec[t], A, aa, s_didx, X, ldx, acc);
   }
   for (int off = nI; off < kWave; off <<= 1) {
 #pragma unroll
@@ -466,10 +466,17 @@ void launch_direct(Ctx& c, const T* X, int ldx, T* Y, int ldy, int64_t row0 = 0,
   const bool vec = ((uintptr_t)X % 16 == 0) && ((size_t)ldx * sizeof(T)) % 16 == 0;
   const int m = 8 * (a.chunk > 0 ? a.chunk : 1);
   const unsigned grid = m * nblk(nblk(a.nrows, 4), m);
-  if (vec)
-    hipLaunchKernelGGL((k_direct<T, K, true>), dim3(grid), dim3(256), 0, c.stream, a, X, ldx, Y, ldy);
-  else
-    hipLaunchKernelGGL((k_direct<T, K, false>), dim3(grid), dim3(256), 0, c.stream, a, X, ldx, Y, ldy);
+  if (a.frac) {
+    if (vec)
+      hipLaunchKernelGGL((k_direct<T, K, true, true>), dim3(grid), dim3(256), 0, c.stream, a, X, ldx, Y, ldy);
+    else
+      hipLaunchKernelGGL((k_direct<T, K, false, true>), dim3(grid), dim3(256), 0, c.stream, a, X, ldx, Y, ldy);
+  } else {
+    if (vec)
+      hipLaunchKernelGGL((k_direct<T, K, true, false>), dim3(grid), dim3(256), 0, c.stream, a, X, ldx, Y, ldy);
+    else
+      hipLaunchKernelGGL((k_direct<T, K, false, false>), dim3(grid), dim3(256), 0, c.stream, a, X, ldx, Y, ldy);
+  }
   PSOCI_HIP(hipGetLastError());
 }
 
