@@ -3,9 +3,10 @@ against the all-FP64 GPU run, and the sigma kernel's HBM roofline fraction.
 
 One "step" is one complete mixed-precision block-Davidson solve for the
 lowest 16 roots, with the space, integrals and (for the stored workload) the
-Hamiltonian already resident in HBM.  The all-FP64 solve of the same space is
-run once beside it; every root of each timed solve must agree with it to
-1e-8 Eh or the run fails.
+Hamiltonian already resident in HBM; on the direct workload 8 guard roots
+ride in its FP32 phase (a 24-column block).  The all-FP64 solve of the same
+space (16 columns, no guard roots) is run once beside it; every root of
+each timed solve must agree with it to 1e-8 Eh or the run fails.
 
 Prints ONE JSON line (rank 0, N = 1).  Usage:
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mrci10x24]
@@ -36,6 +37,8 @@ def parse():
     ap.add_argument("--workload", default="mrci10x24")
     ap.add_argument("--roots", type=int, default=16)
     ap.add_argument("--tol", type=float, default=1e-5)
+    ap.add_argument("--guard-roots", type=int, default=None,
+                    help="guard roots in the FP32 phase of the mixed solve (default 8 direct, 0 stored)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--sigma-reps", type=int, default=50)
     ap.add_argument("--sigma-only", action="store_true",
@@ -165,6 +168,7 @@ def main():
         args.warmup = 0 if direct else 1
     mode = api.DIRECT if direct else api.STORED
     k = args.roots
+    guard = args.guard_roots if args.guard_roots is not None else (8 if direct else 0)
     n_sef = int(cfg.nsef().sum())
     print(f"[bench] {args.workload}: ncfg={cfg.ncfg} nsef={n_sef} mode={'direct' if direct else 'stored'} k={k}",
           file=sys.stderr, flush=True)
@@ -203,14 +207,16 @@ def main():
     print(f"[bench] fp64 solve {fp64_s:.2f} s, {ref['stats']['iterations']} iterations", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
-        H.davidson(k, prec=api.MIXED, mode=mode, tol=args.tol, switch_tol=1e-4, want_vectors=False)
+        H.davidson(k, prec=api.MIXED, mode=mode, tol=args.tol, switch_tol=1e-4, want_vectors=False,
+                   extra_roots=guard)
     torch.cuda.synchronize()
     times, dmax, last = [], 0.0, None
     t_all = time.time()
     for s in range(args.steps):
         torch.cuda.synchronize()
         ts = time.time()
-        out = H.davidson(k, prec=api.MIXED, mode=mode, tol=args.tol, switch_tol=1e-4, want_vectors=False)
+        out = H.davidson(k, prec=api.MIXED, mode=mode, tol=args.tol, switch_tol=1e-4, want_vectors=False,
+                         extra_roots=guard)
         torch.cuda.synchronize()
         times.append(time.time() - ts)
         print(f"[bench] mixed step {s}: {times[-1]:.2f} s", file=sys.stderr, flush=True)
@@ -274,6 +280,7 @@ def main():
         "max_root_diff_eh": dmax,
         "root_diffs_eh": [float(x) for x in np.abs(last["evals"] - ref["evals"])],
         "iterations_mixed": last["stats"]["iterations"],
+        "guard_roots_fp32_phase": guard,
         "switch_iter": last["stats"]["switch_iter"],
         "iterations_fp64": ref["stats"]["iterations"],
         "sigma_calls_mixed": last["stats"]["sigma_calls"],

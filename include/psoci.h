@@ -198,7 +198,9 @@ typedef struct {
   double tol;         /* residual-norm tolerance for convergence */
   double switch_tol;  /* PSOCI_MIXED: FP32 sigma until max residual < switch_tol */
   int extra_roots;    /* guard roots carried in the block beyond nroots (convergence is
-                         tested on the nroots lowest only); 0 for the plain solve */
+                         tested on the nroots lowest only); with PSOCI_MIXED they ride in
+                         the FP32 phase only and the FP64 phase continues with nroots;
+                         0 for the plain solve */
 } psoci_dav_params;
 
 typedef struct {

@@ -262,7 +262,8 @@ class Hamiltonian:
         1 first SEFs (the reference's), 2 guess_vectors ([n][nroots]);
         restart: a CI-vector file (civec_write) read as guess_vectors, missing
         roots completed with unit vectors (the reference's --vector_restart);
-        extra_roots guard roots ride along in the block (not tested)."""
+        extra_roots guard roots ride along in the block (not tested; with
+        MIXED only in the FP32 phase)."""
         if restart is not None:
             guess_vectors, _ = civec_read(restart, self.n, nroots)
             guess = 2

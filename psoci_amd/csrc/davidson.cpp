@@ -50,14 +50,15 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
   PSOCI_CHECK(p.guess != 2 || p.extra_roots == 0, PSOCI_ERR_ARG, "guess 2 takes no extra roots");
   // working block: the wanted roots plus guard roots, which speed up the
   // convergence of the highest wanted ones; only the wanted are tested
-  const int k = (int)std::min<int64_t>((int64_t)kw + p.extra_roots, n);
+  int k = (int)std::min<int64_t>((int64_t)kw + p.extra_roots, n);
+  const int k0 = k;  // buffer width
   PSOCI_CHECK(p.prec == PSOCI_FP64 || p.prec == PSOCI_FP32 || p.prec == PSOCI_MIXED, PSOCI_ERR_ARG, "bad precision");
   PSOCI_CHECK(p.sigma_mode == PSOCI_SIGMA_STORED || p.sigma_mode == PSOCI_SIGMA_DIRECT, PSOCI_ERR_ARG, "bad sigma mode");
   PSOCI_CHECK(p.tol > 0, PSOCI_ERR_ARG, "tol must be positive");
   int mmax = p.max_subspace > 0 ? p.max_subspace : 5 * k;
   mmax = std::max(mmax, 3 * k);
   if (mmax > n) mmax = (int)n;
-  const int kc = std::min(2 * k, mmax - k > k ? mmax - k : k);
+  int kc = std::min(2 * k, mmax - k > k ? mmax - k : k);
   const int maxit = p.max_iter > 0 ? p.max_iter : 200;
   if (p.sigma_mode == PSOCI_SIGMA_STORED)
     PSOCI_CHECK(c.have_h, PSOCI_ERR_STATE, "stored sigma needs psoci_build_h first");
@@ -69,13 +70,13 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
   DevBuf<double> V, HV, R, T;
   V.alloc((size_t)n * mmax);
   HV.alloc((size_t)n * mmax);
-  R.alloc((size_t)n * k);
-  T.alloc((size_t)n * std::max(kc, k));
+  R.alloc((size_t)n * k0);
+  T.alloc((size_t)n * std::max(kc, k0));
   DevBuf<float> X32, Y32;
   const bool want32 = p.prec != PSOCI_FP64;
   if (want32) {
-    X32.alloc((size_t)n * k);
-    Y32.alloc((size_t)n * k);
+    X32.alloc((size_t)n * k0);
+    Y32.alloc((size_t)n * k0);
   }
   psoci_dav_stats S{};
   S.switch_iter = -1;
@@ -227,6 +228,10 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
                    phase32 ? "fp32" : "fp64", maxr, w[0], kw - 1, w[kw - 1]);
     if (phase32 && p.prec == PSOCI_MIXED && maxr < switch_tol) {
       // switch: collapse to the Ritz vectors and recompute their sigma in FP64
+      // guard roots ride in the FP32 phase only: the FP64 phase continues
+      // with the wanted roots (its launches cost more per column)
+      k = kw;
+      kc = std::min(2 * k, mmax - k > k ? mmax - k : k);
       collapse(k);
       phase32 = false;
       S.switch_iter = it;

@@ -485,13 +485,14 @@ void direct_any_k(Ctx& c, int k, const T* X, int ldx, T* Y, int ldy) {
   pad::blocks<T>(c, c.sp.nsef_total, k, 32, X, ldx, Y, ldy, [&](int K, const T* x, int lx, T* y, int ly) {
     switch (K) {
       case 32: launch_direct<T, 32>(c, x, lx, y, ly); break;
+      case 24: launch_direct<T, 24>(c, x, lx, y, ly); break;
       case 16: launch_direct<T, 16>(c, x, lx, y, ly); break;
       case 8: launch_direct<T, 8>(c, x, lx, y, ly); break;
       case 4: launch_direct<T, 4>(c, x, lx, y, ly); break;
       case 2: launch_direct<T, 2>(c, x, lx, y, ly); break;
       default: launch_direct<T, 1>(c, x, lx, y, ly); break;
     }
-  });
+  }, true);
 }
 
 // Schedule autotuner: the row order (natural, or by occupation string from

@@ -1,10 +1,11 @@
-// Column-blocked sigma launches at power-of-two kernel widths.
+// Column-blocked sigma launches at the kernel widths instantiated.
 //
-// The sigma kernels are instantiated for block widths 1, 2, 4, ... maxw.  A
-// block of nb columns runs as ONE launch at the next width through a
-// zero-padded staging copy, rather than as several narrower launches: a
-// launch's cost is dominated by what does not scale with the width (element
-// generation in the direct sigma, reading H in the stored one).
+// The sigma kernels are instantiated for block widths 1, 2, 4, ... maxw
+// (and 24 where w24 is set).  A block of nb columns runs as ONE launch at the
+// next width through a zero-padded staging copy, rather than as several
+// narrower launches: a launch's cost is dominated by what does not scale
+// with the width (element generation in the direct sigma, reading H in the
+// stored one).
 #pragma once
 #include "ctx.hpp"
 
@@ -32,11 +33,13 @@ __global__ void k_unpad_cols(int64_t n, int nb, int K, const T* __restrict__ Yp,
 
 // launch(K, x, ldx, y, ldy) runs the kernel of width K on n rows.
 template <typename T, class Launch>
-void blocks(Ctx& c, int64_t n, int k, int maxw, const T* X, int ldx, T* Y, int ldy, Launch&& launch) {
+void blocks(Ctx& c, int64_t n, int k, int maxw, const T* X, int ldx, T* Y, int ldy, Launch&& launch,
+            bool w24 = false) {
   for (int done = 0; done < k; done += maxw) {
     const int nb = k - done < maxw ? k - done : maxw;
     int K = 1;
     while (K < nb) K <<= 1;
+    if (w24 && nb > 16 && nb <= 24) K = 24;
     if (K == nb) {
       launch(K, X + done, ldx, Y + done, ldy);
       continue;
