@@ -41,6 +41,8 @@ def parse():
                     help="guard roots in the FP32 phase of the mixed solve (default 8 direct, 0 stored)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--sigma-reps", type=int, default=50)
+    ap.add_argument("--sigma-k", type=int, default=None,
+                    help="--sigma-only: block width to time (default: the roots)")
     ap.add_argument("--sigma-only", action="store_true",
                     help="time sigma launches only (no Davidson); for profiling runs")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -193,9 +195,10 @@ def main():
     print(f"[bench] setup {setup_s:.2f} s", file=sys.stderr, flush=True)
     if args.sigma_only:
         reps = 2 if direct else args.sigma_reps
-        ms32 = time_sigma(H, api, torch, k, api.FP32, reps, mode, 1)
-        ms64 = time_sigma(H, api, torch, k, api.FP64, reps, mode, 1)
-        print(json.dumps({"workload": args.workload, "k": k, "sigma_fp32_ms": round(ms32, 4),
+        ks = args.sigma_k or k
+        ms32 = time_sigma(H, api, torch, ks, api.FP32, reps, mode, 1)
+        ms64 = time_sigma(H, api, torch, ks, api.FP64, reps, mode, 1)
+        print(json.dumps({"workload": args.workload, "k": ks, "sigma_fp32_ms": round(ms32, 4),
                           "sigma_fp64_ms": round(ms64, 4), "setup_seconds": round(setup_s, 3)}))
         return
 
@@ -229,25 +232,29 @@ def main():
         raise SystemExit(f"mixed-precision roots differ from FP64 by {dmax:.3e} Eh (> 1e-8)")
     mixed_s = float(np.mean(times))
 
-    # dominant kernel: the FP32 sigma over the k-vector block
+    # dominant kernel: the FP32 sigma over the block of the FP32 phase (the
+    # roots plus the guard roots); the 16-column FP32 and FP64 launches beside it
     n = H.n
     reps = args.sigma_reps if not direct else 2
     warm = 3 if not direct else 1
+    kr = k + guard
     ms32 = time_sigma(H, api, torch, k, api.FP32, reps, mode, warm)
     ms64 = time_sigma(H, api, torch, k, api.FP64, reps, mode, warm)
+    msr = time_sigma(H, api, torch, kr, api.FP32, reps, mode, warm) if kr != k else ms32
     if direct:
         nbf = cfg.nbf
         n2 = (nbf * (nbf + 1) // 2) * (nbf * (nbf + 1) // 2 + 1) // 2
-        bytes32 = k * n * (4 + 4) + 16 * cfg.ncfg + 8 * n2
+        bytes_r = kr * n * (4 + 4) + 16 * cfg.ncfg + 8 * n2
         bytes64 = k * n * (8 + 8) + 16 * cfg.ncfg + 8 * n2
-        kernel = f"k_direct<float,{min(k, 16)},true,false> (direct FP32 sigma, {k} vectors)"
+        kernel = (f"k_direct<float,{kr},true,false> (direct FP32 sigma, {kr} vectors: "
+                  f"{k} roots + {guard} guard roots)")
     else:
-        bytes32 = nnz * (4 + 4) + (n + 1) * 8 + k * n * (4 + 4)
+        bytes_r = nnz * (4 + 4) + (n + 1) * 8 + kr * n * (4 + 4)
         bytes64 = nnz * (8 + 4) + (n + 1) * 8 + k * n * (8 + 8)
-        kernel = f"k_spmm<float,{k}> (stored FP32 sigma, {k} vectors)"
-    ach32 = bytes32 / (ms32 * 1e-3) / 1e9
+        kernel = f"k_spmm<float,{kr}> (stored FP32 sigma, {kr} vectors)"
+    ach32 = bytes_r / (msr * 1e-3) / 1e9
     ach64 = bytes64 / (ms64 * 1e-3) / 1e9
-    traffic = measured_traffic(args.workload, k)
+    traffic = measured_traffic(args.workload, kr)
 
     cpu = None
     if not args.no_cpu:
@@ -287,17 +294,18 @@ def main():
         "setup_seconds": round(setup_s, 4),
         "sigma_fp32_ms": round(ms32, 5),
         "sigma_fp64_ms": round(ms64, 5),
+        f"sigma_fp32_ms_k{kr}": round(msr, 5),
         "sigma_fp64_gbs": round(ach64, 1),
         "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach32 / HBM_PEAK_GBS, 4),
                      "traffic": traffic["bytes_per_launch"] if traffic else None, "kernel": kernel,
-                     "algorithmic_bytes_per_launch": int(bytes32)},
+                     "algorithmic_bytes_per_launch": int(bytes_r), "launch_ms": round(msr, 4)},
         "cpu_baseline": cpu,
         "wall_seconds_timed": round(wall, 3),
     }
     if traffic:
         line["roofline"]["traffic_source"] = traffic["source"]
-        line["roofline"]["traffic_gbs"] = round(traffic["bytes_per_launch"] / (ms32 * 1e-3) / 1e9, 1)
+        line["roofline"]["traffic_gbs"] = round(traffic["bytes_per_launch"] / (msr * 1e-3) / 1e9, 1)
     if not direct:
         line["stored_shape"] = {"fp32": H.stored_shape(api.FP32), "fp64": H.stored_shape(api.FP64)}
     if direct:
