@@ -5,7 +5,9 @@
 // next width through a zero-padded staging copy, rather than as several
 // narrower launches: a launch's cost is dominated by what does not scale
 // with the width (element generation in the direct sigma, reading H in the
-// stored one).
+// stored one).  Blocks that are columns of a wider array (a stride other
+// than the width, or not 16-byte aligned) are staged too: the kernels gather
+// whole x rows, which then share cache lines and take 16-byte loads.
 #pragma once
 #include "ctx.hpp"
 
@@ -40,7 +42,8 @@ void blocks(Ctx& c, int64_t n, int k, int maxw, const T* X, int ldx, T* Y, int l
     int K = 1;
     while (K < nb) K <<= 1;
     if (w24 && nb > 16 && nb <= 24) K = 24;
-    if (K == nb) {
+    const bool compact = ldx == K && ldy == K && (uintptr_t)(X + done) % 16 == 0 && (uintptr_t)(Y + done) % 16 == 0;
+    if (K == nb && compact) {
       launch(K, X + done, ldx, Y + done, ldy);
       continue;
     }
