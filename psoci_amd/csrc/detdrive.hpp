@@ -61,19 +61,32 @@ struct ConnIter {
   int nb, nn;
 };
 
-PSOCI_HD void conn_begin(ConnIter& it, const ConfView& I, const ConfView& J, uint64_t aa, int ex) {
-  const uint64_t C = I.s & J.s, N = J.s & ~I.s;
+// The open shells of J that are not open in I (at most four: each ends a
+// moved electron), packed as orbital (6 bits) and bit of J's pattern
+// (4 bits) per entry, and their count.  A property of the configuration
+// pair, so the direct sigma keeps it in the pair record.
+struct NewShells {
+  uint32_t norb, npos;
+  int nn;
+};
+
+PSOCI_HD NewShells new_shells(uint64_t sI, uint64_t sJ) {
+  NewShells w{0u, 0u, 0};
+  for (uint64_t r = sJ & ~sI; r; r &= r - 1, ++w.nn) {
+    const int o = ctz(r);
+    w.norb |= (uint32_t)o << (6 * w.nn);
+    w.npos |= (uint32_t)popc(sJ & ((1ull << o) - 1)) << (4 * w.nn);
+  }
+  return w;
+}
+
+PSOCI_HD void conn_begin(ConnIter& it, const ConfView& I, const ConfView& J, uint64_t aa, int ex, const NewShells& w) {
+  const uint64_t C = I.s & J.s;
   it.nb = popc(I.d) - popc(J.d) + popc(aa);
   it.base = aa & C;
-  // new open shells (at most four: each ends a moved electron), packed as
-  // orbital (6 bits) and bit of J's pattern (4 bits) per entry
-  it.norb = it.npos = 0;
-  it.nn = 0;
-  for (uint64_t r = N; r; r &= r - 1, ++it.nn) {
-    const int o = ctz(r);
-    it.norb |= (uint32_t)o << (6 * it.nn);
-    it.npos |= (uint32_t)popc(J.s & ((1ull << o) - 1)) << (4 * it.nn);
-  }
+  it.norb = w.norb;
+  it.npos = w.npos;
+  it.nn = w.nn;
   it.pbase = 0;
   for (uint64_t r = it.base; r; r &= r - 1) it.pbase |= 1u << popc(J.s & ((1ull << ctz(r)) - 1));
   const int need0 = it.nb - popc(it.base);
@@ -117,15 +130,22 @@ PSOCI_HD bool conn_next(ConnIter& it, const ConfView& J, const int16_t* __restri
   }
 }
 
-// Calls f(b, ab) for every connection of determinant aa of I in J != I.
+// Calls f(b, ab) for every connection of determinant aa of I in J != I;
+// w = new_shells(I.s, J.s).
 template <int EX = 0, class F>
-PSOCI_HD void for_each_conn(const ConfView& I, const ConfView& J, uint64_t aa, const int16_t* __restrict__ didxJ,
-                            F&& f) {
+PSOCI_HD void for_each_conn(const ConfView& I, const ConfView& J, uint64_t aa, const NewShells& w,
+                            const int16_t* __restrict__ didxJ, F&& f) {
   ConnIter it;
-  conn_begin(it, I, J, aa, EX ? EX : moved(I.d, I.s, J.d, J.s));
+  conn_begin(it, I, J, aa, EX ? EX : moved(I.d, I.s, J.d, J.s), w);
   int b;
   uint64_t ab;
   while (conn_next(it, J, didxJ, b, ab)) f(b, ab);
+}
+
+template <int EX = 0, class F>
+PSOCI_HD void for_each_conn(const ConfView& I, const ConfView& J, uint64_t aa, const int16_t* __restrict__ didxJ,
+                            F&& f) {
+  for_each_conn<EX>(I, J, aa, new_shells(I.s, J.s), didxJ, f);
 }
 
 // The diagonal block (J = I): determinant aa itself, then its alpha-beta

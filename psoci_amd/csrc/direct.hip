@@ -237,7 +237,11 @@ __global__ __launch_bounds__(256) void k_records(int64_t ncfg, const int64_t* __
       const uint64_t m = __ballot(act && cls == k0);
       if (act && cls == k0) {
         const int pos = base[k0] + popc(m & below);
-        rec[a0 + pos] = make_rec(P, J, rowI);
+        PairRec r = make_rec(P, J, rowI);
+        const NewShells w = new_shells(sI, sm[J]);
+        r.norb = w.norb;
+        r.npos = w.npos | ((uint32_t)w.nn << 16);
+        rec[a0 + pos] = r;
         if (J == (int)I) cdiag[I] = P.v0;
         act = false;
       }
@@ -358,7 +362,8 @@ __device__ __forceinline__ void pair_rows(const DirectArgs& a, const PairRec& r,
   const PairCoef pc = pair_coef(A.nopen, B.nopen, a.mult, FRAC);
   constexpr int sh = FRAC ? 0 : 1;  // determinant b -> SEF b >> sh
   const T* __restrict__ xJ = X + (int64_t)rj.sefoff * ldx;
-  for_each_conn<KIND == 3 ? 2 : 1>(A, B, aa, s_didx + (1 << B.nopen) - 1, [&](int b, uint64_t ab) {
+  const NewShells w{r.norb, r.npos & 0xffffu, (int)(r.npos >> 16)};
+  for_each_conn<KIND == 3 ? 2 : 1>(A, B, aa, w, s_didx + (1 << B.nopen) - 1, [&](int b, uint64_t ab) {
     const double v = conn_value<KIND>(P, xrow, rowI, A, B, aa, ab, a.two);
     if (v == 0.0) return;
     const T hv = (T)(((b & 1) ? pc.co : pc.ce) * v);

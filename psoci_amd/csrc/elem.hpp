@@ -54,6 +54,10 @@ struct PairRec {
   int32_t J;
   uint32_t meta;
   double v0, v1;
+  // open shells of J that are not open in the segment's configuration (at
+  // most four): orbitals, 6 bits each; their positions in J's determinant
+  // pattern, 4 bits each (bits 0-15), and their count (bits 16-18)
+  uint32_t norb, npos;
 };
 
 struct ConfView {
