@@ -106,6 +106,30 @@ int psoci_set_integrals_sifs(psoci_ctx* ctx, const char* path);
 int psoci_set_configs(psoci_ctx* ctx, int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ,
                       const int32_t* jsym);
 
+/* Configuration files, both of the reference's readers:
+ * PSOCI_CONFIGS_FILE_ORDER is PsociConfigs::readConfigsToGA (file order,
+ * PsociConfigs.C:336-495, the production path); PSOCI_CONFIGS_SORTED is
+ * PsociConfigs::readConfigs (std::set<pair<jsym, string>> order, duplicates
+ * dropped, PsociConfigs.C:143-300).  Format: a title line, then
+ * "nbf nelec ksym ncfg nopen", then one "jsym occupation" line per
+ * configuration (nbf characters 0/1/2).  Each configuration line is checked
+ * as the reference does: a symmetry and an occupation token, length nbf,
+ * only 0/1/2, at most nopen open shells, nelec electrons; the count must
+ * equal ncfg (sorted: unless duplicates were dropped).  A failure returns
+ * PSOCI_ERR_ARG with *err_line the 1-based line (title = 1, header = 2; a
+ * count mismatch names the last line), and psoci_last_error(NULL) the
+ * message.  nopen receives the largest open-shell count found (the
+ * reference adjusts the header value down to it).  Context free; NULL
+ * outputs are skipped (pass occ = jsym = NULL to query ncfg and nbf first);
+ * occ [ncfg][nbf] as for psoci_set_configs, jsym [ncfg], title at most
+ * title_cap bytes including the terminating NUL. */
+enum { PSOCI_CONFIGS_FILE_ORDER = 0, PSOCI_CONFIGS_SORTED = 1 };
+int psoci_configs_read(const char* path, int order, int* nbf, int* nelec, int* ksym, int64_t* ncfg, int* nopen,
+                       uint8_t* occ, int32_t* jsym, char* title, int title_cap, int64_t* err_line);
+/* psoci_configs_read + psoci_set_configs (the reader's message, with its
+ * line number, goes to psoci_last_error(ctx)). */
+int psoci_set_configs_file(psoci_ctx* ctx, const char* path, int order);
+
 /* Space sizes (findParams, PsociDeterminants.C:749-840): total SEFs (the H
  * dimension), total determinants, and optionally per-configuration nsef and
  * SEF offsets (arrays of length ncfg, may be NULL). */

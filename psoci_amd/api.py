@@ -17,7 +17,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .inputs import Integrals
+from .inputs import ConfigFileError, Configs, Integrals
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpsoci_amd.so")
@@ -26,6 +26,7 @@ HEADER = os.path.join(HERE, "..", "include", "psoci.h")
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_STATE, ERR_HIP, ERR_NOMEM, ERR_CONVERGE, ERR_NUMERIC = 0, -1, -2, -3, -4, -5, -6, -7
 FP64, FP32, MIXED = 0, 1, 2
 STORED, DIRECT = 0, 1
+CONFIGS_FILE_ORDER, CONFIGS_SORTED = 0, 1
 
 _lib = None
 
@@ -83,6 +84,8 @@ def load(path: str | None = None):
         "psoci_sifs_read": (I, [ctypes.c_char_p, V, V, V, V]),
         "psoci_sifs_write": (I, [ctypes.c_char_p, I, V, V, V, V, V, D, ctypes.c_char_p]),
         "psoci_set_configs": (I, [V, I, I, I, I64, V, V]),
+        "psoci_configs_read": (I, [ctypes.c_char_p, I, V, V, V, V, V, V, V, ctypes.c_char_p, I, V]),
+        "psoci_set_configs_file": (I, [V, ctypes.c_char_p, I]),
         "psoci_space_info": (I, [V, V, V, V, V]),
         "psoci_dets": (I, [V, I64, V, V, V, V]),
         "psoci_block": (I, [V, I64, I64, V, V]),
@@ -183,6 +186,20 @@ class Hamiltonian:
         self._check(self.L.psoci_set_integrals_sifs(self._h, os.fsencode(path)))
         nbf, one, two, ecore = sifs_read(path)
         self.ints = Integrals(nbf=nbf, valone=one, valtwo=two, ecore=ecore)
+        self.nnz_lower = None
+
+    def set_configs_file(self, path: str, sorted: bool = False):
+        """Replace the configuration list by a file's (psoci_set_configs_file;
+        sorted=True: the sorted, de-duplicated reader)."""
+        self._check(self.L.psoci_set_configs_file(self._h, os.fsencode(path),
+                                                  CONFIGS_SORTED if sorted else CONFIGS_FILE_ORDER))
+        self.cfg = read_configs_file(path, sorted=sorted)
+        n, nd = ctypes.c_int64(), ctypes.c_int64()
+        nsef = np.zeros(self.cfg.ncfg, np.int32)
+        off = np.zeros(self.cfg.ncfg, np.int64)
+        self._check(self.L.psoci_space_info(self._h, ctypes.byref(n), ctypes.byref(nd), _p(nsef), _p(off)))
+        self.info = SpaceInfo(n.value, nd.value, nsef, off)
+        self.n = n.value
         self.nnz_lower = None
 
     def dets(self, I: int):
@@ -345,6 +362,34 @@ def _ctxfree(rc: int):
     if rc != OK:
         msg = load().psoci_last_error(None)
         raise PsociError(rc, msg.decode() if msg else "")
+
+
+def read_configs_file(path: str, sorted: bool = False) -> Configs:
+    """A configuration file through the library's reader (psoci_configs_read):
+    file order, or sorted by (jsym, occupation) with duplicates dropped.  A
+    malformed line raises ConfigFileError carrying its line number."""
+    L = load()
+    order = CONFIGS_SORTED if sorted else CONFIGS_FILE_ORDER
+    nbf, nelec, ksym, nopen = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    ncfg, line = ctypes.c_int64(), ctypes.c_int64()
+
+    def call(occ, jsym, title, cap):
+        rc = L.psoci_configs_read(os.fsencode(path), order, ctypes.byref(nbf), ctypes.byref(nelec),
+                                  ctypes.byref(ksym), ctypes.byref(ncfg), ctypes.byref(nopen), occ, jsym, title,
+                                  cap, ctypes.byref(line))
+        if rc != OK:
+            msg = (L.psoci_last_error(None) or b"").decode()
+            if line.value > 0:
+                raise ConfigFileError(msg.split(": ", 1)[-1], line.value)
+            raise PsociError(rc, msg)
+
+    call(None, None, None, 0)
+    occ = np.zeros((ncfg.value, nbf.value), np.uint8)
+    jsym = np.zeros(ncfg.value, np.int32)
+    title = ctypes.create_string_buffer(4096)
+    call(_p(occ), _p(jsym), title, len(title))
+    return Configs(nbf=nbf.value, nelec=nelec.value, ksym=ksym.value, occ=occ, jsym=jsym,
+                   title=title.value.decode(errors="replace"), nopen_max=nopen.value)
 
 
 def sifs_read(path: str):

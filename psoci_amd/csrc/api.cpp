@@ -114,6 +114,18 @@ int psoci_set_configs(psoci_ctx* ctx, int nbf, int nelec, int ksym, int64_t ncfg
   });
 }
 
+int psoci_set_configs_file(psoci_ctx* ctx, const char* path, int order) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(order == PSOCI_CONFIGS_FILE_ORDER || order == PSOCI_CONFIGS_SORTED, PSOCI_ERR_ARG,
+                "order must be PSOCI_CONFIGS_FILE_ORDER or PSOCI_CONFIGS_SORTED");
+    psoci::ConfigFile cf;
+    psoci::read_config_file(path, order == PSOCI_CONFIGS_SORTED, cf, nullptr);
+    (void)c;
+    return psoci_set_configs(ctx, cf.nbf, cf.nelec, cf.ksym, (int64_t)cf.jsym.size(), cf.occ.data(),
+                             cf.jsym.data());
+  });
+}
+
 int psoci_space_info(psoci_ctx* ctx, int64_t* nsef_total, int64_t* ndet_total, int32_t* nsef, int64_t* nsef_offset) {
   return guarded(ctx, [&](Ctx& c) {
     PSOCI_CHECK(c.have_space, PSOCI_ERR_STATE, "no configuration list");

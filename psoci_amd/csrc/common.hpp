@@ -93,6 +93,17 @@ struct Space {
 // Builds Space from a configuration list; throws Error on unsupported input.
 void build_space(Space& sp, int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, const int32_t* jsym);
 
+// A configuration file read by the reference's rules (configs.cpp): file
+// order, or sorted by (jsym, occupation) with duplicates dropped.  Throws
+// Error(PSOCI_ERR_ARG) naming the line; *err_line (if given) receives it.
+struct ConfigFile {
+  std::string title;
+  int nbf = 0, nelec = 0, ksym = 0, nopen = 0;  // nopen: the largest found
+  std::vector<uint8_t> occ;                      // [ncfg][nbf], 0/1/2
+  std::vector<int32_t> jsym;                     // [ncfg]
+};
+void read_config_file(const char* path, bool sorted, ConfigFile& cf, int64_t* err_line);
+
 // SIFS-style integral files (sifs.cpp); errors of the context-free entry
 // points are kept per thread for psoci_last_error(NULL).
 void read_sifs(const char* path, int* nbf, double* one, double* two, double* ecore);

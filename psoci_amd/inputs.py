@@ -24,6 +24,7 @@ the same symmetry, so the reference never reaches its spin-orbit branch
 """
 from __future__ import annotations
 
+import builtins
 import itertools
 from dataclasses import dataclass, field
 from math import comb
@@ -214,30 +215,36 @@ class ConfigFileError(ValueError):
         self.line = line
 
 
-def read_configs(path: str) -> Configs:
-    """Read a configuration file in file order, validating each line as the
-    reference reader does (PsociConfigs.C:143-245): the reported line number
-    counts the title and header lines."""
-    with open(path) as f:
-        lines = f.read().splitlines()
+def read_configs(path: str, sorted: bool = False) -> Configs:
+    """Read a configuration file by the reference's rules: file order
+    (PsociConfigs::readConfigsToGA, PsociConfigs.C:336-495), or with
+    sorted=True in std::set<pair<jsym, string>> order with duplicates dropped
+    (PsociConfigs::readConfigs, PsociConfigs.C:143-300).  Every line after
+    the header is a configuration line; the reported line number counts the
+    title and header lines; the count must match the header unless
+    duplicates were dropped; nopen_max is the largest open-shell count found
+    (the reference adjusts the header's value down to it)."""
+    with open(path, newline="") as f:
+        lines = f.read().split("\n")
+    if lines and lines[-1] == "":
+        lines.pop()
+    lines = [ln[:-1] if ln.endswith("\r") else ln for ln in lines]
     if not lines:
         raise ConfigFileError("missing title", 1)
     title = lines[0]
     try:
         nbf, nelec, ksym, ncfg, nopen = (int(x) for x in lines[1].split()[:5])
     except (IndexError, ValueError):
-        raise ConfigFileError("expected 5 integers: nbf nelec ksym ncfgs nopen", 2) from None
+        raise ConfigFileError("expected 5 integers: nbf nelec ksym ncfg nopen", 2) from None
     occ, js = [], []
     for n, line in enumerate(lines[2:], start=3):
-        if not line.strip():
-            continue
         parts = line.split()
         if len(parts) < 2:
             raise ConfigFileError("expected 'jsym occupation'", n)
         try:
             sym = int(parts[0])
         except ValueError:
-            raise ConfigFileError("bad symmetry", n) from None
+            raise ConfigFileError(f"bad symmetry '{parts[0]}'", n) from None
         s = parts[1]
         if len(s) != nbf:
             raise ConfigFileError(f"configuration length {len(s)} != nbf {nbf}", n)
@@ -245,15 +252,23 @@ def read_configs(path: str) -> Configs:
             raise ConfigFileError("occupations must be 0, 1 or 2", n)
         row = [int(ch) for ch in s]
         if row.count(1) > nopen:
-            raise ConfigFileError("more open shells than the header allows", n)
+            raise ConfigFileError(f"{row.count(1)} open shells, the header allows {nopen}", n)
         if sum(row) != nelec:
             raise ConfigFileError(f"electron count {sum(row)} != {nelec}", n)
         occ.append(row)
         js.append(sym)
-    if len(occ) != ncfg:
+    dropped = False
+    if sorted:
+        keyed = {(sym, bytes(row)): None for sym, row in zip(js, occ)}
+        dropped = len(keyed) < len(occ)
+        order = builtins.sorted(keyed)
+        js = [k[0] for k in order]
+        occ = [list(k[1]) for k in order]
+    if len(occ) != ncfg and not dropped:
         raise ConfigFileError(f"found {len(occ)} configurations, header says {ncfg}", len(lines))
+    nopen_found = max((row.count(1) for row in occ), default=0)
     return Configs(nbf=nbf, nelec=nelec, ksym=ksym, occ=np.array(occ, dtype=np.uint8).reshape(-1, nbf),
-                   jsym=np.array(js, dtype=np.int32), title=title, nopen_max=nopen)
+                   jsym=np.array(js, dtype=np.int32), title=title, nopen_max=nopen_found)
 
 
 # The five benchmark workloads.  Seeds are fixed per workload.
