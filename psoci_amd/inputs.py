@@ -291,6 +291,30 @@ def workload(name: str):
     return cfg, ints, w["roots"]
 
 
+def embed(cfg: Configs, ints: Integrals, orbs, nbf: int = 64, seed: int = 7):
+    """The same CI problem on more orbitals: orbital p of (cfg, ints) becomes
+    orbital orbs[p] (orbs ascending, so determinant order and phases are
+    unchanged) of an nbf-orbital basis whose other integrals are random and
+    never reached (those orbitals stay empty)."""
+    orbs = [int(o) for o in orbs]
+    assert len(orbs) == cfg.nbf and orbs == sorted(orbs) and orbs[-1] < nbf
+    rng = np.random.Generator(np.random.PCG64(seed))
+    occ = np.zeros((cfg.ncfg, nbf), np.uint8)
+    occ[:, orbs] = cfg.occ
+    n1 = nndaf(nbf)
+    one = rng.normal(0.0, 0.3, n1)
+    two = rng.normal(0.0, 0.1, nndaf(n1))
+    m = cfg.nbf
+    for p in range(m):
+        for q in range(p + 1):
+            one[pair_index(orbs[p], orbs[q])] = ints.valone[pair_index(p, q)]
+    for p, q, r, s in itertools.product(range(m), repeat=4):
+        two[canonical_index(orbs[p], orbs[q], orbs[r], orbs[s])] = ints.valtwo[canonical_index(p, q, r, s)]
+    return (Configs(nbf=nbf, nelec=cfg.nelec, ksym=cfg.ksym, occ=occ, jsym=cfg.jsym.copy(),
+                    title=cfg.title + f" on orbitals {orbs}"),
+            Integrals(nbf=nbf, valone=one, valtwo=two, ecore=ints.ecore))
+
+
 def count_space(cfg: Configs):
     """(ncfg, sum ndet, sum nsef, sum over configs of 2^nopen)."""
     s = cfg.nopen()
@@ -299,7 +323,7 @@ def count_space(cfg: Configs):
 
 __all__ = ["Integrals", "Configs", "make_integrals", "fci_configs", "mrcisd_configs", "write_configs",
            "read_configs", "ConfigFileError", "workload", "WORKLOADS", "count_space", "nndaf", "pair_index",
-           "canonical_index", "comb"]
+           "canonical_index", "comb", "embed"]
 
 def write_sifs(path: str, ints: Integrals, title: str = "synthetic integrals") -> None:
     """Write integrals as a SIFS-style file through the library: an overlap

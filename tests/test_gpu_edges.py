@@ -1,0 +1,118 @@
+"""Edge cases of the configuration space on the GPU through the C ABI: the
+top bits of the 64-orbital occupation masks, the largest open-shell count
+(10) on the stored path and the direct sigma's limit (8), one-configuration
+spaces, and an empty list."""
+import numpy as np
+import pytest
+
+import oracle
+from psoci_amd import api, inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def test_top_orbitals_equal_the_small_problem(fci6):
+    """fci6 moved to the top orbitals of a 64-orbital basis (0, 1, 2, 61, 62,
+    63; other integrals random and unreachable): the stored H is
+    bit-identical to the 6-orbital one.  The direct sigma's records hold
+    orbitals in 6 bits with 63 as "none", so it takes at most 63 orbitals:
+    ERR_UNSUPPORTED at 64, and at 63 (orbitals 0, 1, 2, 60, 61, 62) its sigma
+    equals the 6-orbital stored sigma and its eigenvalues the 6-orbital ones."""
+    cfg, ints, _ = fci6
+    A = api.Hamiltonian(cfg, ints)
+    nnz = A.build()
+    cfg64, ints64 = inputs.embed(cfg, ints, [0, 1, 2, 61, 62, 63])
+    B = api.Hamiltonian(cfg64, ints64)
+    assert B.build() == nnz
+    for a, b in zip(A.get_h(), B.get_h()):
+        np.testing.assert_array_equal(a, b)
+    X = np.random.default_rng(64).standard_normal((A.n, 4))
+    ys = A.sigma(X, api.STORED, api.FP64)
+    with pytest.raises(api.PsociError) as e:
+        B.sigma(X, api.DIRECT, api.FP64)
+    assert e.value.code == api.ERR_UNSUPPORTED
+    B.close()
+    cfg63, ints63 = inputs.embed(cfg, ints, [0, 1, 2, 60, 61, 62], nbf=63)
+    C = api.Hamiltonian(cfg63, ints63)
+    yd = C.sigma(X, api.DIRECT, api.FP64)
+    assert np.abs(yd - ys).max() <= 1e-12 * np.abs(ys).max()
+    ea = A.davidson(4, prec=api.FP64, mode=api.STORED, tol=1e-8)
+    ec = C.davidson(4, prec=api.MIXED, mode=api.DIRECT, tol=1e-8)
+    assert ea["converged"] and ec["converged"]
+    np.testing.assert_allclose(ec["evals"], ea["evals"], atol=1e-9, rtol=0)
+    A.close()
+    C.close()
+
+
+def ten_open_list():
+    """The 10-orbital, 10-electron configurations with 8 or 10 open shells:
+    1 x 256 + 90 x 64 SEFs."""
+    full = inputs.fci_configs(10, 10)
+    keep = (full.occ == 1).sum(1) >= 8
+    return inputs.Configs(nbf=10, nelec=10, ksym=1, occ=full.occ[keep], jsym=full.jsym[keep],
+                          title="8 and 10 open shells")
+
+
+def test_ten_open_shells_stored_path_and_direct_limit():
+    cfg = ten_open_list()
+    assert cfg.ncfg == 91 and (cfg.occ == 1).sum(1).max() == 10
+    ints = inputs.make_integrals(10, 1010)
+    H = api.Hamiltonian(cfg, ints)
+    sp = oracle.OracleSpace(cfg, ints)
+    sp.build_h(8)
+    assert H.n == 256 + 90 * 64
+    assert H.build() == len(sp.val)
+    rowptr, col, val, diag = H.get_h()
+    np.testing.assert_array_equal(rowptr, sp.rowptr)
+    np.testing.assert_array_equal(col, sp.col)
+    assert np.array_equal(val, sp.val) and np.array_equal(diag, sp.diag)
+    # H conserves |Ms|, so a solve started in one Ms sector (the lowest
+    # diagonal elements) finds that sector's roots; here the lowest exact
+    # level is a multiplet spread over sectors.  Each converged root is an
+    # exact eigenpair of the oracle's matrix, and mixed equals FP64.
+    Hd = sp.dense_h()
+    w = np.linalg.eigvalsh(Hd)
+    ref = H.davidson(4, prec=api.FP64, mode=api.STORED, tol=1e-8)
+    out = H.davidson(4, prec=api.MIXED, mode=api.STORED, tol=1e-8)
+    assert ref["converged"] and out["converged"]
+    np.testing.assert_allclose(out["evals"], ref["evals"], atol=1e-9, rtol=0)
+    for e in out["evals"]:
+        assert np.abs(w - e).min() <= 1e-9
+    V = out["evecs"]
+    assert np.abs(Hd @ V - V * out["evals"]).max() <= 1e-7
+    with pytest.raises(api.PsociError) as e:
+        H.sigma(np.ones((H.n, 1)), api.DIRECT, api.FP64)
+    assert e.value.code == api.ERR_UNSUPPORTED
+    H.close()
+
+
+@pytest.mark.parametrize("occ", ["2200", "1111", "2110"])
+def test_one_configuration(occ):
+    """A one-configuration space: H is the configuration's diagonal block,
+    so the eigenvalues are those of the oracle's block."""
+    row = np.array([[int(c) for c in occ]], np.uint8)
+    cfg = inputs.Configs(nbf=4, nelec=4, ksym=1, occ=row, jsym=np.ones(1, np.int32), title=occ)
+    ints = inputs.make_integrals(4, 44)
+    H = api.Hamiltonian(cfg, ints)
+    sp = oracle.OracleSpace(cfg, ints)
+    _, blk = sp.block(0, 0)
+    w = np.linalg.eigvalsh(blk)
+    assert H.n == len(w)
+    X = np.random.default_rng(1).standard_normal((H.n, 2))
+    H.build()
+    np.testing.assert_allclose(H.sigma(X, api.STORED, api.FP64), blk @ X, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(H.sigma(X, api.DIRECT, api.FP64), blk @ X, rtol=0, atol=1e-13)
+    out = H.davidson(H.n, prec=api.FP64, mode=api.DIRECT, tol=1e-9)
+    assert out["converged"]
+    np.testing.assert_allclose(out["evals"], w, atol=1e-10, rtol=0)
+    with pytest.raises(api.PsociError) as e:
+        H.davidson(H.n + 1, prec=api.FP64, mode=api.DIRECT)
+    assert e.value.code == api.ERR_ARG
+    H.close()
+
+
+def test_empty_configuration_list():
+    cfg = inputs.Configs(nbf=4, nelec=4, ksym=1, occ=np.zeros((0, 4), np.uint8), jsym=np.zeros(0, np.int32))
+    with pytest.raises(api.PsociError) as e:
+        api.Hamiltonian(cfg, inputs.make_integrals(4, 1))
+    assert e.value.code == api.ERR_ARG

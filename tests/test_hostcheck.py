@@ -194,3 +194,22 @@ def test_subspace_eigensolver_vs_lapack(hc):
         V = a
         assert np.abs(V.T @ V - np.eye(m)).max() < 1e-12 * m
         assert np.abs(A @ V - V * w).max() < 1e-12 * scale * m
+
+
+def test_top_orbitals_nbf64_equal_the_small_problem(hc, fci6):
+    """The fci6 problem moved to orbitals 0, 1, 2, 61, 62, 63 of a 64-orbital
+    basis (the top bits of the occupation masks; the other integrals random
+    and unreachable): every block bit-identical between product and oracle,
+    and bit-identical to the 6-orbital blocks, since orbital order, and with
+    it every phase and summation order, is unchanged."""
+    cfg, ints, _ = fci6
+    cfg64, ints64 = inputs.embed(cfg, ints, [0, 1, 2, 61, 62, 63])
+    _compare(hc, cfg64, ints64, range(cfg64.ncfg))
+    _rows_agree(hc, cfg64, ints64, range(0, cfg64.ncfg, 7))
+    small = oracle.OracleSpace(cfg, ints)
+    big = oracle.OracleSpace(cfg64, ints64)
+    for I in range(0, cfg.ncfg, 5):
+        for J in range(I + 1):
+            st, a = small.block(I, J)
+            st64, b = big.block(I, J)
+            assert st == st64 and np.array_equal(a, b), (I, J)
