@@ -563,7 +563,6 @@ void build_pairs(Ctx& c) {
   const Space& sp = c.sp;
   PSOCI_CHECK(sp.max_open <= 8, PSOCI_ERR_UNSUPPORTED,
               "direct sigma supports at most 8 open shells (64 SEFs per configuration)");
-  PSOCI_CHECK(sp.nbf <= 63, PSOCI_ERR_UNSUPPORTED, "direct sigma supports at most 63 orbitals");
   PSOCI_CHECK(sp.ndet_total < ((int64_t)1 << 31), PSOCI_ERR_UNSUPPORTED, "more than 2^31 determinants");
   const int64_t ncfg = sp.ncfg;
   {

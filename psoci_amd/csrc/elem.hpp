@@ -49,7 +49,7 @@ struct ConfRec {
 // One ordered configuration pair (I, J) of the direct sigma: J, and the
 // reference-orientation block data (row = the later configuration): kind,
 // excitation orbitals, vnt[0], vnt[1].  meta bits: kind 0-1, row-is-I 2,
-// e0 3-8, e1 9-14, f0 15-20, f1 21-26 (63 = none).
+// e0 3-9, e1 10-16, f0 17-23, f1 24-30 (127 = none; orbitals 0-63).
 struct PairRec {
   int32_t J;
   uint32_t meta;
@@ -214,25 +214,25 @@ PSOCI_HD void pair_setup(PairInfo& P, int I, int J, uint64_t dI, uint64_t sI, in
 }
 
 PSOCI_HD PairRec make_rec(const PairInfo& P, int J, bool rowI) {
-  auto o6 = [](int v) { return (uint32_t)(v < 0 ? 63 : v); };
+  auto o7 = [](int v) { return (uint32_t)(v < 0 ? 127 : v); };
   PairRec r;
   r.J = J;
-  r.meta = (uint32_t)P.kind | ((rowI ? 1u : 0u) << 2) | (o6(P.e0) << 3) | (o6(P.e1) << 9) | (o6(P.f0) << 15) |
-           (o6(P.f1) << 21);
+  r.meta = (uint32_t)P.kind | ((rowI ? 1u : 0u) << 2) | (o7(P.e0) << 3) | (o7(P.e1) << 10) | (o7(P.f0) << 17) |
+           (o7(P.f1) << 24);
   r.v0 = P.v0;
   r.v1 = P.v1;
   return r;
 }
 
 PSOCI_HD void unpack_rec(const PairRec& r, PairInfo& P, int& J, bool& rowI) {
-  auto u6 = [](uint32_t v) { return (int8_t)(v == 63 ? -1 : (int)v); };
+  auto u7 = [](uint32_t v) { return (int8_t)(v == 127 ? -1 : (int)v); };
   J = r.J;
   P.kind = (int8_t)(r.meta & 3u);
   rowI = (r.meta >> 2) & 1u;
-  P.e0 = u6((r.meta >> 3) & 63u);
-  P.e1 = u6((r.meta >> 9) & 63u);
-  P.f0 = u6((r.meta >> 15) & 63u);
-  P.f1 = u6((r.meta >> 21) & 63u);
+  P.e0 = u7((r.meta >> 3) & 127u);
+  P.e1 = u7((r.meta >> 10) & 127u);
+  P.f0 = u7((r.meta >> 17) & 127u);
+  P.f1 = u7((r.meta >> 24) & 127u);
   P.v0 = r.v0;
   P.v1 = r.v1;
 }

@@ -14,10 +14,9 @@ pytestmark = pytest.mark.gpu
 def test_top_orbitals_equal_the_small_problem(fci6):
     """fci6 moved to the top orbitals of a 64-orbital basis (0, 1, 2, 61, 62,
     63; other integrals random and unreachable): the stored H is
-    bit-identical to the 6-orbital one.  The direct sigma's records hold
-    orbitals in 6 bits with 63 as "none", so it takes at most 63 orbitals:
-    ERR_UNSUPPORTED at 64, and at 63 (orbitals 0, 1, 2, 60, 61, 62) its sigma
-    equals the 6-orbital stored sigma and its eigenvalues the 6-orbital ones."""
+    bit-identical to the 6-orbital one, the direct sigma (whose pair records
+    keep orbitals in 7-bit fields) equals the 6-orbital stored sigma, and
+    the eigenvalues are the 6-orbital ones."""
     cfg, ints, _ = fci6
     A = api.Hamiltonian(cfg, ints)
     nnz = A.build()
@@ -28,20 +27,14 @@ def test_top_orbitals_equal_the_small_problem(fci6):
         np.testing.assert_array_equal(a, b)
     X = np.random.default_rng(64).standard_normal((A.n, 4))
     ys = A.sigma(X, api.STORED, api.FP64)
-    with pytest.raises(api.PsociError) as e:
-        B.sigma(X, api.DIRECT, api.FP64)
-    assert e.value.code == api.ERR_UNSUPPORTED
-    B.close()
-    cfg63, ints63 = inputs.embed(cfg, ints, [0, 1, 2, 60, 61, 62], nbf=63)
-    C = api.Hamiltonian(cfg63, ints63)
-    yd = C.sigma(X, api.DIRECT, api.FP64)
+    yd = B.sigma(X, api.DIRECT, api.FP64)
     assert np.abs(yd - ys).max() <= 1e-12 * np.abs(ys).max()
     ea = A.davidson(4, prec=api.FP64, mode=api.STORED, tol=1e-8)
-    ec = C.davidson(4, prec=api.MIXED, mode=api.DIRECT, tol=1e-8)
-    assert ea["converged"] and ec["converged"]
-    np.testing.assert_allclose(ec["evals"], ea["evals"], atol=1e-9, rtol=0)
+    eb = B.davidson(4, prec=api.MIXED, mode=api.DIRECT, tol=1e-8)
+    assert ea["converged"] and eb["converged"]
+    np.testing.assert_allclose(eb["evals"], ea["evals"], atol=1e-9, rtol=0)
     A.close()
-    C.close()
+    B.close()
 
 
 def ten_open_list():
