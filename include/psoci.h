@@ -101,8 +101,7 @@ int psoci_set_integrals_sifs(psoci_ctx* ctx, const char* path);
  * of jsym and ksym (PsociHamiltonian.C:1487-1495); odd electron counts with
  * one determinant per SEF (the fractional determinants, :1508-1561, alpha
  * and beta swapped when (jsym + 1) / 2 is even).  Mixed jsym returns
- * PSOCI_ERR_UNSUPPORTED. nbf <= 64, at most 10 open shells (8 for the
- * direct sigma and the adjoint). */
+ * PSOCI_ERR_UNSUPPORTED. nbf <= 64, at most 10 open shells. */
 int psoci_set_configs(psoci_ctx* ctx, int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ,
                       const int32_t* jsym);
 

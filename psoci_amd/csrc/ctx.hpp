@@ -55,8 +55,11 @@ struct Ctx {
   DevBuf<double> d_cdiag;     // ncfg: vnt[0] of each diagonal block
   DevBuf<int32_t> d_ndbl;     // ncfg: double-excitation records (first in each segment)
   DevBuf<int32_t> d_nsgl;     // ncfg: single-excitation records (next)
-  DevBuf<int32_t> d_order;    // ncfg: alternative row order of the direct sigma
+  DevBuf<int32_t> d_order;    // alternative row order of the direct sigma (items when split)
+  DevBuf<int32_t> d_items;    // split spaces: the natural order's work items
   bool direct_order = false;  // use d_order
+  bool direct_split = false;  // rows are work items (I << 2 | q): SEFs [64q, 64q + 64) of I
+  int64_t direct_nrows = 0;   // rows (work items) per direct sigma launch
   int direct_chunk = 0;       // XCD run length (workgroups) of the direct sigma schedule
   double direct_tune_ms = -1; // the tuner's best sample time (-1: fixed by the environment)
   DevBuf<ConfRec> d_conf;     // packed configurations

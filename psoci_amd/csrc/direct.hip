@@ -285,6 +285,7 @@ struct DirectArgs {
   const int32_t* order;  // row processing order (null: natural)
   int chunk;             // XCD run length in workgroups (0: plain round-robin)
   int64_t row0, nrows;   // the slice of the row order to process (all rows but when tuning)
+  int split;             // rows are work items I << 2 | q: SEFs [64q, 64q + 64) of I
 };
 
 __device__ __forceinline__ ConfView view_of(const ConfRec& r) {
@@ -297,7 +298,8 @@ __device__ __forceinline__ ConfView view_of(const ConfRec& r) {
   return v;
 }
 
-// The row configuration of this wavefront (-1: none).  Workgroups are dealt
+// The row (configuration, or work item of a split space) of this
+// wavefront (-1: none).  Workgroups are dealt
 // to the 8 XCDs round-robin; with chunk C > 0 each XCD takes runs of C
 // consecutive workgroups of the row order in turn (the grid is a multiple of
 // 8C), so rows that share column configurations meet in the same L2 while
@@ -376,7 +378,10 @@ __device__ __forceinline__ void pair_rows(const DirectArgs& a, const PairRec& r,
 // Each lane takes determinant 2i of I and, per record, enumerates the
 // determinants of J it connects to (for_each_conn), adding
 // coef * D(2i, b) * x_J[b / 2].  Lanes of one i are summed across slots at
-// the end; the order is fixed, so results are deterministic.
+// the end; the order is fixed, so results are deterministic.  A
+// configuration with more than 64 SEFs (9 or 10 open shells: 128 or 256)
+// is split over wavefronts of 64 SEFs each, one record slot per lane; every
+// SEF count is a power of two, so the slots divide the wavefront.
 template <typename T, int K, bool VEC, bool FRAC>
 __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restrict__ X, int ldx, T* __restrict__ Y,
                                                 int ldy) {
@@ -384,13 +389,15 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
   for (int t = threadIdx.x; t < a.ndidx; t += blockDim.x) s_didx[t] = a.didx[t];
   __syncthreads();
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t I = row_of_wave(a);
-  if (I < 0) return;  // uniform per wavefront; no block-wide barrier below
+  const int64_t item = row_of_wave(a);
+  if (item < 0) return;  // uniform per wavefront; no block-wide barrier below
+  const int64_t I = a.split ? item >> 2 : item;
+  const int i0 = a.split ? (int)(item & 3) * kWave : 0;
   const ConfRec ri = a.conf[I];
   const ConfView A = view_of(ri);
-  const int nI = ri.nsef;
-  const int nslot = kWave / nI;
-  const int slot = lane / nI, i = lane - slot * nI;
+  const int nW = ri.nsef - i0 < kWave ? ri.nsef - i0 : kWave;  // SEFs of I in this wavefront
+  const int nslot = kWave / nW;
+  const int slot = lane / nW, i = i0 + lane - slot * nW;
   constexpr int sh = FRAC ? 0 : 1;  // SEF i <- determinant i << sh
   const uint64_t aa = A.nopen ? a.detao[ri.detoff + (i << sh)] : 0ull;
   T acc[K];
@@ -425,7 +432,7 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
     const int64_t t = t0 + slot;
     if (t < r2) pair_rows<T, K, VEC, 2, FRAC>(a, a.rec[t], A, aa, s_didx, X, ldx, acc);
   }
-  for (int off = nI; off < kWave; off <<= 1) {
+  for (int off = nW; off < kWave; off <<= 1) {
 #pragma unroll
     for (int v = 0; v < K; ++v) acc[v] += __shfl_xor(acc[v], off, kWave);
   }
@@ -454,10 +461,11 @@ DirectArgs direct_args(Ctx& c) {
   a.cdiag = c.d_cdiag.p;
   a.ndbl = c.d_ndbl.p;
   a.nsgl = c.d_nsgl.p;
-  a.order = c.direct_order ? c.d_order.p : nullptr;
+  a.order = c.direct_order ? c.d_order.p : (c.direct_split ? c.d_items.p : nullptr);
   a.chunk = c.direct_chunk;
   a.row0 = 0;
-  a.nrows = c.sp.ncfg;
+  a.nrows = c.direct_nrows;
+  a.split = c.direct_split ? 1 : 0;
   return a;
 }
 
@@ -515,9 +523,9 @@ void tune_direct(Ctx& c) {
     c.direct_tune_ms = -1.0;
     return;
   }
-  const int64_t ncfg = c.sp.ncfg, n = c.sp.nsef_total;
-  const int64_t rows = std::min<int64_t>(ncfg, std::max<int64_t>(ncfg / 8, 65536));
-  const int64_t row0 = (ncfg - rows) / 2;
+  const int64_t nr = c.direct_nrows, n = c.sp.nsef_total;
+  const int64_t rows = std::min<int64_t>(nr, std::max<int64_t>(nr / 8, 65536));
+  const int64_t row0 = (nr - rows) / 2;
   const int K = 16;
   DevBuf<float> X, Y;
   X.alloc((size_t)n * K);
@@ -561,10 +569,24 @@ void tune_direct(Ctx& c) {
 
 void build_pairs(Ctx& c) {
   const Space& sp = c.sp;
-  PSOCI_CHECK(sp.max_open <= 8, PSOCI_ERR_UNSUPPORTED,
-              "direct sigma supports at most 8 open shells (64 SEFs per configuration)");
   PSOCI_CHECK(sp.ndet_total < ((int64_t)1 << 31), PSOCI_ERR_UNSUPPORTED, "more than 2^31 determinants");
   const int64_t ncfg = sp.ncfg;
+  const int max_nsef = *std::max_element(sp.nsef.begin(), sp.nsef.end());
+  PSOCI_CHECK(max_nsef <= 4 * kWave, PSOCI_ERR_UNSUPPORTED, "direct sigma supports at most 256 SEFs per configuration");
+  c.direct_split = max_nsef > kWave;
+  if (c.direct_split)
+    PSOCI_CHECK(ncfg < ((int64_t)1 << 29), PSOCI_ERR_UNSUPPORTED,
+                "direct sigma with more than 64 SEFs per configuration takes fewer than 2^29 configurations");
+  // a split space's rows are work items: configuration I << 2 | q for each
+  // 64-SEF quarter q of I
+  auto items_of = [&](const std::vector<int32_t>& rows) {
+    std::vector<int32_t> it;
+    it.reserve(rows.size());
+    for (int32_t r : rows)
+      for (int q = 0; q * kWave < sp.nsef[r]; ++q) it.push_back((int32_t)(((int64_t)r << 2) | q));
+    return it;
+  };
+  std::vector<int32_t> ord, nat_items;  // host sources of the uploads below, alive until the sync
   {
     // alternative row order: lexicographic in the occupation string from
     // orbital 0 up (in an MR-CISD list with the internal orbitals first this
@@ -578,9 +600,19 @@ void build_pairs(Ctx& c) {
       }
       key[i] = k;
     }
-    std::vector<int32_t> ord(ncfg);
+    ord.resize(ncfg);
     for (int64_t i = 0; i < ncfg; ++i) ord[i] = (int32_t)i;
     std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) { return key[x] > key[y]; });
+    if (c.direct_split) {
+      std::vector<int32_t> nat(ncfg);
+      for (int64_t i = 0; i < ncfg; ++i) nat[i] = (int32_t)i;
+      nat_items = items_of(nat);
+      ord = items_of(ord);
+      c.d_items.upload(nat_items, c.stream);
+      c.direct_nrows = (int64_t)nat_items.size();
+    } else {
+      c.direct_nrows = ncfg;
+    }
     c.d_order.upload(ord, c.stream);
   }
   c.d_detoff.upload(sp.det_off, c.stream);

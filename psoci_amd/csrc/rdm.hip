@@ -27,9 +27,11 @@ constexpr int kWave = 64;
 
 inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
+constexpr int kMaxOpen = 10;  // open shells per configuration (exchange slots)
+
 struct GradAcc {
   double wv0, wv1;
-  double xg[8];
+  double xg[kMaxOpen];
 };
 
 __device__ __forceinline__ void add_grad(double* p, double v) {
@@ -75,7 +77,7 @@ __device__ __forceinline__ void det_grad(const PairInfo& P, const ConfView& A, c
     const uint64_t T = S & ~A.d & ~(1ull << P.e0) & ~(1ull << P.f0);
     uint64_t rem = A.s;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
+    for (int t = 0; t < kMaxOpen; ++t) {
       if (rem) {
         int o = ctz(rem);
         rem &= rem - 1;
@@ -95,7 +97,7 @@ __device__ __forceinline__ void det_grad(const PairInfo& P, const ConfView& A, c
       if (P.kind == 2 && c >= 0 && ((A.s >> c) & 1ull) && c != P.e0 && c != P.f0) {
         const int slot = popc(A.s & ((1ull << c) - 1ull));
 #pragma unroll
-        for (int t = 0; t < 8; ++t)
+        for (int t = 0; t < kMaxOpen; ++t)
           if (t == slot) acc.xg[t] += s;
       } else {
         add_grad(g2buf + g2_slot(jex2, jex3, iex2, iex3), s);
@@ -183,7 +185,7 @@ __device__ void spread_pair(const PairInfo& P, const ConfView& A, const ConfView
     }
     uint64_t rem = A.s;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
+    for (int t = 0; t < kMaxOpen; ++t) {
       if (rem) {
         const int o = ctz(rem);
         rem &= rem - 1;
@@ -241,7 +243,7 @@ __global__ __launch_bounds__(256) void k_rdm(RdmArgs a) {
     GradAcc acc;
     acc.wv0 = acc.wv1 = 0.0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) acc.xg[u] = 0.0;
+    for (int u = 0; u < kMaxOpen; ++u) acc.xg[u] = 0.0;
     for (int e = lane; e < nI * nJ; e += kWave) {
       const int i = e / nJ, j = e - i * nJ;
       const double* ci = a.C + ((int64_t)ri.sefoff + i) * a.k;
@@ -261,7 +263,7 @@ __global__ __launch_bounds__(256) void k_rdm(RdmArgs a) {
       acc.wv0 += __shfl_xor(acc.wv0, off, kWave);
       acc.wv1 += __shfl_xor(acc.wv1, off, kWave);
 #pragma unroll
-      for (int u = 0; u < 8; ++u) acc.xg[u] += __shfl_xor(acc.xg[u], off, kWave);
+      for (int u = 0; u < kMaxOpen; ++u) acc.xg[u] += __shfl_xor(acc.xg[u], off, kWave);
     }
     if (lane == 0) spread_pair(P, A, B, acc, a.g1, a.g2);
   }

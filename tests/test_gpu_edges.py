@@ -1,7 +1,7 @@
 """Edge cases of the configuration space on the GPU through the C ABI: the
-top bits of the 64-orbital occupation masks, the largest open-shell count
-(10) on the stored path and the direct sigma's limit (8), one-configuration
-spaces, and an empty list."""
+top bits of the 64-orbital occupation masks, the largest open-shell counts
+(9 and 10: configurations of 256 SEFs, which the direct sigma splits over
+wavefronts), one-configuration spaces, and an empty list."""
 import numpy as np
 import pytest
 
@@ -37,17 +37,21 @@ def test_top_orbitals_equal_the_small_problem(fci6):
     B.close()
 
 
-def ten_open_list():
-    """The 10-orbital, 10-electron configurations with 8 or 10 open shells:
-    1 x 256 + 90 x 64 SEFs."""
-    full = inputs.fci_configs(10, 10)
-    keep = (full.occ == 1).sum(1) >= 8
-    return inputs.Configs(nbf=10, nelec=10, ksym=1, occ=full.occ[keep], jsym=full.jsym[keep],
-                          title="8 and 10 open shells")
+def open_list(nelec, ksym, min_open):
+    """The 10-orbital configurations with at least min_open open shells."""
+    full = inputs.fci_configs(10, nelec, jsym=1, ksym=ksym)
+    keep = (full.occ == 1).sum(1) >= min_open
+    return inputs.Configs(nbf=10, nelec=nelec, ksym=ksym, occ=full.occ[keep], jsym=full.jsym[keep],
+                          title=f"{min_open} or more open shells")
 
 
-def test_ten_open_shells_stored_path_and_direct_limit():
-    cfg = ten_open_list()
+def test_ten_open_shells_stored_and_direct():
+    """8 and 10 open shells (1 x 256 + 90 x 64 SEFs): the stored H is the
+    oracle's bit for bit; the direct sigma, which splits the 256-SEF
+    configuration over four wavefronts, equals the stored sigma; the
+    adjoint reproduces each root's energy (Euler: H is linear in the
+    integrals)."""
+    cfg = open_list(10, 1, 8)
     assert cfg.ncfg == 91 and (cfg.occ == 1).sum(1).max() == 10
     ints = inputs.make_integrals(10, 1010)
     H = api.Hamiltonian(cfg, ints)
@@ -59,23 +63,47 @@ def test_ten_open_shells_stored_path_and_direct_limit():
     np.testing.assert_array_equal(rowptr, sp.rowptr)
     np.testing.assert_array_equal(col, sp.col)
     assert np.array_equal(val, sp.val) and np.array_equal(diag, sp.diag)
+    X = np.random.default_rng(10).standard_normal((H.n, 16))
+    ys = H.sigma(X, api.STORED, api.FP64)
+    yd = H.sigma(X, api.DIRECT, api.FP64)
+    assert np.abs(yd - ys).max() <= 1e-12 * np.abs(ys).max()
+    yd32 = H.sigma(X.astype(np.float32), api.DIRECT, api.FP32)
+    assert np.abs(yd32 - ys).max() <= 2e-5 * np.abs(ys).max()
     # H conserves |Ms|, so a solve started in one Ms sector (the lowest
     # diagonal elements) finds that sector's roots; here the lowest exact
     # level is a multiplet spread over sectors.  Each converged root is an
-    # exact eigenpair of the oracle's matrix, and mixed equals FP64.
+    # exact eigenpair of the oracle's matrix, and mixed direct equals FP64.
     Hd = sp.dense_h()
     w = np.linalg.eigvalsh(Hd)
     ref = H.davidson(4, prec=api.FP64, mode=api.STORED, tol=1e-8)
-    out = H.davidson(4, prec=api.MIXED, mode=api.STORED, tol=1e-8)
+    out = H.davidson(4, prec=api.MIXED, mode=api.DIRECT, tol=1e-8)
     assert ref["converged"] and out["converged"]
     np.testing.assert_allclose(out["evals"], ref["evals"], atol=1e-9, rtol=0)
     for e in out["evals"]:
         assert np.abs(w - e).min() <= 1e-9
     V = out["evecs"]
     assert np.abs(Hd @ V - V * out["evals"]).max() <= 1e-7
-    with pytest.raises(api.PsociError) as e:
-        H.sigma(np.ones((H.n, 1)), api.DIRECT, api.FP64)
-    assert e.value.code == api.ERR_UNSUPPORTED
+    for r in range(4):
+        wr = np.zeros(4)
+        wr[r] = 1.0
+        g1, g2, _ = H.rdm(V, wr)
+        assert abs(g1 @ ints.valone + g2 @ ints.valtwo - out["evals"][r]) < 1e-9
+    H.close()
+
+
+@pytest.mark.parametrize("nelec,ksym", [(9, 1), (10, 3)])
+def test_split_configurations_other_classes(nelec, ksym):
+    """One determinant per SEF with 7 and 9 open shells (64 and 256 SEFs) and
+    the ODD class with 8 and 10: direct sigma equal to the oracle's."""
+    cfg = open_list(nelec, ksym, 7)
+    ints = inputs.make_integrals(10, 2020 + nelec)
+    H = api.Hamiltonian(cfg, ints)
+    sp = oracle.OracleSpace(cfg, ints)
+    sp.build_h(8)
+    X = np.random.default_rng(nelec).standard_normal((H.n, 8))
+    ref = sp.sigma(X)
+    yd = H.sigma(X, api.DIRECT, api.FP64)
+    assert np.abs(yd - ref).max() <= 1e-12 * np.abs(ref).max()
     H.close()
 
 

@@ -213,3 +213,19 @@ def test_top_orbitals_nbf64_equal_the_small_problem(hc, fci6):
             st, a = small.block(I, J)
             st64, b = big.block(I, J)
             assert st == st64 and np.array_equal(a, b), (I, J)
+
+
+@pytest.mark.parametrize("nelec,ksym", [(10, 1), (10, 3), (9, 1)])
+def test_nine_and_ten_open_shells(hc, nelec, ksym):
+    """Configurations of 10 orbitals with 7 to 10 open shells (64 to 256
+    SEFs, the sizes the direct sigma splits over wavefronts), EVEN, ODD and
+    one-determinant-per-SEF classes: sampled blocks bit-identical to the
+    oracle, determinant-driven rows equal to SEF-pair rows."""
+    full = inputs.fci_configs(10, nelec, jsym=1, ksym=ksym)
+    keep = (full.occ == 1).sum(1) >= 7
+    cfg = inputs.Configs(nbf=10, nelec=nelec, ksym=ksym, occ=full.occ[keep], jsym=full.jsym[keep])
+    ints = inputs.make_integrals(10, 910 + nelec + ksym)
+    nopen = (cfg.occ == 1).sum(1)
+    rows = list(np.where(nopen == nopen.max())[0][:2]) + list(np.random.default_rng(ksym).choice(cfg.ncfg, 4))
+    _compare(hc, cfg, ints, rows)
+    _rows_agree(hc, cfg, ints, rows)
