@@ -383,7 +383,31 @@ PSOCI_HD double detval_x(const PairInfo& P, const double* __restrict__ xrow, con
     uint64_t S = ma ? aI : bI;
     cnt += popc(S & between(P.e0, P.f0));
     v = P.v0;
-    for (uint64_t T = S & ~A.d & ~(1ull << P.e0) & ~(1ull << P.f0); T; T &= T - 1) v -= xrow[ctz(T)];
+    // the same-spin exchange terms in ascending orbital order, four loads
+    // issued before their subtractions (a skipped slot subtracts 0.0, which
+    // leaves v unchanged): the result is the plain loop's, bit for bit
+    uint64_t T = S & ~A.d & ~(1ull << P.e0) & ~(1ull << P.f0);
+    while (T) {
+      double x0 = 0.0, x1 = 0.0, x2 = 0.0, x3 = 0.0;
+      x0 = xrow[ctz(T)];
+      T &= T - 1;
+      if (T) {
+        x1 = xrow[ctz(T)];
+        T &= T - 1;
+      }
+      if (T) {
+        x2 = xrow[ctz(T)];
+        T &= T - 1;
+      }
+      if (T) {
+        x3 = xrow[ctz(T)];
+        T &= T - 1;
+      }
+      v -= x0;
+      v -= x1;
+      v -= x2;
+      v -= x3;
+    }
   } else if (KIND == 2 || ma == 1) {
     int iex2 = ctz(xa), jex2 = ctz(xb);
     int iex3 = ctz(aI & ~aJ), jex3 = ctz(bI & ~bJ);
