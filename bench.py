@@ -174,17 +174,22 @@ def main():
     n_sef = int(cfg.nsef().sum())
     print(f"[bench] {args.workload}: ncfg={cfg.ncfg} nsef={n_sef} mode={'direct' if direct else 'stored'} k={k}",
           file=sys.stderr, flush=True)
-    # the integrals travel as the reference's do: written to a SIFS-style file
-    # and read back through the library (outside the timed region)
+    # the inputs travel as the reference's do: the configuration list and a
+    # SIFS-style integral file, written here and read back by the library
+    # (outside the timed region)
     import tempfile
-    sifs_dir = tempfile.mkdtemp(prefix="psoci_bench_")
-    sifs_path = os.path.join(sifs_dir, f"{args.workload}.sifs")
+    in_dir = tempfile.mkdtemp(prefix="psoci_bench_")
+    sifs_path = os.path.join(in_dir, f"{args.workload}.sifs")
+    cfg_path = os.path.join(in_dir, f"{args.workload}.configs")
     inputs.write_sifs(sifs_path, ints)
+    inputs.write_configs(cfg_path, cfg)
     t0 = time.time()
-    H = api.Hamiltonian(cfg, ints)
-    H.set_integrals_sifs(sifs_path)
+    H = api.Hamiltonian.from_files(cfg_path, sifs_path)
+    if not (np.array_equal(H.cfg.occ, cfg.occ) and np.array_equal(H.cfg.jsym, cfg.jsym)):
+        raise SystemExit("configuration file read back differs from the list written")
     os.remove(sifs_path)
-    os.rmdir(sifs_dir)
+    os.remove(cfg_path)
+    os.rmdir(in_dir)
     if direct:
         npairs, nelems = H.direct_info()
         nnz = None
@@ -279,7 +284,8 @@ def main():
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": "f32 sigma -> f64 (mixed); f64 subspace",
-        "data": "synthetic (seeded integrals written to and read from a SIFS-style file; seeded configurations)",
+        "data": ("synthetic (seeded integrals and configuration list, written to a SIFS-style file and a "
+                 "configuration file and read back by the library)"),
         "config": {"workload": desc, "roots": k, "n_sef": n, "n_cfg": cfg.ncfg,
                    "sigma": "direct" if direct else "stored"},
         "fp64_seconds": round(fp64_s, 6),

@@ -139,20 +139,40 @@ class Hamiltonian:
     (optionally) the stored Hamiltonian."""
 
     def __init__(self, cfg, ints, device: int = 0):
+        self._open(device)
+        self.cfg = cfg
+        self.ints = ints
+        self._check(self.L.psoci_set_configs(self._h, cfg.nbf, cfg.nelec, cfg.ksym, cfg.ncfg, _p(cfg.occ),
+                                             _p(cfg.jsym)))
+        self.set_integrals(ints)
+        self._space_info()
+
+    @classmethod
+    def from_files(cls, configs_path: str, sifs_path: str, sorted: bool = False, device: int = 0):
+        """The reference's input path: a configuration file
+        (psoci_set_configs_file) and a SIFS-style integral file
+        (psoci_set_integrals_sifs), both read by the library."""
+        self = cls.__new__(cls)
+        self._open(device)
+        self._check(self.L.psoci_set_configs_file(self._h, os.fsencode(configs_path),
+                                                  CONFIGS_SORTED if sorted else CONFIGS_FILE_ORDER))
+        self.cfg = read_configs_file(configs_path, sorted=sorted)
+        self.set_integrals_sifs(sifs_path)
+        self._space_info()
+        return self
+
+    def _open(self, device: int):
         self.L = load()
         h = ctypes.c_void_p()
         rc = self.L.psoci_ctx_create(device, ctypes.byref(h))
         if rc != OK:
             raise PsociError(rc, "psoci_ctx_create failed (no HIP device?)")
         self._h = h
-        self.cfg = cfg
-        self.ints = ints
-        self._check(self.L.psoci_set_configs(self._h, cfg.nbf, cfg.nelec, cfg.ksym, cfg.ncfg, _p(cfg.occ),
-                                             _p(cfg.jsym)))
-        self.set_integrals(ints)
+
+    def _space_info(self):
         n, nd = ctypes.c_int64(), ctypes.c_int64()
-        nsef = np.zeros(cfg.ncfg, np.int32)
-        off = np.zeros(cfg.ncfg, np.int64)
+        nsef = np.zeros(self.cfg.ncfg, np.int32)
+        off = np.zeros(self.cfg.ncfg, np.int64)
         self._check(self.L.psoci_space_info(self._h, ctypes.byref(n), ctypes.byref(nd), _p(nsef), _p(off)))
         self.info = SpaceInfo(n.value, nd.value, nsef, off)
         self.n = n.value
@@ -194,13 +214,7 @@ class Hamiltonian:
         self._check(self.L.psoci_set_configs_file(self._h, os.fsencode(path),
                                                   CONFIGS_SORTED if sorted else CONFIGS_FILE_ORDER))
         self.cfg = read_configs_file(path, sorted=sorted)
-        n, nd = ctypes.c_int64(), ctypes.c_int64()
-        nsef = np.zeros(self.cfg.ncfg, np.int32)
-        off = np.zeros(self.cfg.ncfg, np.int64)
-        self._check(self.L.psoci_space_info(self._h, ctypes.byref(n), ctypes.byref(nd), _p(nsef), _p(off)))
-        self.info = SpaceInfo(n.value, nd.value, nsef, off)
-        self.n = n.value
-        self.nnz_lower = None
+        self._space_info()
 
     def dets(self, I: int):
         nd = 1 << max(int((self.cfg.occ[I] == 1).sum()) - 1, 0)

@@ -34,25 +34,36 @@ namespace {
 
 inline bool is_space(char c) { return c == ' ' || c == '\t' || c == '\r' || c == '\v' || c == '\f'; }
 
-// whitespace-separated tokens of [b, e)
-std::vector<std::string> tokens(const char* b, const char* e) {
-  std::vector<std::string> out;
-  while (b < e) {
-    while (b < e && is_space(*b)) ++b;
-    const char* t = b;
-    while (b < e && !is_space(*b)) ++b;
-    if (b > t) out.emplace_back(t, b);
-  }
-  return out;
+struct Tok {
+  const char *b = nullptr, *e = nullptr;
+  size_t size() const { return (size_t)(e - b); }
+  std::string str() const { return std::string(b, e); }
+};
+
+// the next whitespace-separated token of [p, e), advancing p (empty at the end)
+inline Tok next_tok(const char*& p, const char* e) {
+  while (p < e && is_space(*p)) ++p;
+  Tok t;
+  t.b = p;
+  while (p < e && !is_space(*p)) ++p;
+  t.e = p;
+  return t;
 }
 
 // a whole token as an int: optional sign, decimal digits, in range
-bool to_int(const std::string& s, int* v) {
-  if (s.empty()) return false;
-  errno = 0;
-  char* end = nullptr;
-  const long x = std::strtol(s.c_str(), &end, 10);
-  if (errno || *end || x < INT_MIN || x > INT_MAX || !std::isdigit((unsigned char)s.back())) return false;
+bool to_int(const Tok& t, int* v) {
+  const char* p = t.b;
+  bool neg = false;
+  if (p < t.e && (*p == '+' || *p == '-')) neg = *p++ == '-';
+  if (p == t.e) return false;
+  long long x = 0;
+  for (; p < t.e; ++p) {
+    if (*p < '0' || *p > '9') return false;
+    x = x * 10 + (*p - '0');
+    if (x > (long long)INT_MAX + 1) return false;
+  }
+  if (neg) x = -x;
+  if (x < INT_MIN || x > INT_MAX) return false;
   *v = (int)x;
   return true;
 }
@@ -88,9 +99,9 @@ void read_config_file(const char* path, bool sorted, ConfigFile& cf, int64_t* er
   int hdr[5];
   {
     if (lines.size() < 2) bad_line(2, err_line, "expected 5 integers: nbf nelec ksym ncfg nopen");
-    const auto t = tokens(lines[1].first, lines[1].second);
-    bool good = t.size() >= 5;
-    for (int i = 0; good && i < 5; ++i) good = to_int(t[i], &hdr[i]);
+    const char* p = lines[1].first;
+    bool good = true;
+    for (int i = 0; good && i < 5; ++i) good = to_int(next_tok(p, lines[1].second), &hdr[i]);
     if (!good) bad_line(2, err_line, "expected 5 integers: nbf nelec ksym ncfg nopen");
   }
   cf.nbf = hdr[0];
@@ -101,30 +112,33 @@ void read_config_file(const char* path, bool sorted, ConfigFile& cf, int64_t* er
 
   const int nbf = cf.nbf;
   int max_open = 0;
-  cf.occ.clear();
-  cf.jsym.clear();
   const int64_t nline = (int64_t)lines.size();
+  cf.occ.assign((size_t)std::max<int64_t>(nline - 2, 0) * (size_t)std::max(nbf, 0), 0);
+  cf.jsym.clear();
+  cf.jsym.reserve((size_t)std::max<int64_t>(nline - 2, 0));
   for (int64_t n = 2; n < nline; ++n) {
     const int64_t ln = n + 1;
-    const auto t = tokens(lines[n].first, lines[n].second);
-    if (t.size() < 2) bad_line(ln, err_line, "expected 'jsym occupation'");
+    const char* p = lines[n].first;
+    const Tok ts = next_tok(p, lines[n].second), to = next_tok(p, lines[n].second);
+    if (!to.size()) bad_line(ln, err_line, "expected 'jsym occupation'");
     int sym;
-    if (!to_int(t[0], &sym)) bad_line(ln, err_line, "bad symmetry '" + t[0] + "'");
-    const std::string& s = t[1];
-    if ((int64_t)s.size() != nbf)
-      bad_line(ln, err_line, "configuration length " + std::to_string(s.size()) + " != nbf " + std::to_string(nbf));
+    if (!to_int(ts, &sym)) bad_line(ln, err_line, "bad symmetry '" + ts.str() + "'");
+    if ((int64_t)to.size() != nbf)
+      bad_line(ln, err_line, "configuration length " + std::to_string(to.size()) + " != nbf " + std::to_string(nbf));
+    uint8_t* row = cf.occ.data() + (size_t)(n - 2) * nbf;
     int nop = 0, nel = 0;
-    for (char ch : s) {
+    for (int o = 0; o < nbf; ++o) {
+      const char ch = to.b[o];
       if (ch < '0' || ch > '2') bad_line(ln, err_line, "occupations must be 0, 1 or 2");
       nop += ch == '1';
       nel += ch - '0';
+      row[o] = (uint8_t)(ch - '0');
     }
     if (nop > nopen_header)
       bad_line(ln, err_line, std::to_string(nop) + " open shells, the header allows " + std::to_string(nopen_header));
     if (nel != cf.nelec)
       bad_line(ln, err_line, "electron count " + std::to_string(nel) + " != " + std::to_string(cf.nelec));
     max_open = std::max(max_open, nop);
-    for (char ch : s) cf.occ.push_back((uint8_t)(ch - '0'));
     cf.jsym.push_back(sym);
   }
   // the reference adjusts the open-shell maximum down to what it found

@@ -202,11 +202,11 @@ def mrcisd_configs(n_internal: int, n_external: int, nelec: int, max_external: i
 
 def write_configs(path: str, cfg: Configs) -> None:
     """Write the reference's configuration file format (PsociConfigs.C:143-245)."""
+    digits = (np.asarray(cfg.occ, np.uint8) + ord("0")).astype(np.uint8)
     with open(path, "w") as f:
         f.write(cfg.title + "\n")
         f.write(f"{cfg.nbf} {cfg.nelec} {cfg.ksym} {cfg.ncfg} {cfg.nopen_max}\n")
-        for js, row in zip(cfg.jsym, cfg.occ):
-            f.write(f"{int(js)} {''.join(str(int(v)) for v in row)}\n")
+        f.writelines(f"{int(js)} {row.tobytes().decode()}\n" for js, row in zip(cfg.jsym, digits))
 
 
 class ConfigFileError(ValueError):

@@ -57,3 +57,19 @@ def test_set_configs_file_file_order_and_sorted(tmp_path):
     assert e.value.code == api.ERR_ARG and "line 4" in str(e.value)
     for H in (A, B, S):
         H.close()
+
+
+def test_from_files_equals_array_input(fci6, tmp_path):
+    """Hamiltonian.from_files (psoci_set_configs_file + psoci_set_integrals_sifs,
+    the reference's input path) builds the same stored H as array input."""
+    cfg, ints, _ = fci6
+    cpath, spath = str(tmp_path / "configs"), str(tmp_path / "moints")
+    inputs.write_configs(cpath, cfg)
+    inputs.write_sifs(spath, ints)
+    A = api.Hamiltonian(cfg, ints)
+    B = api.Hamiltonian.from_files(cpath, spath)
+    assert B.n == A.n and A.build() == B.build()
+    for a, b in zip(A.get_h(), B.get_h()):
+        np.testing.assert_array_equal(a, b)
+    A.close()
+    B.close()
