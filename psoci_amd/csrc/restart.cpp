@@ -92,9 +92,12 @@ VecHeader read_vec_header(File& in) {
   in.get(&h.nroots, 1, "the header");
   in.get(&h.width, 1, "the header");
   in.get(&h.prec, 1, "the header");
-  PSOCI_CHECK(h.nbasis > 0 && h.nroots >= 0, PSOCI_ERR_ARG, in.path + ": bad vector header");
+  // bounds first, so that the size below cannot overflow: nroots * nbasis <= 2^40
+  PSOCI_CHECK(h.nbasis > 0 && h.nbasis <= (int64_t(1) << 40) && h.nroots >= 0 &&
+                  h.nroots <= (int64_t(1) << 40) / h.nbasis,
+              PSOCI_ERR_ARG, in.path + ": bad vector header");
   const int64_t need = 24 + h.nroots * (8 + 8 * h.nbasis);
-  PSOCI_CHECK(h.nroots <= (int64_t(1) << 40) / h.nbasis && in.size() >= need, PSOCI_ERR_ARG,
+  PSOCI_CHECK(in.size() >= need, PSOCI_ERR_ARG,
               in.path + ": file shorter than its header says (" + std::to_string(h.nroots) + " roots of " +
                   std::to_string(h.nbasis) + ")");
   return h;
@@ -221,7 +224,9 @@ void load_h(Ctx& c, const char* path) {
               in.path + ": written for another configuration list");
   PSOCI_CHECK(h.ints_hash == c.ints_hash, PSOCI_ERR_ARG, in.path + ": written for other integrals");
   const int64_t nnz = h.nnz;
-  PSOCI_CHECK(nnz >= n && nnz <= n * (n + 1) / 2 && in.size() == (int64_t)sizeof(HHeader) + 8 * (n + 1) + 12 * nnz + 8 * n + 8,
+  // nnz bounded first (n < 2^31, nnz <= 2^40), so the file size below cannot overflow
+  PSOCI_CHECK(nnz >= n && nnz <= n * (n + 1) / 2 && nnz <= (int64_t(1) << 40) &&
+                  in.size() == (int64_t)sizeof(HHeader) + 8 * (n + 1) + 12 * nnz + 8 * n + 8,
               PSOCI_ERR_ARG, in.path + ": file size does not match its header");
   std::vector<int64_t> rp(n + 1);
   std::vector<int32_t> col(nnz);

@@ -91,14 +91,16 @@ struct Reader {
     PSOCI_CHECK(std::fread(&nb2, 4, 1, f) == 1 && nb2 == nb, PSOCI_ERR_ARG, path + ": record framing mismatch");
     pos = 0;
   }
+  // (pos <= rec.size() always; the checks divide rather than multiply, so a
+  // count read from a damaged file cannot wrap them)
   template <class T>
   void get(T* p, size_t n) {
-    PSOCI_CHECK(pos + n * sizeof(T) <= rec.size(), PSOCI_ERR_ARG, path + ": record shorter than its fields");
+    PSOCI_CHECK(n <= (rec.size() - pos) / sizeof(T), PSOCI_ERR_ARG, path + ": record shorter than its fields");
     if (n) std::memcpy(p, rec.data() + pos, n * sizeof(T));
     pos += n * sizeof(T);
   }
   void skip(size_t nbytes) {
-    PSOCI_CHECK(pos + nbytes <= rec.size(), PSOCI_ERR_ARG, path + ": record shorter than its fields");
+    PSOCI_CHECK(nbytes <= rec.size() - pos, PSOCI_ERR_ARG, path + ": record shorter than its fields");
     pos += nbytes;
   }
 };

@@ -77,6 +77,12 @@ def test_errors(tmp_path):
         api.civec_info(str(tmp_path / "missing.dat"))
     with pytest.raises(api.PsociError, match="bad vector arguments"):
         api.civec_read(str(p), 10, 11)  # k > n: no unit-vector completion
+    # header sizes whose product would overflow the size check are refused
+    # before any size arithmetic
+    for nb, nr in ((1 << 62, 3), (5, 1 << 62), (1 << 41, 1), (-4, 2), (4, -2)):
+        q.write_bytes(struct.pack("<qqii", nb, nr, 18, 14) + data[24:])
+        with pytest.raises(api.PsociError, match="bad vector header"):
+            api.civec_info(str(q))
 
 
 def test_restart_filename():
