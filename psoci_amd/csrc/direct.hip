@@ -305,7 +305,9 @@ __device__ __forceinline__ ConfView view_of(const ConfRec& r) {
 // 8C), so rows that share column configurations meet in the same L2 while
 // the XCDs stay evenly loaded.
 __device__ __forceinline__ int64_t row_of_wave(const DirectArgs& a) {
-  const int wave = threadIdx.x / kWave;
+  // uniform per wavefront; saying so keeps the row's configuration record,
+  // its segment bounds and the loop counters in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   int64_t wg = blockIdx.x;
   if (a.chunk > 0) {
     const int64_t q = blockIdx.x / 8, x = blockIdx.x % 8;
