@@ -47,7 +47,7 @@ __global__ void k_detao(int64_t ncfg, const int64_t* __restrict__ detoff, const 
                         const int8_t* __restrict__ nopen, const uint16_t* __restrict__ pat,
                         const int32_t* __restrict__ patoff, uint64_t* __restrict__ detao) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   if (c >= ncfg) return;
   const int s = nopen[c];
   const int nd = 1 << (s > 0 ? s - 1 : 0);
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void k_adj(int64_t ncfg, const uint64_t* __res
   __shared__ uint64_t tdm[kTile], tsm[kTile];
   __shared__ int8_t tno[kTile];
   const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int64_t I0 = ((int64_t)blockIdx.x * (blockDim.x / kWave) + wave) * kIPerWave;
   uint64_t dI[kIPerWave], sI[kIPerWave];
   int nI[kIPerWave];
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void k_records(int64_t ncfg, const int64_t* __
                                                  unsigned long long* __restrict__ nelems) {
   __shared__ int s_base[4][kNClass];
   const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + wave;
   if (I >= ncfg) return;  // uniform per wavefront; wavefront-level synchronisation only
   int* base = s_base[wave];

@@ -42,7 +42,7 @@ __global__ void k_screen(int64_t ncfg, const uint64_t* __restrict__ dm, const ui
                          const int8_t* __restrict__ nopen, int upper, int32_t* __restrict__ cnt,
                          const int64_t* __restrict__ off, int32_t* __restrict__ out) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   if (I >= ncfg) return;  // uniform per wavefront
   const uint64_t dI = dm[I], sI = sm[I];
   const int nI = nopen[I];
@@ -71,7 +71,7 @@ __global__ void k_i32_to_i64(int64_t n, const int32_t* __restrict__ a, int64_t* 
 // pair -> row configuration (wavefront per I)
 __global__ void k_expand_rows(int64_t ncfg, const int64_t* __restrict__ off, int32_t* __restrict__ rowcfg) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   if (I >= ncfg) return;
   for (int64_t t = off[I] + lane; t < off[I + 1]; t += kWave) rowcfg[t] = (int32_t)I;
 }
@@ -261,7 +261,7 @@ __global__ void k_diag_only(int64_t ncfg, const uint64_t* __restrict__ dm, const
                             int frac, const double* __restrict__ one, const double* __restrict__ two,
                             double* __restrict__ diag) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   if (I >= ncfg) return;
   PairInfo P;
   if (lane == 0) pair_setup(P, (int)I, (int)I, dm[I], sm[I], nopen[I], dm[I], sm[I], nopen[I], nbf, one, two);
@@ -297,7 +297,7 @@ __global__ void k_extract_lower(int64_t n, const int64_t* __restrict__ rowptr, c
                                 const int64_t* __restrict__ lowptr, const int32_t* __restrict__ col,
                                 const double* __restrict__ val, int32_t* __restrict__ lcol, double* __restrict__ lval) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t r = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const int64_t r = (int64_t)blockIdx.x * (blockDim.x / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   if (r >= n) return;
   int64_t a = rowptr[r], b = lowptr[r];
   for (int q = lane; q < lowcnt[r]; q += kWave) {

@@ -27,7 +27,7 @@ __global__ __launch_bounds__(256) void k_gemm_tn_mfma(int64_t n, int m, int b, c
                                                       const double* __restrict__ B, int ldb, int64_t rows_per_chunk,
                                                       double* __restrict__ partial) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int tiles_m = (m + 15) / 16, tiles_b = (b + 15) / 16;
   const int tile = blockIdx.x * (blockDim.x / kWave) + wave;
   if (tile >= tiles_m * tiles_b) return;

@@ -225,7 +225,7 @@ __device__ __forceinline__ ConfView view_rec(const ConfRec& r) {
 
 __global__ __launch_bounds__(256) void k_rdm(RdmArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   if (I >= a.ncfg) return;
   const ConfRec ri = a.conf[I];
   const ConfView A = view_rec(ri);

@@ -46,7 +46,7 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, const int64_t* __restri
   constexpr int NS = LR / G;      // nonzero slots per row
   constexpr int C = K / G;        // columns per lane
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t r = ((int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * R + lane / LR;
+  const int64_t r = ((int64_t)blockIdx.x * (blockDim.x / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave)) * R + lane / LR;
   const int lr = lane % LR, slot = lr / G, cc = lr % G;
   T acc[C];
 #pragma unroll
