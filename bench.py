@@ -22,6 +22,7 @@ import time
 
 import numpy as np
 
+T_START = time.time()  # the wall budget counts from process start (torch import included)
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -32,8 +33,14 @@ METRIC = "time to 16 roots at 1e-8 Eh, FP32-sigma vs FP64 on GPU; sigma HBM GB/s
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed mixed solves (default 1 direct, 3 stored)")
-    ap.add_argument("--warmup", type=int, default=None, help="untimed mixed solves (default 0 direct, 1 stored)")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed mixed solves, at most (default 3 direct, 5 stored); capped by --budget")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed mixed solves, at most (default 0 direct, 1 stored); capped by --budget")
+    ap.add_argument("--budget", type=float, default=420.0,
+                    help="wall seconds from process start within which every solve must end; a solve starts "
+                         "only if the longest one so far still fits, with the tail (sigma timings, CPU leg) "
+                         "reserved; at least one timed solve always runs")
     ap.add_argument("--workload", default="mrci10x24")
     ap.add_argument("--roots", type=int, default=16)
     ap.add_argument("--tol", type=float, default=1e-5)
@@ -106,14 +113,22 @@ def cpu_stored(args, cfg, ints, k, ref):
     sp.build_h(threads)
     tb = time.time() - tc
     tc = time.time()
-    oo = sp.davidson(k, rtol=args.tol, maxit=4000, guess=1)
+    # the same starting vectors as the GPU solves (unit vectors on the k lowest
+    # diagonal elements, guess 0)
+    oo = sp.davidson(k, rtol=args.tol, maxit=4000, guess=0)
     td = time.time() - tc
+    diff = float(np.abs(oo["evals"] - ref["evals"]).max())
+    valid = diff <= 1e-8
     return {"value": round(tb + td, 3), "unit": "s", "cores": threads, "kind": "port",
             "sample": (f"whole solve, not sampled: oracle (C++ restatement of the reference CPU path) on "
                        f"{args.workload}: H build on {threads} OpenMP threads {tb:.2f} s + reference-schedule "
                        f"single-vector Davidson (1 thread) for {k} roots to residual {args.tol:g} "
-                       f"({oo['iterations']} iterations) {td:.2f} s; host {cpu_model()}"),
-            "max_root_diff_vs_gpu_fp64": float(np.abs(oo["evals"] - ref["evals"]).max())}
+                       f"({oo['iterations']} iterations) {td:.2f} s; host {cpu_model()}"
+                       + ("" if valid else
+                          "; INVALID as a baseline: the single-vector schedule converged to other roots than the "
+                          "GPU solve (it corrects one root per iteration from unit guesses and can miss a copy of "
+                          "a degenerate eigenvalue)")),
+            "max_root_diff_vs_gpu_fp64": diff, "valid": valid}
 
 
 def cpu_direct(args, cfg, ints, k, sigma_calls, budget_s=15.0):
@@ -165,9 +180,11 @@ def main():
     cfg, ints, _ = inputs.workload(args.workload)
     direct = inputs.WORKLOADS[args.workload]["kind"] != "fci"
     if args.steps is None:
-        args.steps = 1 if direct else 3
+        args.steps = 3 if direct else 5
     if args.warmup is None:
         args.warmup = 0 if direct else 1
+    args.steps = max(1, args.steps)
+    args.warmup = max(0, args.warmup)
     mode = api.DIRECT if direct else api.STORED
     k = args.roots
     guard = args.guard_roots if args.guard_roots is not None else (8 if direct else 0)
@@ -214,19 +231,41 @@ def main():
     fp64_s = ref["stats"]["seconds"]
     print(f"[bench] fp64 solve {fp64_s:.2f} s, {ref['stats']['iterations']} iterations", file=sys.stderr, flush=True)
 
-    for _ in range(args.warmup):
+    # wall budget: what follows the solves (sigma timings, CPU leg) is
+    # reserved; a solve starts only if the longest solve so far still fits.
+    # Warm-up solves leave room for min(steps, 2) timed ones.
+    tail_s = 60.0 if direct else 30.0
+    if args.no_cpu:
+        tail_s = 20.0 if direct else 5.0
+    durations = []  # every mixed solve run so far; before the first, the FP64 solve's time is the estimate
+
+    def fits(n_more):
+        est = max(durations) if durations else fp64_s
+        return time.time() - T_START + n_more * est + tail_s <= args.budget
+
+    warm_done = 0
+    while warm_done < args.warmup and fits(1 + min(args.steps, 2)):
+        tw = time.time()
         H.davidson(k, prec=api.MIXED, mode=mode, tol=args.tol, switch_tol=1e-4, want_vectors=False,
                    extra_roots=guard)
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        durations.append(time.time() - tw)
+        warm_done += 1
+        print(f"[bench] warm-up solve {warm_done}: {durations[-1]:.2f} s", file=sys.stderr, flush=True)
     times, dmax, last = [], 0.0, None
     t_all = time.time()
     for s in range(args.steps):
+        if s > 0 and not fits(1):
+            print(f"[bench] wall budget {args.budget:.0f} s reached after {s} timed solves", file=sys.stderr,
+                  flush=True)
+            break
         torch.cuda.synchronize()
         ts = time.time()
         out = H.davidson(k, prec=api.MIXED, mode=mode, tol=args.tol, switch_tol=1e-4, want_vectors=False,
                          extra_roots=guard)
         torch.cuda.synchronize()
         times.append(time.time() - ts)
+        durations.append(times[-1])
         print(f"[bench] mixed step {s}: {times[-1]:.2f} s", file=sys.stderr, flush=True)
         if not out["converged"]:
             raise SystemExit("mixed-precision Davidson did not converge")
@@ -277,8 +316,11 @@ def main():
         "value": round(mixed_s, 6),
         "unit": "s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": len(times),
+        "warmup": warm_done,
+        "steps_requested": args.steps,
+        "warmup_requested": args.warmup,
+        "wall_budget_s": args.budget,
         "ms_per_step": round(mixed_s * 1e3, 3),
         "higher_is_better": False,
         "scaling": "strong",

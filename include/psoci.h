@@ -178,7 +178,10 @@ int psoci_h_load(psoci_ctx* ctx, const char* path, int64_t* nnz_lower);
  * ...DirectFetch path (:7549-7823; mode DIRECT, elements regenerated on the
  * fly).  X and Y are [n][k] row-major, FP64 for prec PSOCI_FP64 or FP32 for
  * PSOCI_FP32 (FP32 H values, FP32 accumulation).  on_device != 0: X and Y
- * are device pointers (no copies, launched on the context stream). */
+ * are device pointers (no copies, launched on the context stream).  The
+ * call returns once Y is written.  The context stream is non-blocking: it is
+ * not ordered after the caller's streams, so the caller must finish (or
+ * synchronise) the work that writes X before the call. */
 int psoci_sigma(psoci_ctx* ctx, int mode, int prec, int k, const void* X, void* Y, int on_device);
 
 /* Work counters of the direct sigma (built on first use): connected

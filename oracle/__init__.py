@@ -178,9 +178,10 @@ class OracleSpace:
         lib().orc_sigma(self._h, k, _p(X), _p(Y))
         return Y
 
-    def davidson(self, k: int, rtol: float = 1e-6, maxit: int = 400, guess: int = 0):
-        """guess 0: the reference's unit guesses on the first k SEFs;
-        guess 1: unit guesses on the k lowest diagonal elements."""
+    def davidson(self, k: int, rtol: float = 1e-6, maxit: int = 400, guess: int = 1):
+        """Guess codes as the library's psoci_dav_params.guess: 1 (default)
+        the reference's unit guesses on the first k SEFs; 0 unit guesses on
+        the k lowest diagonal elements."""
         ev = np.zeros(k)
         vecs = np.zeros((self.n, k))
         rn = np.zeros(k)

@@ -1291,11 +1291,11 @@ int orc_davidson(void* h, int k, double rtol, int maxit, int guess, double* eval
     return true;
   };
   int nsigma = 0;
-  // guess 0: unit vectors on the first k SEFs (the reference's guess);
-  // guess 1: on the k lowest diagonal elements (same roots as the GPU runs)
+  // guess codes as psoci_dav_params.guess: 1 unit vectors on the first k
+  // SEFs (the reference's guess); 0 on the k lowest diagonal elements
   std::vector<long> start(k);
   for (int r = 0; r < k; ++r) start[r] = r;
-  if (guess == 1) {
+  if (guess == 0) {
     std::vector<long> ord(n);
     for (long i = 0; i < n; ++i) ord[i] = i;
     std::partial_sort(ord.begin(), ord.begin() + k, ord.end(),

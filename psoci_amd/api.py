@@ -279,7 +279,11 @@ class Hamiltonian:
         return Y[:, 0] if squeeze else Y
 
     def sigma_device(self, X, Y, mode: int = STORED, prec: int = FP64):
-        """X, Y: torch tensors on the GPU, [n][k] contiguous."""
+        """X, Y: torch tensors on the GPU, [n][k] contiguous.  The library's
+        stream is not ordered after torch's, so torch's pending work (what
+        wrote X) is finished first; call sync() before reading Y."""
+        import torch
+        torch.cuda.current_stream().synchronize()
         k = 1 if X.dim() == 1 else X.shape[1]
         self._check(self.L.psoci_sigma(self._h, mode, prec, k, ctypes.c_void_p(X.data_ptr()),
                                        ctypes.c_void_p(Y.data_ptr()), 1))
@@ -324,7 +328,11 @@ class Hamiltonian:
         return out
 
     def project(self, A, B):
-        """A^T B over the rows of two GPU tensors (FP64 MFMA)."""
+        """A^T B over the rows of two GPU tensors (FP64 MFMA); torch's
+        pending work on them is finished first (the library's stream is not
+        ordered after torch's)."""
+        import torch
+        torch.cuda.current_stream().synchronize()
         n, m = A.shape
         b = B.shape[1]
         out = np.zeros((m, b))
@@ -349,6 +357,10 @@ class Hamiltonian:
         g, r, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         self._check(self.L.psoci_stored_shape(self._h, prec, ctypes.byref(g), ctypes.byref(r), ctypes.byref(ms)))
         return {"lanes_per_nz": g.value, "rows_per_wave": r.value, "tune_ms": ms.value}
+
+    def sync(self):
+        """Wait for the work queued on the library's stream (psoci_sync)."""
+        self._check(self.L.psoci_sync(self._h))
 
     def stream_handle(self) -> int:
         """The hipStream_t the library launches on (as an integer)."""

@@ -256,6 +256,12 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
       collapse(kc);
       // the retained Ritz values and vectors are unchanged: w[0..k) still valid
     }
+    // a space of fewer than 2k SEFs (mmax = n < kc + k) leaves fewer free
+    // columns than selected roots after the collapse: keep the corrections of
+    // the lowest roots (sel is ascending); none fit when the basis spans the
+    // whole space, whose Ritz pairs are then exact
+    if (m + nsel > mmax) nsel = mmax - m;
+    if (nsel <= 0) break;
     L.precondition(n, k, c.d_diag.p, w.data(), 1.0e-3, R.p, k);
     // new block = selected preconditioned residuals, placed at V[:, m:m+nsel)
     std::vector<double> Ssel((size_t)k * nsel, 0.0);

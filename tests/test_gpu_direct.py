@@ -45,8 +45,10 @@ def fci10_gpu(fci10):
     H.close()
 
 
-@pytest.mark.parametrize("k", [1, 4, 16, 3])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 8, 16, 24, 32])
 def test_direct_matches_stored_fci10(fci10_gpu, k):
+    """Every kernel width the library instantiates (1, 2, 4, 8, 16, 24, 32;
+    3 runs padded to 4), FP64 and FP32."""
     H = fci10_gpu
     X = np.random.default_rng(k).standard_normal((H.n, k))
     ys = H.sigma(X, api.STORED, api.FP64)
@@ -64,12 +66,15 @@ def test_direct_davidson_matches_stored_fci10(fci10_gpu):
     np.testing.assert_allclose(b["evals"], a["evals"], atol=1e-9, rtol=0)
 
 
-def test_direct_rows_match_oracle_mrci8x24():
+@pytest.mark.parametrize("k", [8, 24])
+def test_direct_rows_match_oracle_mrci8x24(k):
+    """k = 24 is the width of the FP32 phase of the benchmark solve (16 roots
+    + 8 guard roots)."""
     cfg, ints, _ = inputs.workload("mrci8x24")
     H = api.Hamiltonian(cfg, ints)
     sp = oracle.OracleSpace(cfg, ints)
     rng = np.random.default_rng(3)
-    X = rng.standard_normal((H.n, 8))
+    X = rng.standard_normal((H.n, k))
     Y = H.sigma(X, api.DIRECT, api.FP64)
     Y32 = H.sigma(X.astype(np.float32), api.DIRECT, api.FP32)
     nopen = (cfg.occ == 1).sum(1)
@@ -103,6 +108,21 @@ def test_direct_mixed_davidson_mrci8x24():
     V = a["evecs"]
     R = H.sigma(V, api.DIRECT, api.FP64) - V * a["evals"]
     assert np.linalg.norm(R, axis=0).max() < 1e-4
+    H.close()
+
+
+def test_direct_mixed_guard_roots_mrci8x24():
+    """Guard roots in the FP32 phase (the benchmark's 16 + 8 block shape on
+    the 8-root workload: 8 + 8): every wanted root within 1e-8 Eh of the
+    all-FP64 solve."""
+    cfg, ints, k = inputs.workload("mrci8x24")
+    H = api.Hamiltonian(cfg, ints)
+    a = H.davidson(k, prec=api.FP64, mode=api.DIRECT, tol=1e-5, want_vectors=False)
+    b = H.davidson(k, prec=api.MIXED, mode=api.DIRECT, tol=1e-5, switch_tol=1e-4, extra_roots=8,
+                   want_vectors=False)
+    assert a["converged"] and b["converged"]
+    assert b["stats"]["switch_iter"] > 0
+    np.testing.assert_allclose(b["evals"], a["evals"], atol=1e-8, rtol=0)
     H.close()
 
 

@@ -137,3 +137,36 @@ def test_empty_configuration_list():
     with pytest.raises(api.PsociError) as e:
         api.Hamiltonian(cfg, inputs.make_integrals(4, 1))
     assert e.value.code == api.ERR_ARG
+
+
+@pytest.mark.parametrize("nroots,extra", [(12, 0), (8, 6), (16, 0)])
+def test_block_wider_than_half_the_space(nroots, extra):
+    """A working block k with k < n < 2k (here n = 22): the subspace holds
+    at most n columns, so after a collapse fewer columns are free than roots
+    are selected; the corrections of the lowest roots are kept and the
+    eigenpairs are exact: every returned value is an eigenvalue of the
+    oracle's dense H (LAPACK; distinct eigenvalues matched one to one) and
+    every returned vector satisfies the eigen-equation.  (H conserves |Ms|,
+    so a solve started on unit vectors may stay out of a sector: the values
+    are checked as a sub-multiset of the spectrum, not as its lowest k.)"""
+    cfg = inputs.fci_configs(4, 4)
+    ints = inputs.make_integrals(4, 7)
+    H = api.Hamiltonian(cfg, ints)
+    assert nroots + extra < H.n < 2 * (nroots + extra)
+    sp = oracle.OracleSpace(cfg, ints)
+    sp.build_h()
+    w = np.linalg.eigvalsh(sp.dense_h())
+    H.build()
+    for mode, prec in ((api.STORED, api.FP64), (api.DIRECT, api.FP64), (api.DIRECT, api.MIXED)):
+        out = H.davidson(nroots, prec=prec, mode=mode, tol=1e-8, extra_roots=extra, guess=1)
+        assert out["converged"], (mode, prec)
+        free = list(w)
+        for e in out["evals"]:
+            j = int(np.argmin(np.abs(np.array(free) - e)))
+            assert abs(free[j] - e) < 1e-9, (mode, prec, e)
+            free.pop(j)
+        V = out["evecs"]
+        np.testing.assert_allclose(V.T @ V, np.eye(nroots), atol=1e-9)
+        R = sp.dense_h() @ V - V * out["evals"]
+        assert np.linalg.norm(R, axis=0).max() < 1e-6, (mode, prec)
+    H.close()

@@ -72,7 +72,7 @@ def test_stored_h_matches_reference_pattern(which, gpu_fci6, gpu_fci10, oracle_f
     assert np.array_equal(H.diag(), sp.diag)
 
 
-@pytest.mark.parametrize("k", [1, 4, 16, 5])
+@pytest.mark.parametrize("k", [1, 4, 16, 5, 24, 32])
 def test_stored_sigma_fp64_and_fp32(k, gpu_fci10, oracle_fci10):
     H, sp = gpu_fci10, oracle_fci10
     rng = np.random.default_rng(k)

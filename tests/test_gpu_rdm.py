@@ -39,6 +39,29 @@ def test_rdm_homogeneity_and_oracle_density(fci6):
     H.close()
 
 
+def test_rdm_matches_oracle_density_fci10(fci10, oracle_fci10):
+    """The adjoint's one-electron part for converged roots of the
+    31,650-SEF space equals the oracle's CIDEN 1-RDM (the reference's
+    constructDensityMatrix): grad_one[pp] = D_pp, grad_one[pq] = 2 D_pq."""
+    cfg, ints, _ = fci10
+    k = 4
+    H = api.Hamiltonian(cfg, ints)
+    H.build()
+    out = H.davidson(k, prec=api.FP64, mode=api.STORED, tol=1e-9)
+    assert out["converged"]
+    sp = oracle_fci10
+    nbf = cfg.nbf
+    scale = np.array([1.0 if p == q else 2.0 for p in range(nbf) for q in range(p + 1)])
+    for r in range(k):
+        w = np.zeros(k)
+        w[r] = 1.0
+        g1, g2, _ = H.rdm(out["evecs"], w)
+        D = sp.rdm1(out["evecs"][:, r])
+        np.testing.assert_allclose(g1, D * scale, atol=1e-11, rtol=0)
+        assert abs(g1 @ ints.valone + g2 @ ints.valtwo - out["evals"][r]) < 1e-9
+    H.close()
+
+
 def test_rdm_finite_differences_fci10(fci10):
     cfg, ints, _ = fci10
     # roots 3 and 4 of this space are an exactly degenerate pair (spin
