@@ -184,6 +184,21 @@ int psoci_h_load(psoci_ctx* ctx, const char* path, int64_t* nnz_lower);
  * synchronise) the work that writes X before the call. */
 int psoci_sigma(psoci_ctx* ctx, int mode, int prec, int k, const void* X, void* Y, int on_device);
 
+/* psoci_sigma on device pointers (X, Y [n][k] row-major) that allocates
+ * nothing: no reference counterpart; for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops, harnesses
+ * that forbid allocation).  The zero-padded column staging of a block that
+ * is not a kernel width (1, 2, 4, 8, 16, 32; 24 for the direct sigma), or
+ * not 16-byte aligned, goes through a fixed 64 MiB device array per GPU
+ * (held until the process ends) instead of context memory; a block needing
+ * more (2 n K sizeof(element) bytes at launch width K) is refused with
+ * PSOCI_ERR_ARG before anything runs.  The operator must exist already:
+ * psoci_build_h (STORED) or the pair records of the direct sigma (built by
+ * psoci_direct_info or a first psoci_sigma; DIRECT), else PSOCI_ERR_STATE.
+ * Calls that overlap on one GPU share the array and race.  Same ordering
+ * and results as psoci_sigma. */
+int psoci_sigma_noalloc(psoci_ctx* ctx, int mode, int prec, int k, const void* X, void* Y);
+
 /* Work counters of the direct sigma (built on first use): connected
  * configuration pairs visited per sigma (both orders, J = I included) and
  * structural SEF elements generated per sigma (sum of nsef_I * nsef_J over

@@ -52,6 +52,12 @@ int psoci_ctx_create(int device, psoci_ctx** out) {
     delete ctx;
     return PSOCI_ERR_HIP;
   }
+  // the allocation-free sigma's scratch address on this device (once per device)
+  const int rc = psoci::noalloc_init(device);
+  if (rc != PSOCI_OK) {
+    delete ctx;
+    return rc;
+  }
   *out = ctx;
   return PSOCI_OK;
 }
@@ -270,6 +276,31 @@ int psoci_sigma(psoci_ctx* ctx, int mode, int prec, int k, const void* X, void* 
     if (mode == PSOCI_SIGMA_STORED) psoci::sigma_stored(c, prec, k, dx, k, dy, k);
     else psoci::sigma_direct(c, prec, k, dx, k, dy, k);
     if (!on_device) by.download(static_cast<char*>(Y), (size_t)n * k * es, c.stream);
+    c.sync();
+    return PSOCI_OK;
+  });
+}
+
+int psoci_sigma_noalloc(psoci_ctx* ctx, int mode, int prec, int k, const void* X, void* Y) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
+    PSOCI_CHECK(k >= 1 && X && Y, PSOCI_ERR_ARG, "bad sigma arguments");
+    PSOCI_CHECK(prec == PSOCI_FP64 || prec == PSOCI_FP32, PSOCI_ERR_ARG, "sigma precision must be FP64 or FP32");
+    PSOCI_CHECK(mode == PSOCI_SIGMA_STORED || mode == PSOCI_SIGMA_DIRECT, PSOCI_ERR_ARG, "bad sigma mode");
+    // building the operator allocates: it must exist already
+    if (mode == PSOCI_SIGMA_STORED) PSOCI_CHECK(c.have_h, PSOCI_ERR_STATE, "stored sigma needs psoci_build_h first");
+    if (mode == PSOCI_SIGMA_DIRECT)
+      PSOCI_CHECK(c.have_pairs, PSOCI_ERR_STATE,
+                  "the direct sigma's pair records are not built (psoci_direct_info or psoci_sigma builds them)");
+    psoci::Staging st;
+    psoci::noalloc_scratch(c.device, &st.p, &st.bytes);
+    const size_t need = psoci::sigma_staging_bytes(c, mode, prec, k, X, k, Y, k);
+    PSOCI_CHECK(need <= st.bytes, PSOCI_ERR_ARG,
+                "this block needs " + std::to_string(need) + " bytes of staging and the scratch holds " +
+                    std::to_string(st.bytes) + ": pass a 16-byte aligned block of a kernel width (1, 2, 4, 8, 16, 32; "
+                    "24 for the direct sigma) or fewer columns");
+    if (mode == PSOCI_SIGMA_STORED) psoci::sigma_stored(c, prec, k, X, k, Y, k, st);
+    else psoci::sigma_direct(c, prec, k, X, k, Y, k, st);
     c.sync();
     return PSOCI_OK;
   });

@@ -5,6 +5,13 @@
 
 namespace psoci {
 
+// Caller-provided device memory for the padded column staging of a sigma
+// call (p null: the context's own buffer, grown on demand).
+struct Staging {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -88,8 +95,14 @@ void build_h(Ctx& c);                   // stored Hamiltonian
 void tune_stored(Ctx& c);               // stored sigma shape autotuner (after build_h)
 void compute_diag(Ctx& c);              // diagonal only
 void block_on_device(Ctx& c, int64_t I, int64_t J, double* host_out, int* status);
-void sigma_stored(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy);
-void sigma_direct(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy);
+void sigma_stored(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy, const Staging& st = Staging{});
+void sigma_direct(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy, const Staging& st = Staging{});
+// staging bytes a sigma call of k columns at strides ldx, ldy needs (0: none)
+size_t sigma_staging_bytes(const Ctx& c, int mode, int prec, int k, const void* X, int ldx, const void* Y, int ldy);
+// the allocation-free scratch array of the current device (noalloc.hip):
+// set up once per device by psoci_ctx_create, then its address and size
+int noalloc_init(int device);
+void noalloc_scratch(int device, void** p, size_t* bytes);
 void get_h_lower(Ctx& c, int64_t* rowptr, int32_t* col, double* val, double* diag);
 // installs a stored Hamiltonian given as the full symmetric CSR (lower part
 // first in each row, lowcnt entries) and its diagonal (restart.cpp -> hbuild.hip)

@@ -496,7 +496,7 @@ void launch_direct(Ctx& c, const T* X, int ldx, T* Y, int ldy, int64_t row0 = 0,
 }
 
 template <typename T>
-void direct_any_k(Ctx& c, int k, const T* X, int ldx, T* Y, int ldy) {
+void direct_any_k(Ctx& c, int k, const T* X, int ldx, T* Y, int ldy, const Staging& st) {
   pad::blocks<T>(c, c.sp.nsef_total, k, 32, X, ldx, Y, ldy, [&](int K, const T* x, int lx, T* y, int ly) {
     switch (K) {
       case 32: launch_direct<T, 32>(c, x, lx, y, ly); break;
@@ -507,7 +507,7 @@ void direct_any_k(Ctx& c, int k, const T* X, int ldx, T* Y, int ldy) {
       case 2: launch_direct<T, 2>(c, x, lx, y, ly); break;
       default: launch_direct<T, 1>(c, x, lx, y, ly); break;
     }
-  }, true);
+  }, true, st);
 }
 
 // Schedule autotuner: the row order (natural, or by occupation string from
@@ -672,12 +672,15 @@ void build_pairs(Ctx& c) {
   tune_direct(c);
 }
 
-void sigma_direct(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy) {
-  if (!c.have_pairs) build_pairs(c);
+void sigma_direct(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy, const Staging& st) {
+  if (!c.have_pairs) {
+    PSOCI_CHECK(!st.p, PSOCI_ERR_STATE, "the direct sigma's pair records are not built");
+    build_pairs(c);
+  }
   if (prec == PSOCI_FP32)
-    direct_any_k<float>(c, k, static_cast<const float*>(X), ldx, static_cast<float*>(Y), ldy);
+    direct_any_k<float>(c, k, static_cast<const float*>(X), ldx, static_cast<float*>(Y), ldy, st);
   else
-    direct_any_k<double>(c, k, static_cast<const double*>(X), ldx, static_cast<double*>(Y), ldy);
+    direct_any_k<double>(c, k, static_cast<const double*>(X), ldx, static_cast<double*>(Y), ldy, st);
 }
 
 }  // namespace psoci

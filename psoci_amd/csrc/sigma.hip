@@ -115,7 +115,7 @@ void launch_spmm(Ctx& c, const T* val, const T* X, int ldx, T* Y, int ldy) {
 }
 
 template <typename T>
-void spmm_any_k(Ctx& c, const T* val, int k, const T* X, int ldx, T* Y, int ldy) {
+void spmm_any_k(Ctx& c, const T* val, int k, const T* X, int ldx, T* Y, int ldy, const Staging& st) {
   pad::blocks<T>(c, c.sp.nsef_total, k, 32, X, ldx, Y, ldy, [&](int K, const T* x, int lx, T* y, int ly) {
     switch (K) {
       case 32: launch_spmm<T, 32>(c, val, x, lx, y, ly); break;
@@ -125,7 +125,7 @@ void spmm_any_k(Ctx& c, const T* val, int k, const T* X, int ldx, T* Y, int ldy)
       case 2: launch_spmm<T, 2>(c, val, x, lx, y, ly); break;
       default: launch_spmm<T, 1>(c, val, x, lx, y, ly); break;
     }
-  });
+  }, false, st);
 }
 
 }  // namespace
@@ -191,12 +191,23 @@ void tune_stored(Ctx& c) {
   (void)hipEventDestroy(e1);
 }
 
-void sigma_stored(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy) {
+void sigma_stored(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy, const Staging& st) {
   PSOCI_CHECK(c.have_h, PSOCI_ERR_STATE, "stored sigma needs psoci_build_h first");
   if (prec == PSOCI_FP32)
-    spmm_any_k<float>(c, c.d_val32.p, k, static_cast<const float*>(X), ldx, static_cast<float*>(Y), ldy);
+    spmm_any_k<float>(c, c.d_val32.p, k, static_cast<const float*>(X), ldx, static_cast<float*>(Y), ldy, st);
   else
-    spmm_any_k<double>(c, c.d_val.p, k, static_cast<const double*>(X), ldx, static_cast<double*>(Y), ldy);
+    spmm_any_k<double>(c, c.d_val.p, k, static_cast<const double*>(X), ldx, static_cast<double*>(Y), ldy, st);
+}
+
+size_t sigma_staging_bytes(const Ctx& c, int mode, int prec, int k, const void* X, int ldx, const void* Y, int ldy) {
+  // the stored sigma has widths 1..32 (powers of two), the direct one also 24
+  const bool w24 = mode == PSOCI_SIGMA_DIRECT;
+  const int64_t n = c.sp.nsef_total;
+  if (prec == PSOCI_FP32)
+    return pad::staging_bytes<float>(n, k, 32, static_cast<const float*>(X), ldx, static_cast<const float*>(Y), ldy,
+                                     w24);
+  return pad::staging_bytes<double>(n, k, 32, static_cast<const double*>(X), ldx, static_cast<const double*>(Y), ldy,
+                                    w24);
 }
 
 }  // namespace psoci
