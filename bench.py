@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--guard-roots", type=int, default=None,
                     help="guard roots in the FP32 phase of the mixed solve (default 8 direct, 0 stored)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--precond", type=int, default=0,
+                    help="Davidson preconditioner: 0 the reference's diagonal one, 1 the same with Olsen's "
+                         "correction")
     ap.add_argument("--sigma-reps", type=int, default=50)
     ap.add_argument("--sigma-k", type=int, default=None,
                     help="--sigma-only: block width to time (default: the roots)")
@@ -225,7 +228,7 @@ def main():
         return
 
     # all-FP64 reference solve of the same space
-    ref = H.davidson(k, prec=api.FP64, mode=mode, tol=args.tol, want_vectors=False)
+    ref = H.davidson(k, prec=api.FP64, mode=mode, tol=args.tol, want_vectors=False, precond=args.precond)
     if not ref["converged"]:
         raise SystemExit("FP64 Davidson did not converge")
     fp64_s = ref["stats"]["seconds"]
@@ -247,7 +250,7 @@ def main():
     while warm_done < args.warmup and fits(1 + min(args.steps, 2)):
         tw = time.time()
         H.davidson(k, prec=api.MIXED, mode=mode, tol=args.tol, switch_tol=1e-4, want_vectors=False,
-                   extra_roots=guard)
+                   extra_roots=guard, precond=args.precond)
         torch.cuda.synchronize()
         durations.append(time.time() - tw)
         warm_done += 1
@@ -262,7 +265,7 @@ def main():
         torch.cuda.synchronize()
         ts = time.time()
         out = H.davidson(k, prec=api.MIXED, mode=mode, tol=args.tol, switch_tol=1e-4, want_vectors=False,
-                         extra_roots=guard)
+                         extra_roots=guard, precond=args.precond)
         torch.cuda.synchronize()
         times.append(time.time() - ts)
         durations.append(times[-1])
@@ -336,6 +339,8 @@ def main():
         "root_diffs_eh": [float(x) for x in np.abs(last["evals"] - ref["evals"])],
         "iterations_mixed": last["stats"]["iterations"],
         "guard_roots_fp32_phase": guard,
+        "preconditioner": ("diagonal (the reference's)" if args.precond == 0 else
+                           "diagonal with Olsen's correction"),
         "switch_iter": last["stats"]["switch_iter"],
         "iterations_fp64": ref["stats"]["iterations"],
         "sigma_calls_mixed": last["stats"]["sigma_calls"],

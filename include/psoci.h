@@ -206,6 +206,16 @@ int psoci_sigma_noalloc(psoci_ctx* ctx, int mode, int prec, int k, const void* X
  * (PsociGAhamiltonian.C:2822-2839). */
 int psoci_direct_info(psoci_ctx* ctx, int64_t* npairs, int64_t* nelems);
 
+/* Connected determinant pairs one direct sigma visits: the (row
+ * determinant, column determinant) pairs within two spin-orbital
+ * replacements over every configuration pair the reference computes a block
+ * for (J = I included) -- the work unit of the direct sigma, about 6% of
+ * the structural SEF elements on an MR-CISD space.  Counted by one launch of
+ * the sigma's own enumeration without the arithmetic, on first use, then
+ * cached.  No reference counterpart (a measurement counter beside
+ * psoci_direct_info). */
+int psoci_direct_det_pairs(psoci_ctx* ctx, int64_t* det_pairs);
+
 /* The direct sigma's launch schedule as chosen by the autotuner at pair
  * setup: row order (0 natural, 1 by occupation string), XCD run length in
  * workgroups, and the tuner's best sample time in ms (-1 when fixed by the
@@ -242,6 +252,10 @@ typedef struct {
                          tested on the nroots lowest only); with PSOCI_MIXED they ride in
                          the FP32 phase only and the FP64 phase continues with nroots;
                          0 for the plain solve */
+  int precond;        /* 0: the reference's diagonal preconditioner r_i / (D_ii - lambda)
+                            (PsociGAbasis.C:1860-1920); 1: the same with Olsen's correction
+                            against the Ritz vector x: t = f (r - eps x),
+                            eps = (x . f r) / (x . f x).  Both skip |D_ii - lambda| <= 1e-3 */
 } psoci_dav_params;
 
 typedef struct {
@@ -291,6 +305,16 @@ int psoci_civec_read(const char* path, int64_t n, int k, double* C, int* nread);
  * first m resp. b columns used); out is host memory, m x b row-major. */
 int psoci_project(psoci_ctx* ctx, int64_t n, int m, int b, const double* A, int lda, const double* B, int ldb,
                   double* out);
+/* Basis update Out = alpha Out + A Y over n rows on FP64 MFMA: the Ritz
+ * collapse of PsociGAbasis::expandAndReplaceVectorsNew
+ * (PsociGAbasis.C:483-579) and the residual combinations of
+ * PsociGAsubspace::computeRnormSingle (PsociGAsubspace.C:874-989) as one
+ * product.  A [n][lda] (first m columns) and Out [n][ldo] (first k columns)
+ * are device pointers, row-major, not overlapping; Y is host memory, m x k
+ * row-major; alpha 0 ignores Out's old contents.  Returns once Out is
+ * written. */
+int psoci_combine(psoci_ctx* ctx, int64_t n, int m, int k, const double* A, int lda, const double* Y, double alpha,
+                  double* Out, int ldo);
 
 /* Adjoint of E_k: dE/dtheta = c^T (dH/dtheta) c for every packed integral,
  * accumulated over k vectors C[n][k] (row-major, host) with weights w[k]

@@ -40,7 +40,8 @@ class PsociError(RuntimeError):
 class DavParams(ctypes.Structure):
     _fields_ = [("nroots", ctypes.c_int), ("prec", ctypes.c_int), ("sigma_mode", ctypes.c_int),
                 ("max_iter", ctypes.c_int), ("max_subspace", ctypes.c_int), ("guess", ctypes.c_int),
-                ("tol", ctypes.c_double), ("switch_tol", ctypes.c_double), ("extra_roots", ctypes.c_int)]
+                ("tol", ctypes.c_double), ("switch_tol", ctypes.c_double), ("extra_roots", ctypes.c_int),
+                ("precond", ctypes.c_int)]
 
 
 class DavStats(ctypes.Structure):
@@ -97,7 +98,9 @@ def load(path: str | None = None):
         "psoci_rdm": (I, [V, I, V, V, V, V, V]),
         "psoci_device_ptr": (I, [V, I, ctypes.POINTER(V)]),
         "psoci_project": (I, [V, I64, I, I, V, I, V, I, V]),
+        "psoci_combine": (I, [V, I64, I, I, V, I, V, D, V, I]),
         "psoci_direct_info": (I, [V, V, V]),
+        "psoci_direct_det_pairs": (I, [V, V]),
         "psoci_direct_schedule": (I, [V, V, V, V]),
         "psoci_stored_shape": (I, [V, I, V, V, V]),
         "psoci_sync": (I, [V]),
@@ -292,17 +295,18 @@ class Hamiltonian:
     def davidson(self, nroots: int, prec: int = FP64, mode: int = STORED, tol: float = 1e-6,
                  switch_tol: float = 1e-4, max_iter: int = 200, max_subspace: int = 0, guess: int = 0,
                  want_vectors: bool = True, guess_vectors=None, extra_roots: int = 0,
-                 restart: str | None = None):
+                 restart: str | None = None, precond: int = 0):
         """Block Davidson for the nroots lowest roots.  guess: 0 lowest diagonal,
         1 first SEFs (the reference's), 2 guess_vectors ([n][nroots]);
         restart: a CI-vector file (civec_write) read as guess_vectors, missing
         roots completed with unit vectors (the reference's --vector_restart);
         extra_roots guard roots ride along in the block (not tested; with
-        MIXED only in the FP32 phase)."""
+        MIXED only in the FP32 phase); precond 0: the reference's diagonal
+        preconditioner, 1: the same with Olsen's correction."""
         if restart is not None:
             guess_vectors, _ = civec_read(restart, self.n, nroots)
             guess = 2
-        p = DavParams(nroots, prec, mode, max_iter, max_subspace, guess, tol, switch_tol, extra_roots)
+        p = DavParams(nroots, prec, mode, max_iter, max_subspace, guess, tol, switch_tol, extra_roots, precond)
         ev = np.zeros(nroots)
         vec = np.zeros((self.n, nroots)) if want_vectors else None
         if guess == 2:
@@ -340,11 +344,29 @@ class Hamiltonian:
                                          ctypes.c_void_p(B.data_ptr()), B.stride(0), _p(out)))
         return out
 
+    def combine(self, A, Y, Out, alpha: float = 0.0):
+        """Out = alpha Out + A Y over the rows of GPU tensors (FP64 MFMA);
+        Y host [m][k]; torch's pending work on A and Out is finished first."""
+        import torch
+        torch.cuda.current_stream().synchronize()
+        n, m = A.shape
+        k = Out.shape[1]
+        Yh = np.ascontiguousarray(Y, np.float64).reshape(m, k)
+        self._check(self.L.psoci_combine(self._h, n, m, k, ctypes.c_void_p(A.data_ptr()), A.stride(0), _p(Yh),
+                                         float(alpha), ctypes.c_void_p(Out.data_ptr()), Out.stride(0)))
+        return Out
+
     def direct_info(self):
         """(connected pairs, structural SEF elements) per direct sigma."""
         a, b = ctypes.c_int64(), ctypes.c_int64()
         self._check(self.L.psoci_direct_info(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def direct_det_pairs(self) -> int:
+        """Connected determinant pairs per direct sigma (counted once)."""
+        v = ctypes.c_int64()
+        self._check(self.L.psoci_direct_det_pairs(self._h, ctypes.byref(v)))
+        return v.value
 
     def direct_schedule(self):
         """The direct sigma's autotuned schedule: dict(order, chunk, tune_ms)."""

@@ -61,7 +61,29 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
     del newest_header
+    build_cli(verbose)
     return LIB
+
+
+CLI_DIR = os.path.join(HERE, "cli")
+INCLUDE = os.path.join(HERE, "..", "include")
+
+
+def build_cli(verbose: bool = False) -> None:
+    """The C99 programs over the ABI: psoci_run (the command-line driver) and
+    abi_layout (struct layout probe), built with gcc against include/psoci.h;
+    psoci_run links libpsoci_amd.so from its parent directory."""
+    cc = os.environ.get("CC", "gcc")
+    common = [cc, "-std=c99", "-Wall", "-Wextra", "-Werror", "-pedantic", "-O2", "-I", INCLUDE]
+    jobs = [
+        common + [os.path.join(CLI_DIR, "psoci_run.c"), "-o", os.path.join(CLI_DIR, "psoci_run"), "-L", HERE,
+                  "-lpsoci_amd", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib"],
+        common + [os.path.join(CLI_DIR, "abi_layout.c"), "-o", os.path.join(CLI_DIR, "abi_layout")],
+    ]
+    for cmd in jobs:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
 
 
 if __name__ == "__main__":

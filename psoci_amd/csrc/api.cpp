@@ -220,6 +220,15 @@ int psoci_direct_info(psoci_ctx* ctx, int64_t* npairs, int64_t* nelems) {
   });
 }
 
+int psoci_direct_det_pairs(psoci_ctx* ctx, int64_t* det_pairs) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
+    PSOCI_CHECK(det_pairs, PSOCI_ERR_ARG, "null output");
+    *det_pairs = psoci::direct_det_pairs(c);
+    return PSOCI_OK;
+  });
+}
+
 int psoci_direct_schedule(psoci_ctx* ctx, int* order, int* chunk, double* tune_ms) {
   return guarded(ctx, [&](Ctx& c) {
     PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
@@ -361,6 +370,19 @@ int psoci_project(psoci_ctx* ctx, int64_t n, int m, int b, const double* A, int 
     psoci::Linalg L;
     L.stream = c.stream;
     L.gemm_tn_host(n, m, b, A, lda, B, ldb, out);
+    return PSOCI_OK;
+  });
+}
+
+int psoci_combine(psoci_ctx* ctx, int64_t n, int m, int k, const double* A, int lda, const double* Y, double alpha,
+                  double* Out, int ldo) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(n >= 1 && m >= 1 && k >= 1 && m <= lda && k <= ldo && A && Y && Out, PSOCI_ERR_ARG,
+                "bad combine arguments");
+    psoci::Linalg L;
+    L.stream = c.stream;
+    L.gemm_nn(n, m, k, A, lda, Y, alpha, Out, ldo);
+    c.sync();
     return PSOCI_OK;
   });
 }

@@ -57,6 +57,7 @@ struct Ctx {
   bool have_pairs = false;
   int64_t nadj = 0;
   int64_t direct_elems = 0;  // structural SEF elements per direct sigma
+  int64_t direct_det_pairs = -1;  // connected determinant pairs per direct sigma (-1: not counted yet)
   DevBuf<int64_t> d_adjoff;   // ncfg + 1: first record of each configuration
   DevBuf<PairRec> d_rec;      // one record per ordered connected pair
   DevBuf<double> d_cdiag;     // ncfg: vnt[0] of each diagonal block
@@ -97,6 +98,7 @@ void compute_diag(Ctx& c);              // diagonal only
 void block_on_device(Ctx& c, int64_t I, int64_t J, double* host_out, int* status);
 void sigma_stored(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy, const Staging& st = Staging{});
 void sigma_direct(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy, const Staging& st = Staging{});
+int64_t direct_det_pairs(Ctx& c);  // connected determinant pairs per direct sigma (one counting launch, cached)
 // staging bytes a sigma call of k columns at strides ldx, ldy needs (0: none)
 size_t sigma_staging_bytes(const Ctx& c, int mode, int prec, int k, const void* X, int ldx, const void* Y, int ldy);
 // the allocation-free scratch array of the current device (noalloc.hip):

@@ -10,7 +10,9 @@
 // kept from the reference: Rayleigh-Ritz on V^T H V, residual
 // r = sum_j y_j (HV_j - lambda V_j) (PsociGAsubspace.C:874-989), the diagonal
 // preconditioner r_i / (D_ii - lambda) applied where |D_ii - lambda| > 1e-3
-// (DAVIDSON_MIN, PsociGAbasis.C:1860-1920), and Ritz collapse that
+// (DAVIDSON_MIN, PsociGAbasis.C:1860-1920) -- optionally with Olsen's
+// correction against the Ritz vector, which keeps the correction from
+// collapsing onto it -- and Ritz collapse that
 // transforms HV instead of recomputing it (expandAndReplaceVectorsNew,
 // PsociGAbasis.C:483-579).
 //
@@ -55,6 +57,8 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
   PSOCI_CHECK(p.prec == PSOCI_FP64 || p.prec == PSOCI_FP32 || p.prec == PSOCI_MIXED, PSOCI_ERR_ARG, "bad precision");
   PSOCI_CHECK(p.sigma_mode == PSOCI_SIGMA_STORED || p.sigma_mode == PSOCI_SIGMA_DIRECT, PSOCI_ERR_ARG, "bad sigma mode");
   PSOCI_CHECK(p.tol > 0, PSOCI_ERR_ARG, "tol must be positive");
+  PSOCI_CHECK(p.precond == 0 || p.precond == 1, PSOCI_ERR_ARG, "precond must be 0 or 1");
+  const bool olsen = p.precond == 1;
   int mmax = p.max_subspace > 0 ? p.max_subspace : 5 * k;
   mmax = std::max(mmax, 3 * k);
   if (mmax > n) mmax = (int)n;
@@ -214,7 +218,10 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     Yk2 = Yk;
     for (int i = 0; i < m; ++i)
       for (int j = 0; j < k; ++j) Yk2[(size_t)i * k + j] *= w[j];
-    L.gemm_nn2(n, m, k, HV.p, mmax, Yk.data(), V.p, mmax, Yk2.data(), R.p, k);
+    // residuals R = HV Yk - V Yk diag(w); with Olsen's correction also the
+    // Ritz vectors X = V Yk (into T, free until the next collapse) in the
+    // same pass over V and HV
+    L.gemm_nn2(n, m, k, HV.p, mmax, Yk.data(), V.p, mmax, Yk2.data(), R.p, k, olsen ? T.p : nullptr, k);
     G.resize((size_t)k * k);
     L.gemm_tn_host(n, k, k, R.p, k, R.p, k, G.data());
     double maxr = 0.0;
@@ -252,6 +259,8 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     for (int j = 0; j < k; ++j)
       if (rn[j] >= sel_tol) sel.push_back(j);
     int nsel = (int)sel.size();
+    if (olsen) L.precondition_olsen(n, k, c.d_diag.p, w.data(), 1.0e-3, R.p, k, T.p, k);
+    else L.precondition(n, k, c.d_diag.p, w.data(), 1.0e-3, R.p, k);
     if (m + nsel > mmax) {
       collapse(kc);
       // the retained Ritz values and vectors are unchanged: w[0..k) still valid
@@ -262,7 +271,6 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     // whole space, whose Ritz pairs are then exact
     if (m + nsel > mmax) nsel = mmax - m;
     if (nsel <= 0) break;
-    L.precondition(n, k, c.d_diag.p, w.data(), 1.0e-3, R.p, k);
     // new block = selected preconditioned residuals, placed at V[:, m:m+nsel)
     std::vector<double> Ssel((size_t)k * nsel, 0.0);
     for (int j = 0; j < nsel; ++j) Ssel[(size_t)sel[j] * nsel + j] = 1.0;

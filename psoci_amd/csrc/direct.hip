@@ -9,7 +9,7 @@
 //
 // Layout on the GPU:
 //  * once per space and integrals, every connected ordered pair (I, J) gets
-//    a 24-byte record: J, the excitation orbitals and block kind of the
+//    a 32-byte record: J, the excitation orbitals and block kind of the
 //    reference orientation (row = the later configuration), and its
 //    configuration-level sums vnt[0], vnt[1] (pair_setup, computed once and
 //    reused by every sigma of a solve);
@@ -286,6 +286,7 @@ struct DirectArgs {
   int chunk;             // XCD run length in workgroups (0: plain round-robin)
   int64_t row0, nrows;   // the slice of the row order to process (all rows but when tuning)
   int split;             // rows are work items I << 2 | q: SEFs [64q, 64q + 64) of I
+  unsigned long long* count;  // K = 0 (counting launch): connected determinant pairs visited
 };
 
 __device__ __forceinline__ ConfView view_of(const ConfRec& r) {
@@ -355,7 +356,7 @@ __device__ __forceinline__ void axpy_row(T hv, const T* __restrict__ x, T (&acc)
 template <typename T, int K, bool VEC, int KIND, bool FRAC>
 __device__ __forceinline__ void pair_rows(const DirectArgs& a, const PairRec& r, const ConfView& A, uint64_t aa,
                                           const int16_t* __restrict__ s_didx, const T* __restrict__ X, int ldx,
-                                          T (&acc)[K]) {
+                                          T (&acc)[K > 0 ? K : 1], unsigned long long& nconn) {
   PairInfo P;
   bool rowI;
   int J;
@@ -368,10 +369,14 @@ __device__ __forceinline__ void pair_rows(const DirectArgs& a, const PairRec& r,
   const T* __restrict__ xJ = X + (int64_t)rj.sefoff * ldx;
   const NewShells w{r.norb, r.npos & 0xffffu, (int)(r.npos >> 16)};
   for_each_conn<KIND == 3 ? 2 : 1>(A, B, aa, w, s_didx + (1 << B.nopen) - 1, [&](int b, uint64_t ab) {
-    const double v = conn_value<KIND>(P, xrow, rowI, A, B, aa, ab, a.two);
-    if (v == 0.0) return;
-    const T hv = (T)(((b & 1) ? pc.co : pc.ce) * v);
-    axpy_row<T, K, VEC>(hv, xJ + (int64_t)(b >> sh) * ldx, acc);
+    if constexpr (K == 0) {
+      ++nconn;
+    } else {
+      const double v = conn_value<KIND>(P, xrow, rowI, A, B, aa, ab, a.two);
+      if (v == 0.0) return;
+      const T hv = (T)(((b & 1) ? pc.co : pc.ce) * v);
+      axpy_row<T, K, VEC>(hv, xJ + (int64_t)(b >> sh) * ldx, acc);
+    }
   });
 }
 
@@ -383,7 +388,9 @@ __device__ __forceinline__ void pair_rows(const DirectArgs& a, const PairRec& r,
 // the end; the order is fixed, so results are deterministic.  A
 // configuration with more than 64 SEFs (9 or 10 open shells: 128 or 256)
 // is split over wavefronts of 64 SEFs each, one record slot per lane; every
-// SEF count is a power of two, so the slots divide the wavefront.
+// SEF count is a power of two, so the slots divide the wavefront.  K = 0 is
+// a counting launch: the same enumeration, no element values or products,
+// the connected determinant pairs summed into a.count.
 template <typename T, int K, bool VEC, bool FRAC>
 __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restrict__ X, int ldx, T* __restrict__ Y,
                                                 int ldy) {
@@ -402,9 +409,10 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
   const int slot = lane / nW, i = i0 + lane - slot * nW;
   constexpr int sh = FRAC ? 0 : 1;  // SEF i <- determinant i << sh
   const uint64_t aa = A.nopen ? a.detao[ri.detoff + (i << sh)] : 0ull;
-  T acc[K];
+  T acc[K > 0 ? K : 1];
 #pragma unroll
   for (int v = 0; v < K; ++v) acc[v] = (T)0;
+  unsigned long long nconn = 0;
   if (slot == 0) {
     // the diagonal block (J = I): determinant 2i itself and its alpha-beta
     // spin exchanges, from the configuration's vnt[0] and the integral table
@@ -417,10 +425,14 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
     const PairCoef pc = pair_coef(A.nopen, A.nopen, a.mult, FRAC);
     const T* __restrict__ xI = X + (int64_t)ri.sefoff * ldx;
     for_each_conn_diag(A, aa, s_didx + (1 << A.nopen) - 1, [&](int b, uint64_t ab) {
-      const double v = detval_ao(P, A, A, aa, ab, a.two);
-      if (v == 0.0) return;
-      const T hv = (T)(((b & 1) ? pc.co : pc.ce) * v);
-      axpy_row<T, K, VEC>(hv, xI + (int64_t)(b >> sh) * ldx, acc);
+      if constexpr (K == 0) {
+        ++nconn;
+      } else {
+        const double v = detval_ao(P, A, A, aa, ab, a.two);
+        if (v == 0.0) return;
+        const T hv = (T)(((b & 1) ? pc.co : pc.ce) * v);
+        axpy_row<T, K, VEC>(hv, xI + (int64_t)(b >> sh) * ldx, acc);
+      }
     });
   }
   // double excitations, then single excitations (the segment's order)
@@ -428,11 +440,16 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
   const int64_t r1 = r0 + a.ndbl[I], r2 = r1 + a.nsgl[I];
   for (int64_t t0 = r0; t0 < r1; t0 += nslot) {
     const int64_t t = t0 + slot;
-    if (t < r1) pair_rows<T, K, VEC, 3, FRAC>(a, a.rec[t], A, aa, s_didx, X, ldx, acc);
+    if (t < r1) pair_rows<T, K, VEC, 3, FRAC>(a, a.rec[t], A, aa, s_didx, X, ldx, acc, nconn);
   }
   for (int64_t t0 = r1; t0 < r2; t0 += nslot) {
     const int64_t t = t0 + slot;
-    if (t < r2) pair_rows<T, K, VEC, 2, FRAC>(a, a.rec[t], A, aa, s_didx, X, ldx, acc);
+    if (t < r2) pair_rows<T, K, VEC, 2, FRAC>(a, a.rec[t], A, aa, s_didx, X, ldx, acc, nconn);
+  }
+  if constexpr (K == 0) {
+    for (int off = kWave / 2; off > 0; off >>= 1) nconn += __shfl_xor(nconn, off, kWave);
+    if (lane == 0) atomicAdd(a.count, nconn);
+    return;
   }
   for (int off = nW; off < kWave; off <<= 1) {
 #pragma unroll
@@ -468,12 +485,15 @@ DirectArgs direct_args(Ctx& c) {
   a.row0 = 0;
   a.nrows = c.direct_nrows;
   a.split = c.direct_split ? 1 : 0;
+  a.count = nullptr;
   return a;
 }
 
 template <typename T, int K>
-void launch_direct(Ctx& c, const T* X, int ldx, T* Y, int ldy, int64_t row0 = 0, int64_t nrows = -1) {
+void launch_direct(Ctx& c, const T* X, int ldx, T* Y, int ldy, int64_t row0 = 0, int64_t nrows = -1,
+                   unsigned long long* count = nullptr) {
   DirectArgs a = direct_args(c);
+  a.count = count;
   if (nrows >= 0) {
     a.row0 = row0;
     a.nrows = nrows;
@@ -571,6 +591,7 @@ void tune_direct(Ctx& c) {
 
 void build_pairs(Ctx& c) {
   const Space& sp = c.sp;
+  c.direct_det_pairs = -1;
   PSOCI_CHECK(sp.ndet_total < ((int64_t)1 << 31), PSOCI_ERR_UNSUPPORTED, "more than 2^31 determinants");
   const int64_t ncfg = sp.ncfg;
   const int max_nsef = *std::max_element(sp.nsef.begin(), sp.nsef.end());
@@ -670,6 +691,21 @@ void build_pairs(Ctx& c) {
   c.nadj = total;
   c.have_pairs = true;
   tune_direct(c);
+}
+
+int64_t direct_det_pairs(Ctx& c) {
+  if (!c.have_pairs) build_pairs(c);
+  if (c.direct_det_pairs < 0) {
+    DevBuf<unsigned long long> cnt;
+    cnt.alloc(1);
+    PSOCI_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), c.stream));
+    launch_direct<double, 0>(c, nullptr, 0, nullptr, 0, 0, -1, cnt.p);
+    unsigned long long v = 0;
+    cnt.download(&v, 1, c.stream);
+    c.sync();
+    c.direct_det_pairs = (int64_t)v;
+  }
+  return c.direct_det_pairs;
 }
 
 void sigma_direct(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy, const Staging& st) {

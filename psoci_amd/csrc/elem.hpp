@@ -59,6 +59,9 @@ struct PairRec {
   // pattern, 4 bits each (bits 0-15), and their count (bits 16-18)
   uint32_t norb, npos;
 };
+// 32 bytes: mrci10x24's 3,006,213,765 records take 96.2 GB of HBM
+static_assert(sizeof(PairRec) == 32, "PairRec layout");
+static_assert(sizeof(ConfRec) == 32, "ConfRec layout");
 
 struct ConfView {
   uint64_t d, s;

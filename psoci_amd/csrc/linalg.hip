@@ -83,38 +83,137 @@ __global__ __launch_bounds__(256) void k_reduce_chunks(int64_t nchunks, int mb, 
   if (threadIdx.x == 0) out[e] = sh[0];
 }
 
-// Out[row][0..k) = alpha * Out + sum_t A[row][t] * Y1[t][c]  (- sum_t B[row][t] * Y2[t][c])
-// Y matrices [m][k] staged in LDS; one thread per (row, c).
-template <bool TWO, bool LDS>
-__global__ __launch_bounds__(256) void k_gemm_nn(int64_t n, int m, int k, const double* __restrict__ A, int lda,
-                                                 const double* __restrict__ Y1, const double* __restrict__ B, int ldb,
-                                                 const double* __restrict__ Y2, double alpha, double* __restrict__ Out,
-                                                 int ldo) {
-  extern __shared__ double sy[];
-  const double* y1 = Y1;
-  const double* y2 = Y2;
-  if (LDS) {
-    for (int e = threadIdx.x; e < m * k; e += blockDim.x) {
-      sy[e] = Y1[e];
-      if (TWO) sy[m * k + e] = Y2[e];
-    }
-    __syncthreads();
-    y1 = sy;
-    y2 = sy + m * k;
+// Out[r][c] = alpha Out[r][c] + sum_t A[r][t] Y1[t][c] (+ sum_t B[r][t] Y2[t][c]
+// with TWO; the caller passes -Y2 for A Y1 - B Y2) and, with XOUT,
+// Out2[r][c] = sum_t B[r][t] Y1[t][c], on v_mfma_f64_16x16x4f64.  One
+// wavefront per 16-row tile and NCT 16-column tiles (columns from
+// 128 * blockIdx.y).  Per chunk of 16 values of t, lane l (i = l % 16,
+// q = l / 16) reads A[r0 + i][t0 + 4q .. t0 + 4q + 3]; in step u it feeds
+// A[r0 + i][t0 + 4q + u] and Y[t0 + 4q + u][c0 + i], so k slot q of the
+// instruction carries t = t0 + 4q + u for both operands.  Accumulator
+// register j of lane l holds row r0 + 4j + q, column c0 + i.  Each A and B
+// row is read from HBM once; Y (m x k, tiny) is read through the caches.
+template <int NCT, bool TWO, bool XOUT>
+__global__ __launch_bounds__(256) void k_gemm_nn_mfma(int64_t n, int m, int k, const double* __restrict__ A,
+                                                      int lda, const double* __restrict__ B, int ldb,
+                                                      const double* __restrict__ Y1, const double* __restrict__ Y2,
+                                                      double alpha, double* Out, int ldo, double* __restrict__ Out2,
+                                                      int ldo2) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int64_t r0 = ((int64_t)blockIdx.x * (blockDim.x / kWave) + wave) * 16;
+  if (r0 >= n) return;  // uniform per wavefront
+  const int cb = blockIdx.y * 128;
+  const int i = lane & 15, q = lane >> 4;
+  const int64_t row = r0 + i;
+  const bool rv = row < n;
+  const double* __restrict__ a_row = A + (rv ? row : 0) * (int64_t)lda;
+  const double* __restrict__ b_row = TWO ? B + (rv ? row : 0) * (int64_t)ldb : nullptr;
+  double4_t acc[NCT], acc2[NCT];
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct) {
+    acc[ct] = double4_t{0.0, 0.0, 0.0, 0.0};
+    acc2[ct] = acc[ct];
   }
+  for (int t0 = 0; t0 < m; t0 += 16) {
+    const int tq = t0 + 4 * q;
+    double a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool tv = rv && tq + u < m;
+      a[u] = tv ? a_row[tq + u] : 0.0;
+      b[u] = (TWO && tv) ? b_row[tq + u] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int t = tq + u;
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) {
+        const int c = cb + ct * 16 + i;
+        const bool yv = t < m && c < k;
+        const double y1 = yv ? Y1[(int64_t)t * k + c] : 0.0;
+        acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], y1, acc[ct], 0, 0, 0);
+        if (XOUT) acc2[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[u], y1, acc2[ct], 0, 0, 0);
+        if (TWO) {
+          const double y2 = yv ? Y2[(int64_t)t * k + c] : 0.0;
+          acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[u], y2, acc[ct], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct) {
+    const int c = cb + ct * 16 + i;
+    if (c >= k) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t rr = r0 + 4 * j + q;
+      if (rr >= n) continue;
+      double* o = Out + rr * ldo + c;
+      *o = (alpha == 0.0) ? acc[ct][j] : __builtin_fma(alpha, *o, acc[ct][j]);
+      if (XOUT) Out2[rr * ldo2 + c] = acc2[ct][j];
+    }
+  }
+}
+
+// Olsen's correction of the diagonal preconditioner, per column c (root
+// theta_c, residual r, Ritz vector x, f = 1 / (D - theta_c) where
+// |D - theta_c| > dmin, else 1): t = f (r - eps x) with
+// eps = (x . f r) / (x . f x), which makes t orthogonal to x in the metric
+// of the preconditioner.  k_olsen_dots: per block of rows, the partial sums
+// of x f r (partial[blk][c]) and x f x (partial[blk][k + c]); threads are
+// (row slot, column) with 256 / k row slots, summed in a fixed order.
+__global__ __launch_bounds__(256) void k_olsen_dots(int64_t n, int k, const double* __restrict__ D,
+                                                    const double* __restrict__ theta, double dmin,
+                                                    const double* __restrict__ R, int ldr,
+                                                    const double* __restrict__ X, int ldx, int64_t rows_per_blk,
+                                                    double* __restrict__ partial) {
+  __shared__ double sxr[256], sxx[256];
+  const int slots = 256 / k;
+  const int t = threadIdx.x;
+  const int c = t % k, slot = t / k;
+  double axr = 0.0, axx = 0.0;
+  if (slot < slots) {
+    const double th = theta[c];
+    const int64_t lo = (int64_t)blockIdx.x * rows_per_blk;
+    const int64_t hi = lo + rows_per_blk < n ? lo + rows_per_blk : n;
+    for (int64_t row = lo + slot; row < hi; row += slots) {
+      const double den = D[row] - th;
+      const double f = fabs(den) > dmin ? 1.0 / den : 1.0;
+      const double x = X[row * ldx + c];
+      axr += x * (f * R[row * ldr + c]);
+      axx += x * (f * x);
+    }
+  }
+  sxr[t] = axr;
+  sxx[t] = axx;
+  __syncthreads();
+  if (t < k) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int j = 0; j < slots; ++j) {
+      s1 += sxr[j * k + t];
+      s2 += sxx[j * k + t];
+    }
+    partial[(int64_t)blockIdx.x * 2 * k + t] = s1;
+    partial[(int64_t)blockIdx.x * 2 * k + k + t] = s2;
+  }
+}
+
+// R[row][c] <- f (R - eps_c X), eps_c = dots[c] / dots[k + c] (0 when the
+// denominator vanishes)
+__global__ void k_olsen_apply(int64_t n, int k, const double* __restrict__ D, const double* __restrict__ theta,
+                              double dmin, const double* __restrict__ dots, double* __restrict__ R, int ldr,
+                              const double* __restrict__ X, int ldx) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t row = g / k;
   const int c = (int)(g - row * k);
   if (row >= n) return;
-  const double* a = A + row * lda;
-  double s = 0.0;
-  for (int t = 0; t < m; ++t) s = __builtin_fma(a[t], y1[t * k + c], s);
-  if (TWO) {
-    const double* bb = B + row * ldb;
-    for (int t = 0; t < m; ++t) s = __builtin_fma(-bb[t], y2[t * k + c], s);
-  }
-  double* o = Out + row * ldo + c;
-  *o = (alpha == 0.0) ? s : __builtin_fma(alpha, *o, s);
+  const double den = D[row] - theta[c];
+  const double f = fabs(den) > dmin ? 1.0 / den : 1.0;
+  const double bx = dots[k + c];
+  const double eps = bx != 0.0 ? dots[c] / bx : 0.0;
+  double* r = R + row * ldr + c;
+  *r = f * (*r - eps * X[row * ldx + c]);
 }
 
 // t[row][c] = r[row][c] / (D[row] - theta[c]) when |D - theta| > dmin
@@ -190,41 +289,75 @@ void Linalg::gemm_tn_host(int64_t n, int m, int b, const double* A, int lda, con
   PSOCI_HIP(hipStreamSynchronize(stream));
 }
 
-void Linalg::gemm_nn(int64_t n, int m, int k, const double* A, int lda, const double* Y1_host, double alpha, double* Out,
-                     int ldo) {
-  if (k == 0) return;
-  ysmall.reserve((size_t)m * k);
-  PSOCI_HIP(hipMemcpyAsync(ysmall.p, Y1_host, (size_t)m * k * sizeof(double), hipMemcpyHostToDevice, stream));
-  size_t shm = (size_t)m * k * sizeof(double);
-  if (shm <= kMaxLds)
-    hipLaunchKernelGGL((k_gemm_nn<false, true>), dim3(nblk(n * k, 256)), dim3(256), shm, stream, n, m, k, A, lda,
-                       ysmall.p, nullptr, 0, nullptr, alpha, Out, ldo);
-  else
-    hipLaunchKernelGGL((k_gemm_nn<false, false>), dim3(nblk(n * k, 256)), dim3(256), 0, stream, n, m, k, A, lda,
-                       ysmall.p, nullptr, 0, nullptr, alpha, Out, ldo);
+namespace {
+
+template <bool TWO, bool XOUT>
+void launch_nn(hipStream_t st, int64_t n, int m, int k, const double* A, int lda, const double* B, int ldb,
+               const double* Y1, const double* Y2, double alpha, double* Out, int ldo, double* Out2, int ldo2) {
+  const dim3 grid(nblk(n, 64), (unsigned)((k + 127) / 128));
+  const int nct = std::min(8, (k + 15) / 16);
+#define PSOCI_NN(NCT)                                                                                             \
+  hipLaunchKernelGGL((k_gemm_nn_mfma<NCT, TWO, XOUT>), grid, dim3(256), 0, st, n, m, k, A, lda, B, ldb, Y1, Y2, \
+                     alpha, Out, ldo, Out2, ldo2)
+  if (nct <= 1) PSOCI_NN(1);
+  else if (nct <= 2) PSOCI_NN(2);
+  else if (nct <= 4) PSOCI_NN(4);
+  else PSOCI_NN(8);
+#undef PSOCI_NN
   PSOCI_HIP(hipGetLastError());
 }
 
-void Linalg::gemm_nn2(int64_t n, int m, int k, const double* A, int lda, const double* Y1_host, const double* B, int ldb,
-                      const double* Y2_host, double* Out, int ldo) {
-  ysmall.reserve((size_t)2 * m * k);
+}  // namespace
+
+void Linalg::gemm_nn(int64_t n, int m, int k, const double* A, int lda, const double* Y1_host, double alpha, double* Out,
+                     int ldo) {
+  if (k == 0 || n == 0) return;
+  ysmall.reserve((size_t)m * k);
   PSOCI_HIP(hipMemcpyAsync(ysmall.p, Y1_host, (size_t)m * k * sizeof(double), hipMemcpyHostToDevice, stream));
-  PSOCI_HIP(hipMemcpyAsync(ysmall.p + (size_t)m * k, Y2_host, (size_t)m * k * sizeof(double), hipMemcpyHostToDevice,
-                           stream));
-  size_t shm = (size_t)2 * m * k * sizeof(double);
-  if (shm <= kMaxLds)
-    hipLaunchKernelGGL((k_gemm_nn<true, true>), dim3(nblk(n * k, 256)), dim3(256), shm, stream, n, m, k, A, lda,
-                       ysmall.p, B, ldb, ysmall.p + (size_t)m * k, 0.0, Out, ldo);
+  launch_nn<false, false>(stream, n, m, k, A, lda, nullptr, 0, ysmall.p, nullptr, alpha, Out, ldo, nullptr, 0);
+}
+
+void Linalg::gemm_nn2(int64_t n, int m, int k, const double* A, int lda, const double* Y1_host, const double* B, int ldb,
+                      const double* Y2_host, double* Out, int ldo, double* X, int ldx) {
+  if (k == 0 || n == 0) return;
+  ysmall.reserve((size_t)2 * m * k);
+  yneg.assign(Y2_host, Y2_host + (size_t)m * k);
+  for (auto& v : yneg) v = -v;
+  PSOCI_HIP(hipMemcpyAsync(ysmall.p, Y1_host, (size_t)m * k * sizeof(double), hipMemcpyHostToDevice, stream));
+  PSOCI_HIP(hipMemcpyAsync(ysmall.p + (size_t)m * k, yneg.data(), (size_t)m * k * sizeof(double),
+                           hipMemcpyHostToDevice, stream));
+  if (X)
+    launch_nn<true, true>(stream, n, m, k, A, lda, B, ldb, ysmall.p, ysmall.p + (size_t)m * k, 0.0, Out, ldo, X, ldx);
   else
-    hipLaunchKernelGGL((k_gemm_nn<true, false>), dim3(nblk(n * k, 256)), dim3(256), 0, stream, n, m, k, A, lda,
-                       ysmall.p, B, ldb, ysmall.p + (size_t)m * k, 0.0, Out, ldo);
-  PSOCI_HIP(hipGetLastError());
+    launch_nn<true, false>(stream, n, m, k, A, lda, B, ldb, ysmall.p, ysmall.p + (size_t)m * k, 0.0, Out, ldo,
+                           nullptr, 0);
+  // yneg is read by the copy above: keep it until the stream has consumed it
+  PSOCI_HIP(hipStreamSynchronize(stream));
 }
 
 void Linalg::precondition(int64_t n, int k, const double* D, const double* theta_host, double dmin, double* R, int ldr) {
   theta.reserve(k);
   PSOCI_HIP(hipMemcpyAsync(theta.p, theta_host, k * sizeof(double), hipMemcpyHostToDevice, stream));
   hipLaunchKernelGGL(k_precondition, dim3(nblk(n * k, 256)), dim3(256), 0, stream, n, k, D, theta.p, dmin, R, ldr);
+  PSOCI_HIP(hipGetLastError());
+}
+
+void Linalg::precondition_olsen(int64_t n, int k, const double* D, const double* theta_host, double dmin, double* R,
+                                int ldr, const double* X, int ldx) {
+  if (k == 0 || n == 0) return;
+  theta.reserve(k);
+  PSOCI_HIP(hipMemcpyAsync(theta.p, theta_host, k * sizeof(double), hipMemcpyHostToDevice, stream));
+  const int64_t rows_per_blk = 4096;
+  const int64_t nb = (n + rows_per_blk - 1) / rows_per_blk;
+  part.reserve((size_t)nb * 2 * k);
+  small.reserve((size_t)2 * k);
+  hipLaunchKernelGGL(k_olsen_dots, dim3((unsigned)nb), dim3(256), 0, stream, n, k, D, theta.p, dmin, R, ldr, X, ldx,
+                     rows_per_blk, part.p);
+  PSOCI_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)(2 * k)), dim3(256), 0, stream, nb, 2 * k, part.p, small.p);
+  PSOCI_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_olsen_apply, dim3(nblk(n * k, 256)), dim3(256), 0, stream, n, k, D, theta.p, dmin, small.p, R,
+                     ldr, X, ldx);
   PSOCI_HIP(hipGetLastError());
 }
 

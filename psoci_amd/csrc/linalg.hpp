@@ -7,6 +7,7 @@ namespace psoci {
 struct Linalg {
   hipStream_t stream = nullptr;
   DevBuf<double> part, small, ysmall, theta;
+  std::vector<double> yneg;
 
   // out (device, m x b row-major) = A[:, 0:m]^T B[:, 0:b] over n rows (FP64 MFMA)
   void gemm_tn(int64_t n, int m, int b, const double* A, int lda, const double* B, int ldb, double* out_dev);
@@ -15,11 +16,16 @@ struct Linalg {
   // Out[:, 0:k] = alpha * Out + A[:, 0:m] Y1 (Y1 host, m x k row-major)
   void gemm_nn(int64_t n, int m, int k, const double* A, int lda, const double* Y1_host, double alpha, double* Out,
                int ldo);
-  // Out[:, 0:k] = A Y1 - B Y2 (Y1, Y2 host, m x k)
+  // Out[:, 0:k] = A Y1 - B Y2 (Y1, Y2 host, m x k); with X set, also X = B Y1
+  // in the same pass over A and B (synchronises)
   void gemm_nn2(int64_t n, int m, int k, const double* A, int lda, const double* Y1_host, const double* B, int ldb,
-                const double* Y2_host, double* Out, int ldo);
+                const double* Y2_host, double* Out, int ldo, double* X = nullptr, int ldx = 0);
   // R[:, c] /= (D - theta_c) where |D - theta_c| > dmin
   void precondition(int64_t n, int k, const double* D, const double* theta_host, double dmin, double* R, int ldr);
+  // the same with Olsen's correction against the Ritz vectors X:
+  // R[:, c] = f (R - eps_c X) with eps_c = (x . f r) / (x . f x)
+  void precondition_olsen(int64_t n, int k, const double* D, const double* theta_host, double dmin, double* R, int ldr,
+                          const double* X, int ldx);
   void to_f32(int64_t n, int k, const double* A, int lda, float* B, int ldb);
   void to_f64(int64_t n, int k, const float* A, int lda, double* B, int ldb);
   void copy_cols(int64_t n, int k, const double* A, int lda, double* B, int ldb);

@@ -151,3 +151,28 @@ def test_direct_full_size_mrci10x24():
     v = X[:, 1].copy()
     assert abs(v @ Y[:, 0] - u @ Y[:, 1]) <= 1e-11 * abs(v @ Y[:, 0])
     H.close()
+
+
+def test_det_pair_count_fci6(fci6):
+    """The work counter of the direct sigma (connected determinant pairs per
+    launch) equals a brute-force count: every ordered configuration pair
+    within two moved electrons and four open shells (J = I included), every
+    row determinant 2i (one per SEF) against every column determinant with
+    the same alpha count and at most two spin-orbital replacements."""
+    cfg, ints, _ = fci6
+    H = api.Hamiltonian(cfg, ints)
+    dets = [H.dets(I) for I in range(cfg.ncfg)]
+    pc = np.vectorize(lambda x: bin(int(x)).count("1"))
+    total = 0
+    for I in range(cfg.ncfg):
+        a = dets[I]
+        rows = np.arange(0, len(a["alpha"]), 2)[:int(H.info.nsef[I])]
+        aa, ab = a["alpha"][rows], a["beta"][rows]
+        for J in connected(cfg, I):
+            b = dets[int(J)]
+            da = np.bitwise_xor.outer(aa, b["alpha"])
+            db = np.bitwise_xor.outer(ab, b["beta"])
+            same_ms = pc(aa)[:, None] == pc(b["alpha"])[None, :]
+            total += int(((pc(da) + pc(db) <= 4) & same_ms).sum())
+    assert H.direct_det_pairs() == total
+    H.close()

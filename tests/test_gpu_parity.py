@@ -133,3 +133,18 @@ def test_error_codes_not_faults(gpu_fci6):
         H2.sigma(np.ones(H2.n), api.STORED)
     assert e.value.code == api.ERR_STATE
     H2.close()
+
+
+def test_olsen_preconditioner_same_roots_fci10(gpu_fci10):
+    """precond 1 (the diagonal preconditioner with Olsen's correction)
+    converges to the roots of the reference's plain diagonal one."""
+    H = gpu_fci10
+    a = H.davidson(8, prec=api.FP64, tol=1e-7, precond=0, want_vectors=False)
+    b = H.davidson(8, prec=api.FP64, tol=1e-7, precond=1, want_vectors=False)
+    c = H.davidson(8, prec=api.MIXED, tol=1e-7, precond=1, want_vectors=False)
+    assert a["converged"] and b["converged"] and c["converged"]
+    np.testing.assert_allclose(b["evals"], a["evals"], atol=1e-9, rtol=0)
+    np.testing.assert_allclose(c["evals"], a["evals"], atol=1e-8, rtol=0)
+    with pytest.raises(api.PsociError) as e:
+        H.davidson(2, prec=api.FP64, precond=2)
+    assert e.value.code == api.ERR_ARG
