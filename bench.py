@@ -362,12 +362,19 @@ def main():
     if not direct:
         line["stored_shape"] = {"fp32": H.stored_shape(api.FP32), "fp64": H.stored_shape(api.FP64)}
     if direct:
-        line["direct"] = {"ordered_pairs": npairs, "structural_elements_per_sigma": nelems,
+        dp = H.direct_det_pairs()
+        line["direct"] = {"ordered_configuration_pairs": npairs, "structural_sef_elements_per_sigma": nelems,
+                          "connected_determinant_pairs_per_sigma": dp,
                           "schedule": H.direct_schedule(),
-                          "elements_per_s_fp32": round(nelems / (ms32 * 1e-3), 1),
-                          "elements_per_s_fp64": round(nelems / (ms64 * 1e-3), 1),
-                          "note": "compute-bound (integer + FP64 VALU); the HBM fraction is reported for the "
-                                  "algorithmic bytes only"}
+                          f"det_pairs_per_s_fp32_k{k}": round(dp / (ms32 * 1e-3), 1),
+                          f"det_pairs_per_s_fp32_k{kr}": round(dp / (msr * 1e-3), 1),
+                          f"det_pairs_per_s_fp64_k{k}": round(dp / (ms64 * 1e-3), 1),
+                          "note": ("the work unit is the connected determinant pair (the structural SEF elements "
+                                   "are ~16x more, most of them zero and never generated)")}
+        line["roofline"]["limiter"] = (
+            "not the algorithmic bytes: element generation on the VALU (84 % of the issue rate at 16 FP32 columns) "
+            "and x-row gathers that miss L2 (24 FP32 and all FP64 launches; fabric traffic 4.7-5.5 TB/s), "
+            "profiles/k_direct_mrci10x24_counters.md")
     print(json.dumps(line))
 
 

@@ -94,14 +94,20 @@ int psoci_set_integrals_sifs(psoci_ctx* ctx, const char* path);
  * PsociDeterminants::computeConfigs (PsociDeterminants.C:1083-1191) and
  * computeNsefEntryPoints (PsociGAhamiltonian.C:4022-4048).
  * occ[ncfg][nbf] holds 0/1/2 occupations, jsym[ncfg] the spatial symmetry.
- * Implemented: one common jsym (a C1 double group).  Even electron
- * counts in the state's symmetry class ((jsym + 1) / 2 == (ksym + 1) / 2:
- * the EVEN determinants, PsociDeterminants.C:1563-1610) or not (the ODD
- * determinants of odd Ms, :1613-1658), the SEF sign following the parities
- * of jsym and ksym (PsociHamiltonian.C:1487-1495); odd electron counts with
- * one determinant per SEF (the fractional determinants, :1508-1561, alpha
- * and beta swapped when (jsym + 1) / 2 is even).  Mixed jsym returns
- * PSOCI_ERR_UNSUPPORTED. nbf <= 64, at most 10 open shells. */
+ * Each configuration gets the determinants of its jsym's class: even
+ * electron counts in the state's symmetry class ((jsym + 1) / 2 ==
+ * (ksym + 1) / 2: the EVEN determinants, PsociDeterminants.C:1563-1610) or
+ * not (the ODD determinants of odd Ms, :1613-1658), the SEF sign following
+ * the parities of jsym and ksym (PsociHamiltonian.C:1487-1495); odd
+ * electron counts with one determinant per SEF (the fractional
+ * determinants, :1508-1561, alpha and beta swapped when (jsym + 1) / 2 is
+ * even).  Configurations of different jsym one electron apart couple
+ * through the spin-orbit blocks (processSingleExcitation's SO branch and
+ * socig, PsociHamiltonian.C:223-238, 709-858, with the integral from the
+ * valone slot the SIFS reader's spin-orbit records overwrite); a space
+ * with more than one jsym runs on the stored Hamiltonian (the direct sigma
+ * and psoci_rdm return PSOCI_ERR_UNSUPPORTED for it).  nbf <= 64, at most
+ * 10 open shells. */
 int psoci_set_configs(psoci_ctx* ctx, int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ,
                       const int32_t* jsym);
 
@@ -319,8 +325,9 @@ int psoci_combine(psoci_ctx* ctx, int64_t n, int m, int k, const double* A, int 
 /* Adjoint of E_k: dE/dtheta = c^T (dH/dtheta) c for every packed integral,
  * accumulated over k vectors C[n][k] (row-major, host) with weights w[k]
  * (NULL: all 1).  grad_one[NNDAF(nbf)], grad_two[NNDAF(NNDAF(nbf))];
- * grad_so[NNDAF(nbf)] is the spin-orbit gradient (identically zero for one
- * common jsym, where the reference never enters socig, PsociHamiltonian.C:223-238).
+ * grad_so[NNDAF(nbf)] is the spin-orbit gradient (identically zero: the
+ * adjoint takes spaces of one jsym, where the reference never enters socig,
+ * PsociHamiltonian.C:223-238; others return PSOCI_ERR_UNSUPPORTED).
  * No reference counterpart for the 2-RDM; the 1-RDM part relates to
  * PsociNOpopulations::constructDensityMatrix (PsociNOpopulations.C:858-989)
  * as grad_one[pp] = D_pp and grad_one[pq] = 2 D_pq (p != q). */

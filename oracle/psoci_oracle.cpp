@@ -493,6 +493,104 @@ struct Ham {
   }
 
   // processDiagExcitation (:306-388)
+  // Planted mistakes for the tests that show the spin-orbit invariants catch
+  // them (orc_set_so_mutation; 0 = the restatement as is).
+  static int& so_mutation() {
+    static int m = 0;
+    return m;
+  }
+
+  // processSingleExcitation, spin-orbit branch (:223-238): configurations of
+  // different jsym one electron apart.  vnt[0] = valone(iex, jex) with the
+  // sign flipped when jex > iex, halved; a zero integral leaves vnt empty
+  // (socig is not called) but the block still goes through ciout.
+  bool so_single(const std::vector<int>& iex, const std::vector<int>& jex, std::vector<double>& vnt) const {
+    vnt.clear();
+    int iext = iex[0], jext = jex[0];
+    double vnt1 = in.h(iext, jext);
+    if (vnt1 == 0.0) return false;
+    if (jext > iext && so_mutation() != 1) vnt1 = -vnt1;
+    vnt1 = vnt1 / 2;
+    vnt.push_back(vnt1);
+    return true;
+  }
+
+  // socig (:709-858): determinant pairs at most one spin-orbital apart get
+  // +-vnt[0]; the sign counts the configuration phases, the orbitals the
+  // moved electron passes in its spin list(s), and the Lx / Ly / Lz parity
+  // rules from jsym.
+  void socig(const Joutfg& ic, const Joutfg& jc, const std::vector<int>& iex, const std::vector<int>& jex,
+             const std::vector<double>& vnt, std::vector<double>& detdet) const {
+    int ndeti = ic.ndeti, ndetj = jc.ndeti;
+    int jsymi = ic.jsym, jsymj = jc.jsym;
+    for (int ii = 0; ii < ndeti; ++ii) {
+      const std::vector<int>& aorbsi = ic.alpha_spins[ii];
+      const std::vector<int>& borbsi = ic.beta_spins[ii];
+      int nalphai = ic.nalpha[ii], nbetai = ic.nbeta[ii];
+      std::vector<int> ifiga(nbf, 0), ifigb(nbf, 0);
+      for (int ka = 0; ka < nalphai; ++ka) ifiga[aorbsi[ka]] = 1;
+      for (int ka = 0; ka < nbetai; ++ka) ifigb[borbsi[ka]] = 1;
+      for (int jj = 0; jj < ndetj; ++jj) {
+        const std::vector<int>& aorbsj = jc.alpha_spins[jj];
+        const std::vector<int>& borbsj = jc.beta_spins[jj];
+        int nalphaj = jc.nalpha[jj], nbetaj = jc.nbeta[jj];
+        int nmarka = 0;
+        for (int ka = 0; ka < nalphaj; ++ka) {
+          if (ifiga[aorbsj[ka]] != 1) ++nmarka;
+          if (nmarka > 1) break;
+        }
+        int nmarkb = nmarka;
+        for (int ka = 0; ka < nbetaj; ++ka) {
+          if (ifigb[borbsj[ka]] != 1) ++nmarkb;
+          if (nmarkb > 1) break;
+        }
+        if (nmarka <= 1 && nmarkb <= 1) {
+          int ncount = so_mutation() == 6 ? 0 : ic.phases[ii] + jc.phases[jj];
+          if (nalphai < nalphaj) {
+            for (int i = 0; i < nbetai; ++i)
+              if (borbsi[i] < iex[0]) ++ncount;
+            for (int i = 0; i < nalphaj; ++i)
+              if (aorbsj[i] > jex[0]) ++ncount;
+            if (so_mutation() == 2) {
+            } else if ((jsymi + jsymj) % 2 == 0) {
+              ++ncount;
+            } else if ((jsymi % 2) == 1) {
+              ++ncount;
+            }
+          } else if (nalphai > nalphaj) {
+            for (int i = 0; i < nalphai; ++i)
+              if (aorbsi[i] > iex[0]) ++ncount;
+            for (int i = 0; i < nbetaj; ++i)
+              if (borbsj[i] < jex[0]) ++ncount;
+            if ((jsymi + jsymj) % 2 != 0 && so_mutation() != 3) {
+              if ((jsymi % 2) == 1) ++ncount;
+            }
+          } else {
+            if (nmarka == 0 && so_mutation() != 4) ++ncount;
+            if ((jsymi % 2) == 1) ++ncount;
+            if (std::abs(iex[0] - jex[0]) > 1 && so_mutation() != 5) {
+              int kbs = std::max(iex[0], jex[0]) - 1;
+              int ka = iex[0] + jex[0] - kbs;
+              if (nmarka != 0) {
+                for (int ia = 0; ia < nalphai; ++ia) {
+                  if (aorbsi[ia] > kbs) break;
+                  if (aorbsi[ia] >= ka) ++ncount;
+                }
+              } else {
+                for (int ia = 0; ia < nbetai; ++ia) {
+                  if (borbsi[ia] > kbs) break;
+                  if (borbsi[ia] >= ka) ++ncount;
+                }
+              }
+            }
+          }
+          int index = ii * ndetj + jj;
+          detdet[index] = (ncount % 2 == 1) ? -vnt[0] : vnt[0];
+        }
+      }
+    }
+  }
+
   int diag_exc(const Joutfg& c, const std::vector<int>& iocc, std::vector<double>& vnt) const {
     int nopeni = c.num_singly;
     if (nopeni > 1) {
@@ -789,14 +887,20 @@ struct Ham {
         sefstatus = 1;
       }
     } else {
-      if (ic.jsym != jc.jsym) {
-        err = "spin-orbit (different jsym) blocks are not restated";
-        return -1;
-      }
       int kk = (int)iex.size();
       int status = 0;
-      if (kk == 2) status = double_exc(iex, jex, vnt);
-      else if (kk == 1) status = single_exc(iocc, jocc, iex, jex, vnt);
+      if (kk == 2) {
+        status = double_exc(iex, jex, vnt);
+      } else if (kk == 1 && ic.jsym != jc.jsym) {
+        // spin-orbit single: socig fills detdet (when the integral is not
+        // zero) and processSingleExcitation returns 2, which skips sefcig
+        // and keeps sefstatus = 1, so ciout runs even on a zero detdet
+        // (the outer status is shadowed, PsociHamiltonian.C:531-548)
+        if (so_single(iex, jex, vnt)) socig(ic, jc, iex, jex, vnt, detdet);
+        status = 2;
+      } else if (kk == 1) {
+        status = single_exc(iocc, jocc, iex, jex, vnt);
+      }
       sefstatus = 1;
       if (status == 1) sefcig(ic, jc, iex, jex, vnt, detdet);
       if (status == 0) sefstatus = 0;
@@ -992,6 +1096,9 @@ extern "C" {
 // computePhaseAndSort per configuration (PsociDeterminants.C:749-840,
 // 1083-1191) and computeNsefEntryPoints (PsociGAhamiltonian.C:4022-4048).
 // occ is [ncfg][nbf] with values 0/1/2 in file order.
+// test hook: plant mistake m in the spin-orbit restatement (0: none)
+void orc_set_so_mutation(int m) { Ham::so_mutation() = m; }
+
 void* orc_space_new(int nbf, int nelec, int ksym, long ncfg, const unsigned char* occ, const int* jsym, char* err,
                     int errlen) {
   Space* sp = new Space;

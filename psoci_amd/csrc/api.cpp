@@ -149,7 +149,7 @@ int psoci_dets(psoci_ctx* ctx, int64_t I, uint64_t* alpha, uint64_t* beta, int32
     PSOCI_CHECK(I >= 0 && I < c.sp.ncfg, PSOCI_ERR_ARG, "configuration index out of range");
     const psoci::Space& sp = c.sp;
     int s = sp.nopen[I];
-    const uint16_t* pat = sp.pat.data() + sp.pat_off[s];
+    const uint16_t* pat = sp.pat.data() + sp.pat_off[12 * sp.cls[I] + s];
     for (int d = 0; d < sp.ndet[I]; ++d) {
       uint64_t ao = 0, rest = sp.sm[I];
       uint32_t pbits = pat[d];

@@ -73,19 +73,28 @@ struct DevBuf {
 struct Space {
   int nbf = 0, nelec = 0, ksym = 1, jsym0 = 1;
   int64_t ncfg = 0;
+  // class of configuration 0 (the only one unless mixed):
   double mult = 1.0;  // SEF combination sign: -1 when jsym and ksym parities differ
   bool odd = false;   // ODD class (jsym class != ksym class): determinants of odd Ms
   bool frac = false;  // odd electron count: one determinant per SEF (no pairing)
+  // configurations of more than one jsym: spin-orbit blocks between them;
+  // each configuration has its own determinant class (pattern family 0 =
+  // that of configuration 0, 1 = the other) and SEF combination sign
+  bool mixed = false;
+  std::vector<int32_t> jsym;
+  std::vector<uint8_t> cls;
+  std::vector<double> cmult;
   std::vector<uint64_t> dm, sm;   // doubly / singly occupied orbital masks
   std::vector<int8_t> nopen;
   std::vector<int32_t> nsef, ndet;
   std::vector<int64_t> sef_off, det_off;
   std::vector<uint8_t> cphase;
   int64_t nsef_total = 0, ndet_total = 0;
-  std::vector<uint16_t> pat;  // determinant patterns, all open-shell counts
-  int pat_off[12] = {0};      // pat_off[s]: first pattern of s open shells
-  // didx[(1 << s) - 1 + p]: index of the determinant with alpha pattern p
-  // among those of a configuration with s open shells (-1: not in the class)
+  std::vector<uint16_t> pat;  // determinant patterns, all open-shell counts, families 0 then 1
+  int pat_off[24] = {0};      // pat_off[12 f + s]: first pattern of family f, s open shells
+  // didx[2047 f + (1 << s) - 1 + p]: index of the determinant with alpha
+  // pattern p among those of a family-f configuration with s open shells
+  // (-1: not in the class)
   std::vector<int16_t> didx;
   int max_open = 0;
 };

@@ -33,7 +33,10 @@ struct Ctx {
   DevBuf<int64_t> d_sefoff;
   DevBuf<uint8_t> d_cph;
   DevBuf<uint16_t> d_pat;
-  DevBuf<int32_t> d_patoff;
+  DevBuf<int32_t> d_patoff;    // 24: two pattern families
+  DevBuf<uint8_t> d_cls;       // per configuration: pattern family
+  DevBuf<double> d_cmult;      // per configuration: SEF combination sign
+  DevBuf<int32_t> d_jsym;      // per configuration
   DevBuf<double> d_one, d_two;
 
   // stored Hamiltonian: symmetric full CSR (lower part first in each row)
@@ -125,7 +128,7 @@ inline ConfView conf_view_host(const Space& sp, int64_t c, const uint16_t* pat_d
   v.s = sp.sm[c];
   v.nopen = sp.nopen[c];
   v.ph = sp.cphase[c];
-  v.pat = pat_dev + sp.pat_off[v.nopen];
+  v.pat = pat_dev + sp.pat_off[12 * sp.cls[c] + v.nopen];
   return v;
 }
 

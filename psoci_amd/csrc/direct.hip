@@ -592,6 +592,11 @@ void tune_direct(Ctx& c) {
 void build_pairs(Ctx& c) {
   const Space& sp = c.sp;
   c.direct_det_pairs = -1;
+  // the pair records and the adjoint take one determinant class; spaces with
+  // spin-orbit blocks (configurations of more than one jsym) run stored
+  PSOCI_CHECK(!sp.mixed, PSOCI_ERR_UNSUPPORTED,
+              "the direct sigma and the adjoint take configurations of one jsym; a space with spin-orbit blocks "
+              "needs the stored Hamiltonian (psoci_build_h)");
   PSOCI_CHECK(sp.ndet_total < ((int64_t)1 << 31), PSOCI_ERR_UNSUPPORTED, "more than 2^31 determinants");
   const int64_t ncfg = sp.ncfg;
   const int max_nsef = *std::max_element(sp.nsef.begin(), sp.nsef.end());

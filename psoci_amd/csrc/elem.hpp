@@ -29,9 +29,12 @@ constexpr double kRthlf = 0x1.6a09e667f3bcdp-1;
 
 struct PairInfo {
   int32_t I, J;
-  int8_t kind;  // 0 skipped by the reference, 1 diagonal, 2 single, 3 double
+  // 0 skipped by the reference, 1 diagonal, 2 single, 3 double, 4 spin-orbit
+  // single (configurations of different jsym one electron apart)
+  int8_t kind;
   int8_t e0, e1, f0, f1;  // iex[0..1], jex[0..1] (orbitals leaving I / entering J)
-  int8_t pad[3];
+  int8_t sop;             // kind 4: bit 0 jsym_I odd, bit 1 jsym_I + jsym_J odd
+  int8_t pad[2];
   double v0, v1;  // vnt[0], vnt[1]
 };
 
@@ -122,12 +125,16 @@ PSOCI_HD int moved(uint64_t dI, uint64_t sI, uint64_t dJ, uint64_t sJ) {
 // (PsociHamiltonian.C:436-512) and the configuration-level integral sums
 // processDiagExcitation / processSingleExcitation / processDoubleExcitation
 // (:306-388, :190-261, :139-186), each in the reference's loop order.
+// jsI, jsJ: the configurations' jsym; a single excitation between different
+// jsym is the spin-orbit branch of processSingleExcitation (:223-238).
 PSOCI_HD void pair_setup(PairInfo& P, int I, int J, uint64_t dI, uint64_t sI, int nI, uint64_t dJ, uint64_t sJ,
-                                  int nJ, int nbf, const double* __restrict__ one, const double* __restrict__ two) {
+                                  int nJ, int nbf, const double* __restrict__ one, const double* __restrict__ two,
+                                  int jsI = 1, int jsJ = 1) {
   P.I = I;
   P.J = J;
   P.kind = 0;
   P.e0 = P.e1 = P.f0 = P.f1 = -1;
+  P.sop = 0;
   P.v0 = P.v1 = 0.0;
   int dn = nI - nJ;
   if (dn > 4 || dn < -4) return;
@@ -194,6 +201,20 @@ PSOCI_HD void pair_setup(PairInfo& P, int I, int J, uint64_t dI, uint64_t sI, in
     P.kind = (a == 0.0 && b == 0.0) ? 0 : 3;
     return;
   }
+  if (ni == 1 && jsI != jsJ) {
+    // vnt[0] = +-valone(iex, jex) / 2, negated when jex > iex; the block is
+    // computed (ciout runs) even when the integral is zero
+    int ex = ie[0], jx = je[0];
+    double v = h1(one, ex, jx);
+    if (v != 0.0) {
+      if (jx > ex) v = -v;
+      v = v / 2;
+    }
+    P.v0 = v;
+    P.sop = (int8_t)((jsI % 2 == 1 ? 1 : 0) | ((jsI + jsJ) % 2 == 1 ? 2 : 0));
+    P.kind = 4;
+    return;
+  }
   if (ni == 1) {
     int ex = ie[0], jx = je[0];
     double v = h1(one, ex, jx) - g2(two, ex, jx, ex, ex);
@@ -240,11 +261,46 @@ PSOCI_HD void unpack_rec(const PairRec& r, PairInfo& P, int& J, bool& rowI) {
   P.v1 = r.v1;
 }
 
-// One determinant-pair element (generateSEFCIG, PsociHamiltonian.C:860-1118).
-// ao / bo: the alpha open shells (orbital masks) of determinant ii of the row
-// configuration and jj of the column configuration.
+// Spin-orbit determinant-pair element (socig, PsociHamiltonian.C:709-858)
+// of a kind-4 pair: +-vnt[0] when the column determinant holds at most one
+// spin-orbital the row determinant does not (nmarka, nmarkb <= 1); the sign
+// counts the configuration phases, the orbitals of the spin list(s) the
+// moved electron passes, and the Lx / Ly / Lz parity rules from jsym.
+PSOCI_HD double detval_so(const PairInfo& P, const ConfView& A, const ConfView& B, uint64_t ao, uint64_t bo) {
+  const uint64_t aI = A.d | ao, bI = A.d | (A.s & ~ao);
+  const uint64_t aJ = B.d | bo, bJ = B.d | (B.s & ~bo);
+  const int nmarka = popc(aJ & ~aI);
+  if (nmarka + popc(bJ & ~bI) > 1) return 0.0;
+  const int iex = P.e0, jex = P.f0;
+  const bool jsi_odd = P.sop & 1, sum_odd = (P.sop >> 1) & 1;
+  const uint64_t below_i = (1ull << iex) - 1ull, below_j = (1ull << jex) - 1ull;
+  const uint64_t above_i = ~((2ull << iex) - 1ull), above_j = ~((2ull << jex) - 1ull);
+  const int nai = popc(aI), naj = popc(aJ);
+  int ncount = A.ph + B.ph;
+  if (nai < naj) {
+    // i-beta j-alpha: beta orbitals of ii below iex, alpha of jj above jex
+    ncount += popc(bI & below_i) + popc(aJ & above_j);
+    if (!sum_odd || jsi_odd) ++ncount;  // Ly, or Lx with jsym_I odd
+  } else if (nai > naj) {
+    // i-alpha j-beta
+    ncount += popc(aI & above_i) + popc(bJ & below_j);
+    if (sum_odd && jsi_odd) ++ncount;
+  } else {
+    // i-alpha j-alpha or i-beta j-beta: orbitals strictly between iex and jex
+    if (nmarka == 0) ++ncount;
+    if (jsi_odd) ++ncount;
+    ncount += popc((nmarka != 0 ? aI : bI) & between(iex, jex));
+  }
+  return (ncount & 1) ? -P.v0 : P.v0;
+}
+
+// One determinant-pair element (generateSEFCIG, PsociHamiltonian.C:860-1118;
+// socig for spin-orbit pairs).  ao / bo: the alpha open shells (orbital
+// masks) of determinant ii of the row configuration and jj of the column
+// configuration.
 PSOCI_HD double detval_ao(const PairInfo& P, const ConfView& A, const ConfView& B, uint64_t ao, uint64_t bo,
                           const double* __restrict__ two) {
+  if (P.kind == 4) return detval_so(P, A, B, ao, bo);
   uint64_t aI = A.d | ao, bI = A.d | (A.s & ~ao);
   uint64_t aJ = B.d | bo, bJ = B.d | (B.s & ~bo);
   if (popc(aI) != popc(aJ)) return 0.0;  // spin projections differ

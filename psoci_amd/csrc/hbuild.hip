@@ -23,14 +23,17 @@ namespace {
 constexpr int kWave = 64;
 constexpr double kThresh = 1.0e-8;  // MIN_DOUBLE_VALUE (PsociHamiltonian.hpp:37)
 
+// the configuration's determinant patterns come from its class's family
+// (12 offsets per family)
 __device__ __forceinline__ ConfView conf_view(int64_t c, const uint64_t* dm, const uint64_t* sm, const int8_t* nopen,
-                                              const uint8_t* cph, const uint16_t* pat, const int32_t* patoff) {
+                                              const uint8_t* cph, const uint16_t* pat, const int32_t* patoff,
+                                              const uint8_t* cls) {
   ConfView v;
   v.d = dm[c];
   v.s = sm[c];
   v.nopen = nopen[c];
   v.ph = cph[c];
-  v.pat = pat + patoff[v.nopen];
+  v.pat = pat + patoff[12 * cls[c] + v.nopen];
   return v;
 }
 
@@ -78,13 +81,14 @@ __global__ void k_expand_rows(int64_t ncfg, const int64_t* __restrict__ off, int
 
 __global__ void k_pair_setup(int64_t npairs, const int32_t* __restrict__ rowcfg, const int32_t* __restrict__ colcfg,
                              const uint64_t* __restrict__ dm, const uint64_t* __restrict__ sm,
-                             const int8_t* __restrict__ nopen, int nbf, const double* __restrict__ one,
-                             const double* __restrict__ two, PairInfo* __restrict__ pairs) {
+                             const int8_t* __restrict__ nopen, const int32_t* __restrict__ jsym, int nbf,
+                             const double* __restrict__ one, const double* __restrict__ two,
+                             PairInfo* __restrict__ pairs) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npairs) return;
   int I = rowcfg[p], J = colcfg[p];
   PairInfo P;
-  pair_setup(P, I, J, dm[I], sm[I], nopen[I], dm[J], sm[J], nopen[J], nbf, one, two);
+  pair_setup(P, I, J, dm[I], sm[I], nopen[I], dm[J], sm[J], nopen[J], nbf, one, two, jsym[I], jsym[J]);
   pairs[p] = P;
 }
 
@@ -110,8 +114,9 @@ __global__ void k_block_elems(int64_t total, int64_t npairs, const int64_t* __re
                               const PairInfo* __restrict__ pairs, const uint64_t* __restrict__ dm,
                               const uint64_t* __restrict__ sm, const int8_t* __restrict__ nopen,
                               const uint8_t* __restrict__ cph, const uint16_t* __restrict__ pat,
-                              const int32_t* __restrict__ patoff, const int32_t* __restrict__ nsef, double mult,
-                              int frac, const double* __restrict__ two, double* __restrict__ staging) {
+                              const int32_t* __restrict__ patoff, const uint8_t* __restrict__ cls,
+                              const double* __restrict__ cmult, const int32_t* __restrict__ nsef, int frac,
+                              const double* __restrict__ two, double* __restrict__ staging) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= total) return;
   int64_t p = upper_bound_i64(blk_off, npairs, e) - 1;
@@ -121,9 +126,9 @@ __global__ void k_block_elems(int64_t total, int64_t npairs, const int64_t* __re
     int64_t loc = e - blk_off[p];
     int nj = nsef[P.J];
     int i = (int)(loc / nj), j = (int)(loc - (int64_t)i * nj);
-    ConfView A = conf_view(P.I, dm, sm, nopen, cph, pat, patoff);
-    ConfView B = conf_view(P.J, dm, sm, nopen, cph, pat, patoff);
-    v = sef_elem(P, A, B, i, j, mult, mult, two, frac != 0);
+    ConfView A = conf_view(P.I, dm, sm, nopen, cph, pat, patoff, cls);
+    ConfView B = conf_view(P.J, dm, sm, nopen, cph, pat, patoff, cls);
+    v = sef_elem(P, A, B, i, j, cmult[P.I], cmult[P.J], two, frac != 0);
   }
   staging[e] = v;
 }
@@ -257,8 +262,9 @@ __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restr
 __global__ void k_diag_only(int64_t ncfg, const uint64_t* __restrict__ dm, const uint64_t* __restrict__ sm,
                             const int8_t* __restrict__ nopen, const uint8_t* __restrict__ cph,
                             const uint16_t* __restrict__ pat, const int32_t* __restrict__ patoff,
-                            const int32_t* __restrict__ nsef, const int64_t* __restrict__ sefoff, int nbf, double mult,
-                            int frac, const double* __restrict__ one, const double* __restrict__ two,
+                            const uint8_t* __restrict__ cls, const double* __restrict__ cmult,
+                            const int32_t* __restrict__ nsef, const int64_t* __restrict__ sefoff, int nbf, int frac,
+                            const double* __restrict__ one, const double* __restrict__ two,
                             double* __restrict__ diag) {
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t I = (int64_t)blockIdx.x * (blockDim.x / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -268,28 +274,30 @@ __global__ void k_diag_only(int64_t ncfg, const uint64_t* __restrict__ dm, const
   P.v0 = __shfl(P.v0, 0, kWave);
   P.kind = 1;
   P.I = P.J = (int)I;
-  ConfView A = conf_view(I, dm, sm, nopen, cph, pat, patoff);
+  ConfView A = conf_view(I, dm, sm, nopen, cph, pat, patoff, cls);
+  const double mult = cmult[I];
   for (int i = lane; i < nsef[I]; i += kWave) diag[sefoff[I] + i] = sef_elem(P, A, A, i, i, mult, mult, two, frac != 0);
 }
 
 __global__ void k_single_block(PairInfo* __restrict__ Pout, int I, int J, const uint64_t* __restrict__ dm,
                                const uint64_t* __restrict__ sm, const int8_t* __restrict__ nopen,
                                const uint8_t* __restrict__ cph, const uint16_t* __restrict__ pat,
-                               const int32_t* __restrict__ patoff, const int32_t* __restrict__ nsef, int nbf,
-                               double mult, int frac, const double* __restrict__ one, const double* __restrict__ two,
-                               double* __restrict__ out) {
+                               const int32_t* __restrict__ patoff, const uint8_t* __restrict__ cls,
+                               const double* __restrict__ cmult, const int32_t* __restrict__ jsym,
+                               const int32_t* __restrict__ nsef, int nbf, int frac, const double* __restrict__ one,
+                               const double* __restrict__ two, double* __restrict__ out) {
   __shared__ PairInfo P;
   if (threadIdx.x == 0) {
-    pair_setup(P, I, J, dm[I], sm[I], nopen[I], dm[J], sm[J], nopen[J], nbf, one, two);
+    pair_setup(P, I, J, dm[I], sm[I], nopen[I], dm[J], sm[J], nopen[J], nbf, one, two, jsym[I], jsym[J]);
     *Pout = P;
   }
   __syncthreads();
-  ConfView A = conf_view(I, dm, sm, nopen, cph, pat, patoff);
-  ConfView B = conf_view(J, dm, sm, nopen, cph, pat, patoff);
+  ConfView A = conf_view(I, dm, sm, nopen, cph, pat, patoff, cls);
+  ConfView B = conf_view(J, dm, sm, nopen, cph, pat, patoff, cls);
   int ni = nsef[I], nj = nsef[J];
   for (int e = threadIdx.x; e < ni * nj; e += blockDim.x) {
     int i = e / nj, j = e - i * nj;
-    out[e] = (P.kind != 0) ? sef_elem(P, A, B, i, j, mult, mult, two, frac != 0) : 0.0;
+    out[e] = (P.kind != 0) ? sef_elem(P, A, B, i, j, cmult[I], cmult[J], two, frac != 0) : 0.0;
   }
 }
 
@@ -332,9 +340,12 @@ void upload_space(Ctx& c) {
   c.d_sefoff.upload(sp.sef_off, c.stream);
   c.d_cph.upload(sp.cphase, c.stream);
   c.d_pat.upload(sp.pat, c.stream);
-  std::vector<int32_t> po(sp.pat_off, sp.pat_off + 12);
+  std::vector<int32_t> po(sp.pat_off, sp.pat_off + 24);
   c.d_patoff.upload(po, c.stream);
   c.d_didx.upload(sp.didx, c.stream);
+  c.d_cls.upload(sp.cls, c.stream);
+  c.d_cmult.upload(sp.cmult, c.stream);
+  c.d_jsym.upload(sp.jsym, c.stream);
   c.sync();
 }
 
@@ -378,7 +389,7 @@ void lower_pairs(Ctx& c, PairLists& L) {
   PSOCI_HIP(hipGetLastError());
   L.info.alloc(L.nL);
   hipLaunchKernelGGL(k_pair_setup, dim3(blocks_for(L.nL, 256)), dim3(256), 0, c.stream, L.nL, L.rowcfg.p, L.colL.p,
-                     c.d_dm.p, c.d_sm.p, c.d_nopen.p, c.sp.nbf, c.d_one.p, c.d_two.p, L.info.p);
+                     c.d_dm.p, c.d_sm.p, c.d_nopen.p, c.d_jsym.p, c.sp.nbf, c.d_one.p, c.d_two.p, L.info.p);
   PSOCI_HIP(hipGetLastError());
 }
 
@@ -406,8 +417,8 @@ void build_h(Ctx& c) {
   DevBuf<double> staging;
   staging.alloc(nstage);
   hipLaunchKernelGGL(k_block_elems, dim3(blocks_for(nstage, 256)), dim3(256), 0, c.stream, nstage, np, boff.p,
-                     L.info.p, c.d_dm.p, c.d_sm.p, c.d_nopen.p, c.d_cph.p, c.d_pat.p, c.d_patoff.p, c.d_nsef.p,
-                     sp.mult, sp.frac ? 1 : 0, c.d_two.p, staging.p);
+                     L.info.p, c.d_dm.p, c.d_sm.p, c.d_nopen.p, c.d_cph.p, c.d_pat.p, c.d_patoff.p, c.d_cls.p,
+                     c.d_cmult.p, c.d_nsef.p, sp.frac ? 1 : 0, c.d_two.p, staging.p);
   PSOCI_HIP(hipGetLastError());
   // slots: per row of J, nl(J) lower segments, the diagonal block's upper
   // triangle, then nu(J) transposed segments
@@ -463,8 +474,8 @@ void compute_diag(Ctx& c) {
   const Space& sp = c.sp;
   c.d_diag.alloc(sp.nsef_total);
   hipLaunchKernelGGL(k_diag_only, dim3(blocks_for(sp.ncfg, 4)), dim3(256), 0, c.stream, sp.ncfg, c.d_dm.p, c.d_sm.p,
-                     c.d_nopen.p, c.d_cph.p, c.d_pat.p, c.d_patoff.p, c.d_nsef.p, c.d_sefoff.p, sp.nbf, sp.mult,
-                     sp.frac ? 1 : 0, c.d_one.p, c.d_two.p, c.d_diag.p);
+                     c.d_nopen.p, c.d_cph.p, c.d_pat.p, c.d_patoff.p, c.d_cls.p, c.d_cmult.p, c.d_nsef.p,
+                     c.d_sefoff.p, sp.nbf, sp.frac ? 1 : 0, c.d_one.p, c.d_two.p, c.d_diag.p);
   PSOCI_HIP(hipGetLastError());
   c.sync();
   c.have_diag = true;
@@ -478,8 +489,8 @@ void block_on_device(Ctx& c, int64_t I, int64_t J, double* host_out, int* status
   out.alloc((size_t)ni * nj);
   P.alloc(1);
   hipLaunchKernelGGL(k_single_block, dim3(1), dim3(256), 0, c.stream, P.p, (int)I, (int)J, c.d_dm.p, c.d_sm.p,
-                     c.d_nopen.p, c.d_cph.p, c.d_pat.p, c.d_patoff.p, c.d_nsef.p, sp.nbf, sp.mult, sp.frac ? 1 : 0,
-                     c.d_one.p, c.d_two.p, out.p);
+                     c.d_nopen.p, c.d_cph.p, c.d_pat.p, c.d_patoff.p, c.d_cls.p, c.d_cmult.p, c.d_jsym.p,
+                     c.d_nsef.p, sp.nbf, sp.frac ? 1 : 0, c.d_one.p, c.d_two.p, out.p);
   PSOCI_HIP(hipGetLastError());
   PairInfo hp;
   out.download(host_out, (size_t)ni * nj, c.stream);

@@ -95,26 +95,41 @@ void build_space(Space& sp, int nbf, int nelec, int ksym, int64_t ncfg, const ui
   sp.ksym = ksym;
   sp.ncfg = ncfg;
   sp.jsym0 = jsym[0];
-  for (int64_t c = 0; c < ncfg; ++c) {
-    PSOCI_CHECK(jsym[c] == sp.jsym0, PSOCI_ERR_UNSUPPORTED,
-                "configurations of different spatial symmetry (spin-orbit blocks) are not implemented");
-  }
   sp.frac = nelec % 2 == 1;
-  sp.odd = !sp.frac && (sp.jsym0 + 1) / 2 != (ksym + 1) / 2;
+  // determinant class of a configuration (generateDeterminants,
+  // PsociDeterminants.C:1467-1503): N odd -> fractional, alpha and beta
+  // swapped when (jsym + 1) / 2 is even; N even -> EVEN when (jsym + 1) / 2
+  // equals the state's, else ODD.  Either way a boolean selects the family.
+  auto family_flag = [&](int js) {
+    return sp.frac ? ((js + 1) / 2) % 2 == 0 : (js + 1) / 2 != (ksym + 1) / 2;
+  };
+  const bool flag0 = family_flag(sp.jsym0);
+  sp.odd = !sp.frac && flag0;
   sp.mult = (sp.jsym0 % 2 != ksym % 2) ? -1.0 : 1.0;
-  const bool swap = ((sp.jsym0 + 1) / 2) % 2 == 0;  // isgn = -1
-  for (int s = 0; s <= 10; ++s) {
-    sp.pat_off[s] = (int)sp.pat.size();
-    if (sp.frac && s % 2 == 1)
-      patterns_frac(s, swap, sp.pat);
-    else if (!sp.frac && s % 2 == 0)
-      patterns_for(s, sp.odd, sp.pat);
+  sp.jsym.assign(jsym, jsym + ncfg);
+  sp.cls.resize(ncfg);
+  sp.cmult.resize(ncfg);
+  for (int64_t c = 0; c < ncfg; ++c) {
+    if (jsym[c] != sp.jsym0) sp.mixed = true;
+    sp.cls[c] = family_flag(jsym[c]) != flag0 ? 1 : 0;
+    sp.cmult[c] = (jsym[c] % 2 != ksym % 2) ? -1.0 : 1.0;
   }
-  sp.pat_off[11] = (int)sp.pat.size();
-  sp.didx.assign(2047, (int16_t)-1);
-  for (int s = 0; s <= 10; ++s) {
-    const int nd = sp.pat_off[s + 1] - sp.pat_off[s];
-    for (int d = 0; d < nd; ++d) sp.didx[(1 << s) - 1 + sp.pat[sp.pat_off[s] + d]] = (int16_t)d;
+  sp.didx.assign(2 * 2047, (int16_t)-1);
+  for (int f = 0; f < 2; ++f) {
+    const bool flag = f == 0 ? flag0 : !flag0;
+    for (int s = 0; s <= 10; ++s) {
+      sp.pat_off[12 * f + s] = (int)sp.pat.size();
+      if (sp.frac && s % 2 == 1)
+        patterns_frac(s, flag, sp.pat);
+      else if (!sp.frac && s % 2 == 0)
+        patterns_for(s, flag, sp.pat);
+    }
+    sp.pat_off[12 * f + 11] = (int)sp.pat.size();
+    for (int s = 0; s <= 10; ++s) {
+      const int nd = sp.pat_off[12 * f + s + 1] - sp.pat_off[12 * f + s];
+      for (int d = 0; d < nd; ++d)
+        sp.didx[2047 * f + (1 << s) - 1 + sp.pat[sp.pat_off[12 * f + s] + d]] = (int16_t)d;
+    }
   }
   sp.dm.resize(ncfg);
   sp.sm.resize(ncfg);

@@ -30,7 +30,7 @@ static ConfView view(const Space& sp, int64_t c) {
   v.s = sp.sm[c];
   v.nopen = sp.nopen[c];
   v.ph = sp.cphase[c];
-  v.pat = sp.pat.data() + sp.pat_off[v.nopen];
+  v.pat = sp.pat.data() + sp.pat_off[12 * sp.cls[c] + v.nopen];
   return v;
 }
 
@@ -38,13 +38,23 @@ static ConfView view(const Space& sp, int64_t c) {
 int hc_block(void* h, const double* one, const double* two, int64_t I, int64_t J, double* out) {
   const Space& sp = *static_cast<Space*>(h);
   PairInfo P;
-  pair_setup(P, (int)I, (int)J, sp.dm[I], sp.sm[I], sp.nopen[I], sp.dm[J], sp.sm[J], sp.nopen[J], sp.nbf, one, two);
+  pair_setup(P, (int)I, (int)J, sp.dm[I], sp.sm[I], sp.nopen[I], sp.dm[J], sp.sm[J], sp.nopen[J], sp.nbf, one, two,
+             sp.jsym[I], sp.jsym[J]);
   ConfView A = view(sp, I), B = view(sp, J);
   int ni = sp.nsef[I], nj = sp.nsef[J];
   for (int i = 0; i < ni; ++i)
     for (int j = 0; j < nj; ++j)
-      out[i * nj + j] = P.kind ? sef_elem(P, A, B, i, j, sp.mult, sp.mult, two, sp.frac) : 0.0;
+      out[i * nj + j] = P.kind ? sef_elem(P, A, B, i, j, sp.cmult[I], sp.cmult[J], two, sp.frac) : 0.0;
   return P.kind ? 1 : 0;
+}
+
+// block kind of (I, J) (4: spin-orbit single)
+int hc_kind(void* h, const double* one, const double* two, int64_t I, int64_t J) {
+  const Space& sp = *static_cast<Space*>(h);
+  PairInfo P;
+  pair_setup(P, (int)I, (int)J, sp.dm[I], sp.sm[I], sp.nopen[I], sp.dm[J], sp.sm[J], sp.nopen[J], sp.nbf, one, two,
+             sp.jsym[I], sp.jsym[J]);
+  return P.kind;
 }
 
 // block (I, J) through the three-index-table off-diagonal path (sef_elem_x);

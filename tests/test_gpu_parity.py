@@ -121,12 +121,15 @@ def test_error_codes_not_faults(gpu_fci6):
     with pytest.raises(api.PsociError) as e:
         H.block(10**6, 0)
     assert e.value.code == api.ERR_ARG
-    # configurations of different spatial symmetry (spin-orbit blocks) are outside the model
+    # configurations of different spatial symmetry (spin-orbit blocks) run
+    # stored only: the direct sigma refuses them
     mixed = inputs.fci_configs(4, 4)
     mixed.jsym[1] = 2
+    Hm = api.Hamiltonian(mixed, inputs.make_integrals(4, 1))
     with pytest.raises(api.PsociError) as e:
-        api.Hamiltonian(mixed, inputs.make_integrals(4, 1))
+        Hm.sigma(np.ones(Hm.n), api.DIRECT)
     assert e.value.code == api.ERR_UNSUPPORTED
+    Hm.close()
     cfg, ints, _ = inputs.workload("fci6")
     H2 = api.Hamiltonian(cfg, ints)
     with pytest.raises(api.PsociError) as e:

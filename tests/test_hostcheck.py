@@ -35,6 +35,7 @@ def hc():
     L.hc_space_new.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, P, P]
     L.hc_space_free.argtypes = [P]
     L.hc_block.argtypes = [P, P, P, ctypes.c_int64, ctypes.c_int64, P]
+    L.hc_kind.argtypes = [P, P, P, ctypes.c_int64, ctypes.c_int64]
     L.hc_block_x.argtypes = [P, P, P, ctypes.c_int64, ctypes.c_int64, P]
     L.hc_connected.argtypes = [P, ctypes.c_int64, ctypes.c_int64]
     L.hc_row.argtypes = [P, P, P, ctypes.c_int64, ctypes.c_int, P, P, ctypes.c_int]
@@ -45,8 +46,10 @@ def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def _compare(L, cfg, ints, rows):
+def _compare(L, cfg, ints, rows, both=False):
+    """Blocks (I, J) for J <= I (and J > I too with both=True) bit for bit."""
     sp = oracle.OracleSpace(cfg, ints)
+    mixed = len(set(cfg.jsym.tolist())) > 1
     h = L.hc_space_new(cfg.nbf, cfg.nelec, cfg.ksym, cfg.ncfg, _p(cfg.occ), _p(cfg.jsym))
     assert h
     nblocks = 0
@@ -54,9 +57,10 @@ def _compare(L, cfg, ints, rows):
         for I in rows:
             I = int(I)
             jl = set(sp.jlist(I).tolist())
-            for J in range(I + 1):
+            for J in range(cfg.ncfg if both else I + 1):
                 con = L.hc_connected(h, I, J)
-                assert bool(con) == (J in jl), (I, J)
+                if J <= I:
+                    assert bool(con) == (J in jl), (I, J)
                 if not con:
                     continue
                 st, ref = sp.block(I, J)
@@ -64,8 +68,9 @@ def _compare(L, cfg, ints, rows):
                 st2 = L.hc_block(h, _p(ints.valone), _p(ints.valtwo), I, J, _p(out))
                 assert st == st2, (I, J)
                 assert np.array_equal(ref, out), (I, J, np.abs(ref - out).max())
-                if I != J:
+                if I != J and not mixed:
                     # the three-index-table path used by the direct sigma
+                    # (one jsym: spaces with spin-orbit blocks run stored)
                     out_x = np.zeros_like(ref)
                     st3 = L.hc_block_x(h, _p(ints.valone), _p(ints.valtwo), I, J, _p(out_x))
                     assert st3 == st, (I, J)
@@ -229,3 +234,26 @@ def test_nine_and_ten_open_shells(hc, nelec, ksym):
     rows = list(np.where(nopen == nopen.max())[0][:2]) + list(np.random.default_rng(ksym).choice(cfg.ncfg, 4))
     _compare(hc, cfg, ints, rows)
     _rows_agree(hc, cfg, ints, rows)
+
+
+def _mixed_space(nbf, nelec, jsyms, seed=5):
+    """Every FCI configuration once per jsym value (distinct configurations
+    of the reference's list), so configurations of different jsym are one
+    electron apart: spin-orbit blocks."""
+    base = inputs.fci_configs(nbf, nelec)
+    occ = np.concatenate([base.occ] * len(jsyms))
+    jsym = np.concatenate([np.full(base.ncfg, j, np.int32) for j in jsyms])
+    cfg = inputs.Configs(nbf=nbf, nelec=nelec, ksym=1, occ=occ, jsym=jsym, title="mixed jsym")
+    return cfg, inputs.make_integrals(nbf, seed)
+
+
+@pytest.mark.parametrize("nbf,nelec,jsyms", [(4, 4, (1, 2)), (4, 4, (1, 3)), (4, 4, (2, 3, 4)), (5, 4, (1, 3)),
+                                             (4, 3, (1, 3)), (5, 5, (2, 3))])
+def test_spin_orbit_blocks_bit_identical(hc, nbf, nelec, jsyms):
+    """Spaces with configurations of several jsym (EVEN and ODD classes,
+    both SEF signs; an odd electron count: both fractional orientations):
+    every block, in both orientations, spin-orbit singles included, equals
+    the oracle's restatement of generateSOCIblocks/socig bit for bit."""
+    cfg, ints = _mixed_space(nbf, nelec, jsyms)
+    n = _compare(hc, cfg, ints, range(cfg.ncfg), both=True)
+    assert n > 100
