@@ -179,6 +179,27 @@ int psoci_get_h(psoci_ctx* ctx, int64_t* rowptr, int32_t* col, double* val, doub
  * stands in for psoci_build_h: *nnz_lower as there. */
 int psoci_h_save(psoci_ctx* ctx, const char* path);
 int psoci_h_load(psoci_ctx* ctx, const char* path, int64_t* nnz_lower);
+/* Transition Hamiltonian between the context's configuration list (rows)
+ * and a second list of the same orbitals, electrons and ksym (jsym2 per
+ * configuration; any mix): constructGlobalTransitionHamiltonian2WAY
+ * (PsociGAhamiltonian.C:2089-2287, driver.Transition.C) with packSefsef
+ * (:4077-4136).  For configuration I of the context's list and every J <= I
+ * of the second (positions in the two lists) the block of the two
+ * configurations, through the diagonal path when I == J whatever the
+ * configurations; rows are the context's SEFs, columns J's SEFs placed at
+ * the context list's entry point of J (the reference computes the entry
+ * points from the first list only); |h| >= 1e-8 kept, and for I == J the
+ * lower triangle with the diagonal always kept.  Refused
+ * (PSOCI_ERR_UNSUPPORTED) where the reference reads out of bounds: past a
+ * one-entry diagonal vnt (an equal-index pair a double excitation apart
+ * whose first configuration has fewer than two open shells), or past an
+ * equal-index block whose second configuration has fewer SEFs.  psoci_build_transition
+ * returns the element count; psoci_get_transition copies the rows
+ * (rowptr[n+1], n = the context's SEF count), columns and values to host
+ * arrays (any may be NULL).  The context's stored H is left as it is. */
+int psoci_build_transition(psoci_ctx* ctx, int64_t ncfg2, const uint8_t* occ2, const int32_t* jsym2, int64_t* nnz);
+int psoci_get_transition(psoci_ctx* ctx, int64_t* rowptr, int32_t* col, double* val);
+
 /* Sigma Y = H X for k vectors: matrixVectorProductsSplitChunkMultipleJchunks
  * (PsociGAhamiltonian.C:6921-7269; mode STORED) and the direct
  * ...DirectFetch path (:7549-7823; mode DIRECT, elements regenerated on the

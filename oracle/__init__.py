@@ -25,6 +25,14 @@ Parity status of each oracle function (what pins it):
                                   (its iteration trajectory: parity unpinned)
   * CIDEN 1-RDM                -- brute-force <Psi|E_pq|Psi> from the
                                   second-quantised determinant expansion
+  * spin-orbit blocks          -- Kramers pairs of odd-electron spectra,
+                                  orientation symmetry, zero integrals, the
+                                  single-jsym reduction, planted mistakes;
+                                  the window count and phases inside socig:
+                                  parity unpinned (tests/test_spin_orbit.py)
+  * transition Hamiltonian     -- equals the stored H when both lists are the
+                                  same; the index-equality and entry-point
+                                  rules by hand-built cases (tests/test_oracle.py)
   * CI energy contributions    -- closed forms: a 2x2 eigenproblem (each
                                   configuration's contribution is E minus the
                                   other's diagonal), the c^2 == 1 reset, zero
@@ -79,6 +87,9 @@ def lib():
         L.orc_davidson.restype = ctypes.c_int
         L.orc_davidson.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, P, P, P, P]
         L.orc_rdm1.argtypes = [P, P, P]
+        L.orc_transition_h.restype = ctypes.c_long
+        L.orc_transition_h.argtypes = [P, P, P, P, P, ctypes.c_long]
+        L.orc_set_so_mutation.argtypes = [ctypes.c_int]
         L.orc_energy_contrib.argtypes = [ctypes.c_long, P, ctypes.c_long, ctypes.c_int, P, P, P, P]
         L.orc_direct_rows.restype = ctypes.c_int
         L.orc_direct_rows.argtypes = [P, ctypes.c_long, P, ctypes.c_int, P, P, ctypes.c_int]
@@ -200,6 +211,20 @@ class OracleSpace:
         if rc != 0:
             raise RuntimeError(f"oracle direct rows failed ({rc})")
         return Y
+
+    def transition_h(self, other: "OracleSpace"):
+        """Transition Hamiltonian with self's list as rows and other's as
+        columns (constructGlobalTransitionHamiltonian2WAY + packSefsef):
+        (rowptr, col, val) over self's SEFs.  self holds the integrals."""
+        L = lib()
+        nnz = L.orc_transition_h(self._h, other._h, None, None, None, 0)
+        if nnz < 0:
+            raise RuntimeError(f"oracle transition build failed ({nnz})")
+        rowptr = np.zeros(self.n + 1, np.int64)
+        col = np.zeros(nnz, np.int32)
+        val = np.zeros(nnz, np.float64)
+        L.orc_transition_h(self._h, other._h, _p(rowptr), _p(col), _p(val), nnz)
+        return rowptr, col, val
 
     def rdm1(self, c: np.ndarray):
         c = np.ascontiguousarray(c, np.float64)

@@ -26,6 +26,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <set>
+#include <stdexcept>
 #include <string>
 #include <vector>
 #ifdef _OPENMP
@@ -746,6 +747,10 @@ struct Ham {
               }
             }
           }
+          // a diagonal-path block (equal indices) of two configurations a
+          // double excitation apart reads vnt[1], which the diagonal vnt
+          // holds only with two or more open shells (the transition build)
+          if (vnt.size() < 2) throw std::runtime_error("vnt[1] read past a one-entry diagonal vnt");
           detdet[index] = vnt[0] - vnt[1];
           if (ncount % 2 == 1) detdet[index] = -detdet[index];
         }
@@ -1322,6 +1327,73 @@ long orc_build_h(void* h, int nthreads) {
       std::copy(ico[I][i].begin(), ico[I][i].end(), sp->col.begin() + p);
       if (!cim[I][i].empty()) sp->diag[r] = cim[I][i].back();
     }
+  return nnz;
+}
+
+// Transition Hamiltonian between two configuration lists of one orbital
+// basis: constructGlobalTransitionHamiltonian2WAY (PsociGAhamiltonian.C:
+// 2089-2287) with packSefsef (:4077-4136).  Rows: the SEFs of list 1; for
+// each configuration I of list 1 and each J <= I of list 2 (positions in the
+// two lists), generateSOCIblocks(list1[I], list2[J]) (its diagonal path
+// whenever the 1-based indices agree, I == J, whatever the configurations)
+// and packSefsef with list 1's SEF entry points for both the row and the
+// column (the driver computes them from list 1 only, driver.Transition.C:
+// 579): columns sef_off1[J] + j, the lower triangle j <= i when I == J, and
+// |h| >= 1e-8 or the (I == J, i == j) element.  h1 and h2 share integrals
+// (h1's are used).  Returns nnz, or < 0 on error (-3: a diagonal-path block
+// reading vnt[1] of a configuration with fewer than two open shells; -4: an
+// equal-index block whose list-2 configuration has fewer SEFs, where
+// packSefsef reads past the block).
+long orc_transition_h(void* h1, void* h2, long* rowptr, int* col, double* val, long cap) {
+  Space* s1 = static_cast<Space*>(h1);
+  Space* s2 = static_cast<Space*>(h2);
+  if (!s1->have_ints || s1->nbf != s2->nbf || s1->nelec != s2->nelec || s1->ksym != s2->ksym) return -1;
+  Ham hm = make_ham(*s1);
+  long n1 = s1->nsef_total, nnz = 0;
+  std::vector<std::vector<double>> vrow(n1);
+  std::vector<std::vector<int>> crow(n1);
+  try {
+    for (long I = 0; I < s1->ncfg; ++I) {
+      Joutfg ti;
+      const Joutfg& ic = conf_of(*s1, I, ti);
+      const long jend = std::min(I, s2->ncfg - 1);
+      for (long J = 0; J <= jend; ++J) {
+        Joutfg tj;
+        const Joutfg& jc = conf_of(*s2, J, tj);
+        std::vector<double> sefsef;
+        std::string err;
+        int st = hm.block(ic, jc, sefsef, err);
+        if (st < 0) return -2;
+        if (!st) continue;
+        // packSefsef's lower triangle of an equal-index block reads
+        // sefsef[i * nsefj + j] for j <= i: past the block when nsefj < nsefi
+        if (I == J && jc.nsefi < ic.nsefi) return -4;
+        long isef = s1->sef_off[I], jsef = s1->sef_off[J];
+        for (int i = 0; i < ic.nsefi; ++i) {
+          int jmax = (isef == jsef) ? i : jc.nsefi - 1;
+          for (int j = 0; j <= jmax; ++j) {
+            double v = sefsef[i * jc.nsefi + j];
+            if (std::fabs(v) >= 1.0e-8 || (isef == jsef && i == j)) {
+              vrow[isef + i].push_back(v);
+              crow[isef + i].push_back((int)(j + jsef));
+            }
+          }
+        }
+      }
+    }
+  } catch (const std::runtime_error&) {
+    return -3;
+  }
+  for (long r = 0; r < n1; ++r) {
+    if (rowptr) rowptr[r] = nnz;
+    for (size_t t = 0; t < vrow[r].size(); ++t, ++nnz) {
+      if (nnz < cap) {
+        if (col) col[nnz] = crow[r][t];
+        if (val) val[nnz] = vrow[r][t];
+      }
+    }
+  }
+  if (rowptr) rowptr[n1] = nnz;
   return nnz;
 }
 

@@ -99,6 +99,8 @@ def load(path: str | None = None):
         "psoci_device_ptr": (I, [V, I, ctypes.POINTER(V)]),
         "psoci_project": (I, [V, I64, I, I, V, I, V, I, V]),
         "psoci_combine": (I, [V, I64, I, I, V, I, V, D, V, I]),
+        "psoci_build_transition": (I, [V, I64, V, V, V]),
+        "psoci_get_transition": (I, [V, V, V, V]),
         "psoci_direct_info": (I, [V, V, V]),
         "psoci_direct_det_pairs": (I, [V, V]),
         "psoci_direct_schedule": (I, [V, V, V, V]),
@@ -244,6 +246,20 @@ class Hamiltonian:
         self._check(self.L.psoci_build_h(self._h, ctypes.byref(nnz)))
         self.nnz_lower = nnz.value
         return nnz.value
+
+    def transition(self, cfg2):
+        """Transition Hamiltonian with this list as rows and cfg2's as
+        columns (psoci_build_transition + psoci_get_transition): rowptr,
+        col, val over this list's SEFs."""
+        occ = np.ascontiguousarray(cfg2.occ, np.uint8)
+        js = np.ascontiguousarray(cfg2.jsym, np.int32)
+        nnz = ctypes.c_int64()
+        self._check(self.L.psoci_build_transition(self._h, cfg2.ncfg, _p(occ), _p(js), ctypes.byref(nnz)))
+        rowptr = np.zeros(self.n + 1, np.int64)
+        col = np.zeros(nnz.value, np.int32)
+        val = np.zeros(nnz.value, np.float64)
+        self._check(self.L.psoci_get_transition(self._h, _p(rowptr), _p(col), _p(val)))
+        return rowptr, col, val
 
     def save_h(self, path: str):
         """Checkpoint the stored Hamiltonian (the reference's DRA dump)."""

@@ -98,7 +98,7 @@ int psoci_set_integrals(psoci_ctx* ctx, int nbf, const double* valone, const dou
     c.nbf_ints = nbf;
     c.ecore = ecore;
     c.have_ints = true;
-    c.have_h = c.have_diag = c.have_pairs = false;
+    c.have_h = c.have_diag = c.have_pairs = c.have_trans = false;
     return PSOCI_OK;
   });
 }
@@ -115,7 +115,7 @@ int psoci_set_configs(psoci_ctx* ctx, int nbf, int nelec, int ksym, int64_t ncfg
     c.space_hash = psoci::hash_bytes(jsym, ncfg * sizeof(int32_t),
                                      psoci::hash_bytes(occ, ncfg * nbf, psoci::hash_bytes(dims, sizeof(dims))));
     c.have_space = true;
-    c.have_h = c.have_diag = c.have_pairs = false;
+    c.have_h = c.have_diag = c.have_pairs = c.have_trans = false;
     return PSOCI_OK;
   });
 }
@@ -310,6 +310,30 @@ int psoci_sigma_noalloc(psoci_ctx* ctx, int mode, int prec, int k, const void* X
                     "24 for the direct sigma) or fewer columns");
     if (mode == PSOCI_SIGMA_STORED) psoci::sigma_stored(c, prec, k, X, k, Y, k, st);
     else psoci::sigma_direct(c, prec, k, X, k, Y, k, st);
+    c.sync();
+    return PSOCI_OK;
+  });
+}
+
+int psoci_build_transition(psoci_ctx* ctx, int64_t ncfg2, const uint8_t* occ2, const int32_t* jsym2, int64_t* nnz) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
+    PSOCI_CHECK(ncfg2 >= 1 && occ2 && jsym2, PSOCI_ERR_ARG, "bad second configuration list");
+    psoci::Space s2;
+    psoci::build_space(s2, c.sp.nbf, c.sp.nelec, c.sp.ksym, ncfg2, occ2, jsym2);
+    c.have_trans = false;
+    psoci::build_transition(c, s2, nnz);
+    return PSOCI_OK;
+  });
+}
+
+int psoci_get_transition(psoci_ctx* ctx, int64_t* rowptr, int32_t* col, double* val) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_trans, PSOCI_ERR_STATE, "no transition Hamiltonian: psoci_build_transition first");
+    const int64_t n = c.sp.nsef_total;
+    if (rowptr) c.d_trow.download(rowptr, n + 1, c.stream);
+    if (col) c.d_tcol.download(col, c.nnz_trans, c.stream);
+    if (val) c.d_tval.download(val, c.nnz_trans, c.stream);
     c.sync();
     return PSOCI_OK;
   });

@@ -81,6 +81,7 @@ struct Space {
   // each configuration has its own determinant class (pattern family 0 =
   // that of configuration 0, 1 = the other) and SEF combination sign
   bool mixed = false;
+  bool flag0 = false;  // class flag of configuration 0 (class_flag)
   std::vector<int32_t> jsym;
   std::vector<uint8_t> cls;
   std::vector<double> cmult;
@@ -101,6 +102,9 @@ struct Space {
 
 // Builds Space from a configuration list; throws Error on unsupported input.
 void build_space(Space& sp, int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, const int32_t* jsym);
+// determinant class of a jsym as a flag: N odd -> alpha and beta swapped
+// ((jsym + 1) / 2 even), N even -> ODD ((jsym + 1) / 2 != (ksym + 1) / 2)
+bool class_flag(int nelec, int ksym, int jsym);
 
 // A configuration file read by the reference's rules (configs.cpp): file
 // order, or sorted by (jsym, occupation) with duplicates dropped.  Throws

@@ -79,6 +79,14 @@ struct Ctx {
   DevBuf<uint64_t> d_detao;   // ndet_total
   DevBuf<int16_t> d_didx;     // determinant index by open-shell pattern (Space::didx)
 
+  // transition Hamiltonian (rows: this list's SEFs; columns: a second
+  // list's SEFs placed at this list's entry points), lower by index
+  bool have_trans = false;
+  int64_t nnz_trans = 0;
+  DevBuf<int64_t> d_trow;
+  DevBuf<int32_t> d_tcol;
+  DevBuf<double> d_tval;
+
   // scratch
   DevBuf<char> d_pad;  // zero-padded column staging for the direct sigma
   void* scan_tmp = nullptr;
@@ -118,6 +126,7 @@ void energy_contrib(Ctx& c, int k, const double* dC, const double* d_evals, doub
 void save_h(Ctx& c, const char* path);  // stored-Hamiltonian checkpoint (restart.cpp)
 void load_h(Ctx& c, const char* path);
 void rdm_grad(Ctx& c, int k, const double* dC, const double* w, double* d_g1, double* d_g2);
+void build_transition(Ctx& c, const Space& list2, int64_t* nnz);  // transition.hip
 
 struct DavParams;
 int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, psoci_dav_stats* st);

@@ -85,6 +85,10 @@ static void patterns_frac(int s, bool swap, std::vector<uint16_t>& out) {
   }
 }
 
+bool class_flag(int nelec, int ksym, int js) {
+  return nelec % 2 == 1 ? ((js + 1) / 2) % 2 == 0 : (js + 1) / 2 != (ksym + 1) / 2;
+}
+
 void build_space(Space& sp, int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, const int32_t* jsym) {
   PSOCI_CHECK(nbf > 0 && nbf <= 64, PSOCI_ERR_UNSUPPORTED, "nbf must be in 1..64");
   PSOCI_CHECK(ncfg > 0, PSOCI_ERR_ARG, "empty configuration list");
@@ -100,10 +104,9 @@ void build_space(Space& sp, int nbf, int nelec, int ksym, int64_t ncfg, const ui
   // PsociDeterminants.C:1467-1503): N odd -> fractional, alpha and beta
   // swapped when (jsym + 1) / 2 is even; N even -> EVEN when (jsym + 1) / 2
   // equals the state's, else ODD.  Either way a boolean selects the family.
-  auto family_flag = [&](int js) {
-    return sp.frac ? ((js + 1) / 2) % 2 == 0 : (js + 1) / 2 != (ksym + 1) / 2;
-  };
+  auto family_flag = [&](int js) { return class_flag(nelec, ksym, js); };
   const bool flag0 = family_flag(sp.jsym0);
+  sp.flag0 = flag0;
   sp.odd = !sp.frac && flag0;
   sp.mult = (sp.jsym0 % 2 != ksym % 2) ? -1.0 : 1.0;
   sp.jsym.assign(jsym, jsym + ncfg);

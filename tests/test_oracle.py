@@ -362,3 +362,65 @@ def test_rdm1_matches_bruteforce(fci6, oracle_fci6):
         for q in range(p + 1):
             assert abs(D[inputs.pair_index(p, q)] - gam[p, q]) < 1e-12
     assert abs(sum(D[inputs.pair_index(p, p)] for p in range(cfg.nbf)) - cfg.nelec) < 1e-10
+
+
+def _sub(cfg, occ, jsym):
+    return inputs.Configs(nbf=cfg.nbf, nelec=cfg.nelec, ksym=cfg.ksym, occ=np.ascontiguousarray(occ),
+                          jsym=np.ascontiguousarray(jsym, np.int32), title="list 2")
+
+
+def test_transition_of_a_list_with_itself_is_the_stored_h():
+    """constructGlobalTransitionHamiltonian2WAY with both lists the same is
+    the ordinary build: same loops, blocks and packing (fci6 and a space with
+    spin-orbit blocks)."""
+    from tests.test_spin_orbit import mixed_space
+    for cfg, ints in [inputs.workload("fci6")[:2], mixed_space(4, 4, (1, 3))]:
+        a = oracle.OracleSpace(cfg, ints)
+        a.build_h()
+        r, c, v = a.transition_h(oracle.OracleSpace(cfg, ints))
+        assert np.array_equal(r, a.rowptr) and np.array_equal(c, a.col) and np.array_equal(v, a.val)
+
+
+def test_transition_rules_against_blocks():
+    """Rows of list 1 against list 2 = list 1 shifted by five positions:
+    for I != J the kept elements of the ordinary block of the two
+    configurations (J <= I by position, the threshold, columns at list 1's
+    entry point of J); equal positions hold different configurations here,
+    so only their diagonal element is checked (kept whenever the block is
+    computed: the two configurations within two moved electrons and four
+    open shells, generateSOCIblocks' early returns)."""
+    cfg, ints, _ = inputs.workload("fci6")
+    c2 = _sub(cfg, cfg.occ[5:], cfg.jsym[5:])
+    a = oracle.OracleSpace(cfg, ints)
+    r, c, v = a.transition_h(oracle.OracleSpace(c2, ints))
+    both = oracle.OracleSpace(_sub(cfg, np.concatenate([cfg.occ, c2.occ]), np.concatenate([cfg.jsym, c2.jsym])), ints)
+    n1 = cfg.ncfg
+    off = a.sef_off
+    for I in range(0, n1, 3):
+        for i in range(int(a.nsef[I])):
+            row = r[off[I] + i], r[off[I] + i + 1]
+            cols, vals = c[row[0]:row[1]], v[row[0]:row[1]]
+            want_c, want_v = [], []
+            for J in range(min(I, c2.ncfg - 1) + 1):
+                if J == I:
+                    continue
+                st, b = both.block(I, n1 + J)
+                if not st:
+                    continue
+                for j in range(b.shape[1]):
+                    if abs(b[i, j]) >= 1e-8:
+                        want_c.append(off[J] + j)
+                        want_v.append(b[i, j])
+            # the J < I segments come first, in J order; the I == J segment
+            # (when computed) last, ending with the always-kept diagonal
+            o1, o2 = cfg.occ[I].astype(int), c2.occ[I].astype(int) if I < c2.ncfg else None
+            diag_seg = o2 is not None and np.maximum(o1 - o2, 0).sum() <= 2 and \
+                abs(int((o1 == 1).sum()) - int((o2 == 1).sum())) <= 4
+            k = len(want_c)
+            assert cols[:k].tolist() == want_c, (I, i)
+            assert np.array_equal(vals[:k], np.array(want_v)), (I, i)
+            if diag_seg:
+                assert len(cols) > k and cols[-1] == off[I] + i
+                assert all(off[I] <= x <= off[I] + i for x in cols[k:])
+            else:
+                assert len(cols) == k
