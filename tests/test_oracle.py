@@ -382,7 +382,9 @@ def test_transition_of_a_list_with_itself_is_the_stored_h():
 
 
 def test_transition_rules_against_blocks():
-    """Rows of list 1 against list 2 = list 1 shifted by five positions:
+    """Rows of list 1 against list 2 = list 1 with its configurations of two
+    or more open shells shuffled within their open-shell count (the inputs
+    for which the reference's build stays within its arrays), cut to 120:
     for I != J the kept elements of the ordinary block of the two
     configurations (J <= I by position, the threshold, columns at list 1's
     entry point of J); equal positions hold different configurations here,
@@ -390,7 +392,9 @@ def test_transition_rules_against_blocks():
     computed: the two configurations within two moved electrons and four
     open shells, generateSOCIblocks' early returns)."""
     cfg, ints, _ = inputs.workload("fci6")
-    c2 = _sub(cfg, cfg.occ[5:], cfg.jsym[5:])
+    from tests.test_gpu_transition import class_permutation
+    p = class_permutation(cfg, 1)[:120]
+    c2 = _sub(cfg, cfg.occ[p], cfg.jsym[p])
     a = oracle.OracleSpace(cfg, ints)
     r, c, v = a.transition_h(oracle.OracleSpace(c2, ints))
     both = oracle.OracleSpace(_sub(cfg, np.concatenate([cfg.occ, c2.occ]), np.concatenate([cfg.jsym, c2.jsym])), ints)
