@@ -283,6 +283,10 @@ typedef struct {
                             (PsociGAbasis.C:1860-1920); 1: the same with Olsen's correction
                             against the Ritz vector x: t = f (r - eps x),
                             eps = (x . f r) / (x . f x).  Both skip |D_ii - lambda| <= 1e-3 */
+  int graph;          /* 0: with the stored sigma, each step's device work (sigma of the new
+                            block and its projection V^T H V) runs as a captured hipGraph per
+                            block shape, replayed; 1: always launched eagerly.  The results
+                            are the same bit for bit */
 } psoci_dav_params;
 
 typedef struct {
@@ -294,6 +298,8 @@ typedef struct {
   double seconds;     /* wall time of the solve */
   double sigma_seconds;
   double max_rnorm;
+  double eig_seconds;  /* host time in the subspace eigensolves (Rayleigh-Ritz, orthonormalisation) */
+  int graph_launches;  /* captured steps replayed */
 } psoci_dav_stats;
 
 /* Block Davidson on the GPU: replaces the production driver loop

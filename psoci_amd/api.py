@@ -41,13 +41,14 @@ class DavParams(ctypes.Structure):
     _fields_ = [("nroots", ctypes.c_int), ("prec", ctypes.c_int), ("sigma_mode", ctypes.c_int),
                 ("max_iter", ctypes.c_int), ("max_subspace", ctypes.c_int), ("guess", ctypes.c_int),
                 ("tol", ctypes.c_double), ("switch_tol", ctypes.c_double), ("extra_roots", ctypes.c_int),
-                ("precond", ctypes.c_int)]
+                ("precond", ctypes.c_int), ("graph", ctypes.c_int)]
 
 
 class DavStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("sigma_calls", ctypes.c_int), ("sigma_vectors", ctypes.c_int),
                 ("switch_iter", ctypes.c_int), ("converged", ctypes.c_int), ("seconds", ctypes.c_double),
-                ("sigma_seconds", ctypes.c_double), ("max_rnorm", ctypes.c_double)]
+                ("sigma_seconds", ctypes.c_double), ("max_rnorm", ctypes.c_double), ("eig_seconds", ctypes.c_double),
+                ("graph_launches", ctypes.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -311,18 +312,19 @@ class Hamiltonian:
     def davidson(self, nroots: int, prec: int = FP64, mode: int = STORED, tol: float = 1e-6,
                  switch_tol: float = 1e-4, max_iter: int = 200, max_subspace: int = 0, guess: int = 0,
                  want_vectors: bool = True, guess_vectors=None, extra_roots: int = 0,
-                 restart: str | None = None, precond: int = 0):
+                 restart: str | None = None, precond: int = 0, graph: int = 0):
         """Block Davidson for the nroots lowest roots.  guess: 0 lowest diagonal,
         1 first SEFs (the reference's), 2 guess_vectors ([n][nroots]);
         restart: a CI-vector file (civec_write) read as guess_vectors, missing
         roots completed with unit vectors (the reference's --vector_restart);
         extra_roots guard roots ride along in the block (not tested; with
         MIXED only in the FP32 phase); precond 0: the reference's diagonal
-        preconditioner, 1: the same with Olsen's correction."""
+        preconditioner, 1: the same with Olsen's correction; graph 0:
+        stored-sigma steps as captured hipGraphs, 1: eager launches."""
         if restart is not None:
             guess_vectors, _ = civec_read(restart, self.n, nroots)
             guess = 2
-        p = DavParams(nroots, prec, mode, max_iter, max_subspace, guess, tol, switch_tol, extra_roots, precond)
+        p = DavParams(nroots, prec, mode, max_iter, max_subspace, guess, tol, switch_tol, extra_roots, precond, graph)
         ev = np.zeros(nroots)
         vec = np.zeros((self.n, nroots)) if want_vectors else None
         if guess == 2:

@@ -20,6 +20,7 @@ int main(void) {
   F(psoci_dav_params, switch_tol);
   F(psoci_dav_params, extra_roots);
   F(psoci_dav_params, precond);
+  F(psoci_dav_params, graph);
   printf("psoci_dav_stats %zu\n", sizeof(psoci_dav_stats));
   F(psoci_dav_stats, iterations);
   F(psoci_dav_stats, sigma_calls);
@@ -29,5 +30,7 @@ int main(void) {
   F(psoci_dav_stats, seconds);
   F(psoci_dav_stats, sigma_seconds);
   F(psoci_dav_stats, max_rnorm);
+  F(psoci_dav_stats, eig_seconds);
+  F(psoci_dav_stats, graph_launches);
   return 0;
 }

@@ -265,16 +265,39 @@ inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per
 
 }  // namespace
 
-void Linalg::gemm_tn(int64_t n, int m, int b, const double* A, int lda, const double* B, int ldb, double* out_dev) {
-  if (m == 0 || b == 0) return;
-  // about 4096 wavefronts in flight: chunks of a multiple of 16 rows
+namespace {
+
+// about 4096 wavefronts in flight: chunks of a multiple of 16 rows
+void tn_chunks(int64_t n, int m, int b, int64_t& rows_per_chunk, int64_t& nchunks) {
   const int tiles = ((m + 15) / 16) * ((b + 15) / 16);
   int64_t want = std::max<int64_t>(1, 4096 / tiles);
   want = std::min<int64_t>(want, 65535);
-  int64_t rows_per_chunk = (n + want - 1) / want;
+  rows_per_chunk = (n + want - 1) / want;
   rows_per_chunk = std::max<int64_t>(64, (rows_per_chunk + 15) / 16 * 16);
-  int64_t nchunks = (n + rows_per_chunk - 1) / rows_per_chunk;
+  nchunks = (n + rows_per_chunk - 1) / rows_per_chunk;
+}
+
+}  // namespace
+
+size_t Linalg::tn_part_size(int64_t n, int m, int b) {
+  int64_t rpc, nch;
+  tn_chunks(n, m, b, rpc, nch);
+  return (size_t)nch * m * b;
+}
+
+void Linalg::reserve_tn(int64_t n, int mmax, int bmax) {
+  size_t need = 0;
+  for (int m = 1; m <= mmax; ++m)
+    for (int b = 1; b <= bmax; ++b) need = std::max(need, tn_part_size(n, m, b));
+  part.reserve(need);
+}
+
+void Linalg::gemm_tn(int64_t n, int m, int b, const double* A, int lda, const double* B, int ldb, double* out_dev) {
+  if (m == 0 || b == 0) return;
+  int64_t rows_per_chunk, nchunks;
+  tn_chunks(n, m, b, rows_per_chunk, nchunks);
   part.reserve((size_t)nchunks * m * b);
+  const int tiles = ((m + 15) / 16) * ((b + 15) / 16);
   dim3 grid(nblk(tiles, 4), (unsigned)nchunks);
   hipLaunchKernelGGL(k_gemm_tn_mfma, grid, dim3(256), 0, stream, n, m, b, A, lda, B, ldb, rows_per_chunk, part.p);
   PSOCI_HIP(hipGetLastError());

@@ -9,6 +9,10 @@ struct Linalg {
   DevBuf<double> part, small, ysmall, theta;
   std::vector<double> yneg;
 
+  // partial-sum buffer of gemm_tn for (n, m, b); reserve_tn: for every
+  // m <= mmax, b <= bmax (so captured launches allocate nothing)
+  static size_t tn_part_size(int64_t n, int m, int b);
+  void reserve_tn(int64_t n, int mmax, int bmax);
   // out (device, m x b row-major) = A[:, 0:m]^T B[:, 0:b] over n rows (FP64 MFMA)
   void gemm_tn(int64_t n, int m, int b, const double* A, int lda, const double* B, int ldb, double* out_dev);
   // same, result copied to host (synchronises)

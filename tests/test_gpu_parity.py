@@ -151,3 +151,18 @@ def test_olsen_preconditioner_same_roots_fci10(gpu_fci10):
     with pytest.raises(api.PsociError) as e:
         H.davidson(2, prec=api.FP64, precond=2)
     assert e.value.code == api.ERR_ARG
+
+
+@pytest.mark.parametrize("prec", [api.FP64, api.MIXED])
+def test_graph_steps_equal_eager_steps_fci10(gpu_fci10, prec):
+    """The stored-sigma steps captured as hipGraphs and replayed (many
+    replays per solve, with eager launches and host work between them) give
+    the eager launches' solve bit for bit: same eigenvalues, vectors and
+    iteration count."""
+    H = gpu_fci10
+    a = H.davidson(8, prec=prec, tol=1e-7, graph=0)
+    b = H.davidson(8, prec=prec, tol=1e-7, graph=1)
+    assert a["converged"] and b["converged"]
+    assert a["stats"]["graph_launches"] >= a["stats"]["iterations"] and b["stats"]["graph_launches"] == 0
+    assert a["stats"]["iterations"] == b["stats"]["iterations"]
+    assert np.array_equal(a["evals"], b["evals"]) and np.array_equal(a["evecs"], b["evecs"])
