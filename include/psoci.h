@@ -258,6 +258,18 @@ int psoci_direct_schedule(psoci_ctx* ctx, int* order, int* chunk, double* tune_m
  * "<lanes>x<rows>").  No reference counterpart; any output may be NULL. */
 int psoci_stored_shape(psoci_ctx* ctx, int prec, int* lanes_per_nz, int* rows_per_wave, double* tune_ms);
 
+/* The stored sigma's row-length buckets (full symmetric rows of length
+ * < 64, 64-255, 256-1023, 1024-4095, >= 4096): rows[5] per bucket, the
+ * tuner's shape for each (lanes_per_nz[5], rows_per_wave[5]), whether the
+ * per-bucket launches replace the single launch of psoci_stored_shape for
+ * precision prec (*used: when they beat it by 3 % or more), and their
+ * measured 16-column time (-1 when fewer than two buckets hold rows).  The
+ * reference splits rows by length into its ELL table and the supplemental
+ * overflow (PsociGAhamiltonian.C:4223-4312, 9325-9580); no other
+ * counterpart.  Any output may be NULL. */
+int psoci_stored_buckets(psoci_ctx* ctx, int prec, int* used, int64_t* rows, int* lanes_per_nz, int* rows_per_wave,
+                         double* bucketed_ms);
+
 /* Diagonal of H in the SEF basis: computeOnlyDiagElements
  * (PsociGAhamiltonian.C:13171-13250).  diag[n] host. */
 int psoci_diag(psoci_ctx* ctx, double* diag);

@@ -106,6 +106,7 @@ def load(path: str | None = None):
         "psoci_direct_det_pairs": (I, [V, V]),
         "psoci_direct_schedule": (I, [V, V, V, V]),
         "psoci_stored_shape": (I, [V, I, V, V, V]),
+        "psoci_stored_buckets": (I, [V, I, V, V, V, V, V]),
         "psoci_sync": (I, [V]),
         "psoci_h_save": (I, [V, ctypes.c_char_p]),
         "psoci_h_load": (I, [V, ctypes.c_char_p, V]),
@@ -397,6 +398,18 @@ class Hamiltonian:
         g, r, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         self._check(self.L.psoci_stored_shape(self._h, prec, ctypes.byref(g), ctypes.byref(r), ctypes.byref(ms)))
         return {"lanes_per_nz": g.value, "rows_per_wave": r.value, "tune_ms": ms.value}
+
+    def stored_buckets(self, prec: int = FP32):
+        """The stored sigma's row-length buckets: dict(used, rows, lanes_per_nz,
+        rows_per_wave, bucketed_ms)."""
+        used, ms = ctypes.c_int(), ctypes.c_double()
+        rows = np.zeros(5, np.int64)
+        g = np.zeros(5, np.int32)
+        r = np.zeros(5, np.int32)
+        self._check(self.L.psoci_stored_buckets(self._h, prec, ctypes.byref(used), _p(rows), _p(g), _p(r),
+                                                ctypes.byref(ms)))
+        return {"used": bool(used.value), "rows": rows.tolist(), "lanes_per_nz": g.tolist(),
+                "rows_per_wave": r.tolist(), "bucketed_ms": ms.value}
 
     def sync(self):
         """Wait for the work queued on the library's stream (psoci_sync)."""

@@ -252,6 +252,23 @@ int psoci_stored_shape(psoci_ctx* ctx, int prec, int* lanes_per_nz, int* rows_pe
   });
 }
 
+int psoci_stored_buckets(psoci_ctx* ctx, int prec, int* used, int64_t* rows, int* lanes_per_nz, int* rows_per_wave,
+                         double* bucketed_ms) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_h, PSOCI_ERR_STATE, "stored sigma needs psoci_build_h first");
+    PSOCI_CHECK(prec == PSOCI_FP64 || prec == PSOCI_FP32, PSOCI_ERR_ARG, "prec must be PSOCI_FP64 or PSOCI_FP32");
+    const int pi = prec == PSOCI_FP32 ? 1 : 0;
+    if (used) *used = c.spmm_bucketed[pi] ? 1 : 0;
+    if (bucketed_ms) *bucketed_ms = c.spmm_bucket_ms[pi];
+    for (int b = 0; b < psoci::kSpmmBuckets; ++b) {
+      if (rows) rows[b] = c.spmm_boff[b + 1] - c.spmm_boff[b];
+      if (lanes_per_nz) lanes_per_nz[b] = c.spmm_bg[pi][b];
+      if (rows_per_wave) rows_per_wave[b] = c.spmm_br[pi][b];
+    }
+    return PSOCI_OK;
+  });
+}
+
 int psoci_diag(psoci_ctx* ctx, double* diag) {
   return guarded(ctx, [&](Ctx& c) {
     PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");

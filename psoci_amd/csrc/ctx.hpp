@@ -5,6 +5,8 @@
 
 namespace psoci {
 
+constexpr int kSpmmBuckets = 5;  // row-length classes of the stored sigma
+
 // Caller-provided device memory for the padded column staging of a sigma
 // call (p null: the context's own buffer, grown on demand).
 struct Staging {
@@ -51,6 +53,15 @@ struct Ctx {
   // nonzero, rows per wavefront, and the tuner's best 16-column time (-1: fixed)
   int spmm_g[2] = {1, 1}, spmm_r[2] = {1, 1};
   double spmm_tune_ms[2] = {-1, -1};
+  // row-length buckets (kSpmmBuckets length classes): the rows of each in
+  // d_brows[spmm_boff[b] .. spmm_boff[b + 1]), a shape per bucket and
+  // precision, and whether the bucketed launches are used (set_buckets and
+  // the tuner in sigma.hip)
+  DevBuf<int32_t> d_brows;
+  int64_t spmm_boff[kSpmmBuckets + 1] = {0};
+  int spmm_bg[2][kSpmmBuckets] = {}, spmm_br[2][kSpmmBuckets] = {};
+  bool spmm_bucketed[2] = {false, false};
+  double spmm_bucket_ms[2] = {-1, -1};
   DevBuf<double> d_diag;
   bool have_diag = false;
 
@@ -105,6 +116,7 @@ void upload_space(Ctx& c);
 void build_pairs(Ctx& c);               // adjacency + determinant masks (direct sigma, adjoint)
 void build_h(Ctx& c);                   // stored Hamiltonian
 void tune_stored(Ctx& c);               // stored sigma shape autotuner (after build_h)
+void set_buckets(Ctx& c);               // the stored matrix's row-length buckets
 void compute_diag(Ctx& c);              // diagonal only
 void block_on_device(Ctx& c, int64_t I, int64_t J, double* host_out, int* status);
 void sigma_stored(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy, const Staging& st = Staging{});

@@ -167,6 +167,13 @@ struct HHeader {
   double tune_ms[2];
 };
 static_assert(sizeof(HHeader) == 88, "checkpoint header layout");
+// version 2 follows the header with the stored sigma's row-length bucket
+// shapes (version 1 files have none: one shape for all rows)
+struct HBuckets {
+  int32_t used[2];
+  int32_t lanes[2][kSpmmBuckets], rows[2][kSpmmBuckets];
+};
+static_assert(sizeof(HBuckets) == 88, "checkpoint bucket block layout");
 const char kMagic[8] = {'P', 'S', 'O', 'C', 'I', 'H', 'C', 'K'};
 
 uint64_t payload_hash(const std::vector<int64_t>& rp, const std::vector<int32_t>& col, const std::vector<double>& val,
@@ -190,7 +197,7 @@ void save_h(Ctx& c, const char* path) {
   get_h_lower(c, nullptr, col.data(), val.data(), diag.data());
   HHeader h{};
   std::memcpy(h.magic, kMagic, 8);
-  h.version = 1;
+  h.version = 2;
   h.n = n;
   h.ncfg = c.sp.ncfg;
   h.nnz = nnz;
@@ -201,9 +208,18 @@ void save_h(Ctx& c, const char* path) {
     h.rows[p] = c.spmm_r[p];
     h.tune_ms[p] = c.spmm_tune_ms[p];
   }
+  HBuckets hb{};
+  for (int p = 0; p < 2; ++p) {
+    hb.used[p] = c.spmm_bucketed[p] ? 1 : 0;
+    for (int b = 0; b < kSpmmBuckets; ++b) {
+      hb.lanes[p][b] = c.spmm_bg[p][b];
+      hb.rows[p][b] = c.spmm_br[p][b];
+    }
+  }
   const uint64_t sum = payload_hash(rp, col, val, diag);
   File out(path, "wb");
   out.put(&h, 1);
+  out.put(&hb, 1);
   out.put(rp.data(), rp.size());
   out.put(col.data(), col.size());
   out.put(val.data(), val.size());
@@ -217,8 +233,10 @@ void load_h(Ctx& c, const char* path) {
   File in(path, "rb");
   HHeader h{};
   in.get(&h, 1, "the header");
-  PSOCI_CHECK(std::memcmp(h.magic, kMagic, 8) == 0 && h.version == 1, PSOCI_ERR_ARG,
+  PSOCI_CHECK(std::memcmp(h.magic, kMagic, 8) == 0 && (h.version == 1 || h.version == 2), PSOCI_ERR_ARG,
               in.path + ": not a stored-Hamiltonian checkpoint");
+  HBuckets hb{};
+  const int64_t hsize = (int64_t)sizeof(HHeader) + (h.version == 2 ? (int64_t)sizeof(HBuckets) : 0);
   const int64_t n = c.sp.nsef_total;
   PSOCI_CHECK(h.n == n && h.ncfg == c.sp.ncfg && h.space_hash == c.space_hash, PSOCI_ERR_ARG,
               in.path + ": written for another configuration list");
@@ -226,8 +244,9 @@ void load_h(Ctx& c, const char* path) {
   const int64_t nnz = h.nnz;
   // nnz bounded first (n < 2^31, nnz <= 2^40), so the file size below cannot overflow
   PSOCI_CHECK(nnz >= n && nnz <= n * (n + 1) / 2 && nnz <= (int64_t(1) << 40) &&
-                  in.size() == (int64_t)sizeof(HHeader) + 8 * (n + 1) + 12 * nnz + 8 * n + 8,
+                  in.size() == hsize + 8 * (n + 1) + 12 * nnz + 8 * n + 8,
               PSOCI_ERR_ARG, in.path + ": file size does not match its header");
+  if (h.version == 2) in.get(&hb, 1, "the bucket shapes");
   std::vector<int64_t> rp(n + 1);
   std::vector<int32_t> col(nnz);
   std::vector<double> val(nnz), diag(n);
@@ -272,16 +291,29 @@ void load_h(Ctx& c, const char* path) {
       fval[cur[q]++] = val[e];
     }
   install_h(c, frp, fcol, fval, lowcnt, diag);
+  auto lanes_ok = [](int g) { return g == 1 || g == 2 || g == 4; };
+  auto rows_ok = [](int r) { return r == 1 || r == 2 || r == 4 || r == 8; };
   bool shapes_ok = true;
-  for (int p = 0; p < 2; ++p)
-    shapes_ok = shapes_ok && (h.lanes[p] == 1 || h.lanes[p] == 2 || h.lanes[p] == 4) &&
-                (h.rows[p] == 1 || h.rows[p] == 2 || h.rows[p] == 4 || h.rows[p] == 8);
+  for (int p = 0; p < 2; ++p) {
+    shapes_ok = shapes_ok && lanes_ok(h.lanes[p]) && rows_ok(h.rows[p]);
+    if (h.version == 2 && hb.used[p]) {
+      shapes_ok = shapes_ok && hb.used[p] == 1;
+      for (int b = 0; b < kSpmmBuckets; ++b) shapes_ok = shapes_ok && lanes_ok(hb.lanes[p][b]) && rows_ok(hb.rows[p][b]);
+    }
+  }
   if (shapes_ok && !std::getenv("PSOCI_SPMM_SHAPE")) {
-    // the writer's tuned stored-sigma shapes: the same kernels, bit-for-bit
+    // the writer's tuned stored-sigma shapes (and row-length buckets, which
+    // depend on the matrix alone): the same kernels, bit for bit
+    set_buckets(c);
     for (int p = 0; p < 2; ++p) {
       c.spmm_g[p] = h.lanes[p];
       c.spmm_r[p] = h.rows[p];
       c.spmm_tune_ms[p] = h.tune_ms[p];
+      c.spmm_bucketed[p] = h.version == 2 && hb.used[p] == 1;
+      for (int b = 0; b < kSpmmBuckets; ++b) {
+        c.spmm_bg[p][b] = h.version == 2 ? hb.lanes[p][b] : 1;
+        c.spmm_br[p][b] = h.version == 2 ? hb.rows[p][b] : 1;
+      }
     }
   } else {
     tune_stored(c);

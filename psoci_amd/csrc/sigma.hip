@@ -37,21 +37,26 @@ __device__ __forceinline__ float fmadd<float>(float a, float b, float c) {
 // the G lanes of a nonzero read one contiguous piece of its x row).  Lanes
 // of a row stride over its nonzeros; partial sums are reduced across the
 // nonzero slots of the row with shuffles.  (R, G) = (1, 1) is one wavefront
-// per row with whole x rows per lane.
+// per row with whole x rows per lane.  rows (optional): the launch covers
+// the nrows rows listed there (one row-length bucket) instead of 0..n-1.
 template <typename T, int K, int G, int R>
-__global__ __launch_bounds__(256) void k_spmm(int64_t n, const int64_t* __restrict__ rowptr,
-                                              const int32_t* __restrict__ col, const T* __restrict__ val,
-                                              const T* __restrict__ X, int ldx, T* __restrict__ Y, int ldy) {
+__global__ __launch_bounds__(256) void k_spmm(int64_t nrows, const int32_t* __restrict__ rows,
+                                              const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                              const T* __restrict__ val, const T* __restrict__ X, int ldx,
+                                              T* __restrict__ Y, int ldy) {
   constexpr int LR = kWave / R;   // lanes per row
   constexpr int NS = LR / G;      // nonzero slots per row
   constexpr int C = K / G;        // columns per lane
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t r = ((int64_t)blockIdx.x * (blockDim.x / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave)) * R + lane / LR;
+  const int64_t slot_r =
+      ((int64_t)blockIdx.x * (blockDim.x / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave)) * R + lane / LR;
+  const bool live = slot_r < nrows;
+  const int64_t r = live ? (rows ? (int64_t)rows[slot_r] : slot_r) : 0;
   const int lr = lane % LR, slot = lr / G, cc = lr % G;
   T acc[C];
 #pragma unroll
   for (int v = 0; v < C; ++v) acc[v] = (T)0;
-  if (r < n) {
+  if (live) {
     const int64_t lo = rowptr[r], hi = rowptr[r + 1];
     for (int64_t q = lo + slot; q < hi; q += NS) {
       const T a = val[q];
@@ -66,7 +71,7 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, const int64_t* __restri
 #pragma unroll
     for (int v = 0; v < C; ++v) acc[v] += __shfl_xor(acc[v], off, kWave);
   }
-  if (r < n && slot == 0) {
+  if (live && slot == 0) {
     T* y = Y + r * ldy + cc * C;
 #pragma unroll
     for (int v = 0; v < C; ++v) y[v] = acc[v];
@@ -74,27 +79,28 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, const int64_t* __restri
 }
 
 template <typename T, int K, int G, int R>
-void launch_gr(Ctx& c, const T* val, const T* X, int ldx, T* Y, int ldy) {
-  const int64_t n = c.sp.nsef_total;
+void launch_gr(Ctx& c, const T* val, const T* X, int ldx, T* Y, int ldy, const int32_t* rows, int64_t nrows) {
   const int64_t rows_per_block = 4 * R;
-  hipLaunchKernelGGL((k_spmm<T, K, G, R>), dim3((unsigned)((n + rows_per_block - 1) / rows_per_block)), dim3(256), 0,
-                     c.stream, n, c.d_rowptr.p, c.d_col.p, val, X, ldx, Y, ldy);
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL((k_spmm<T, K, G, R>), dim3((unsigned)((nrows + rows_per_block - 1) / rows_per_block)), dim3(256),
+                     0, c.stream, nrows, rows, c.d_rowptr.p, c.d_col.p, val, X, ldx, Y, ldy);
   PSOCI_HIP(hipGetLastError());
 }
 
-// the tuned shape of this precision, where the width allows it
+// one launch of shape (g lanes per nonzero, r rows per wavefront) over the
+// listed rows (rows null: all n)
 template <typename T, int K>
-void launch_spmm(Ctx& c, const T* val, const T* X, int ldx, T* Y, int ldy) {
-  const int pi = sizeof(T) == 4 ? 1 : 0;
-  const int g = K >= c.spmm_g[pi] ? c.spmm_g[pi] : 1;
-  const int key = g * 16 + c.spmm_r[pi];
+void launch_shape(Ctx& c, int g0, int r, const T* val, const T* X, int ldx, T* Y, int ldy, const int32_t* rows,
+                  int64_t nrows) {
+  const int g = K >= g0 ? g0 : 1;
+  const int key = g * 16 + r;
   switch (key) {
-#define PSOCI_SPMM_CASE(GG, RR)                                     \
-  case GG * 16 + RR:                                                \
-    if constexpr (K >= GG) {                                        \
-      launch_gr<T, K, (K >= GG ? GG : 1), RR>(c, val, X, ldx, Y, ldy); \
-      return;                                                       \
-    }                                                               \
+#define PSOCI_SPMM_CASE(GG, RR)                                                    \
+  case GG * 16 + RR:                                                               \
+    if constexpr (K >= GG) {                                                       \
+      launch_gr<T, K, (K >= GG ? GG : 1), RR>(c, val, X, ldx, Y, ldy, rows, nrows); \
+      return;                                                                      \
+    }                                                                              \
     break;
     PSOCI_SPMM_CASE(1, 1)
     PSOCI_SPMM_CASE(1, 2)
@@ -111,7 +117,23 @@ void launch_spmm(Ctx& c, const T* val, const T* X, int ldx, T* Y, int ldy) {
 #undef PSOCI_SPMM_CASE
     default: break;
   }
-  launch_gr<T, K, 1, 1>(c, val, X, ldx, Y, ldy);
+  launch_gr<T, K, 1, 1>(c, val, X, ldx, Y, ldy, rows, nrows);
+}
+
+// the tuned shape(s) of this precision: one launch over all rows, or one
+// per non-empty row-length bucket with that bucket's shape
+template <typename T, int K>
+void launch_spmm(Ctx& c, const T* val, const T* X, int ldx, T* Y, int ldy) {
+  const int pi = sizeof(T) == 4 ? 1 : 0;
+  if (!c.spmm_bucketed[pi]) {
+    launch_shape<T, K>(c, c.spmm_g[pi], c.spmm_r[pi], val, X, ldx, Y, ldy, nullptr, c.sp.nsef_total);
+    return;
+  }
+  for (int b = 0; b < kSpmmBuckets; ++b) {
+    const int64_t lo = c.spmm_boff[b], cnt = c.spmm_boff[b + 1] - lo;
+    if (cnt > 0)
+      launch_shape<T, K>(c, c.spmm_bg[pi][b], c.spmm_br[pi][b], val, X, ldx, Y, ldy, c.d_brows.p + lo, cnt);
+  }
 }
 
 template <typename T>
@@ -134,6 +156,26 @@ void spmm_any_k(Ctx& c, const T* val, int k, const T* X, int ldx, T* Y, int ldy,
 // nonzero, rows per wavefront) pair times the 16-column product on the built
 // matrix (best of two after a warm-up) and the fastest is kept for all
 // widths.  PSOCI_SPMM_SHAPE=<G>x<R> fixes it instead.
+// best of two timed 16-column launches (after a warm-up) of f
+template <class F>
+double time_ms(Ctx& c, hipEvent_t e0, hipEvent_t e1, F&& f) {
+  double t = 1e300;
+  for (int rep = 0; rep < 3; ++rep) {
+    PSOCI_HIP(hipEventRecord(e0, c.stream));
+    f();
+    PSOCI_HIP(hipEventRecord(e1, c.stream));
+    PSOCI_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    PSOCI_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (rep > 0) t = std::min(t, (double)ms);
+  }
+  return t;
+}
+
+// Per precision: the best (lanes per nonzero, rows per wavefront) for all
+// rows in one launch, and the best for each row-length bucket on its own;
+// the bucketed launches are kept when together they beat the single one by
+// 3 % or more (PSOCI_SPMM_BUCKETS=0 / 1 fixes the choice).
 template <typename T>
 double tune_prec(Ctx& c, const T* val, hipEvent_t e0, hipEvent_t e1) {
   const int pi = sizeof(T) == 4 ? 1 : 0;
@@ -143,34 +185,73 @@ double tune_prec(Ctx& c, const T* val, hipEvent_t e0, hipEvent_t e1) {
   X.alloc((size_t)n * K);
   Y.alloc((size_t)n * K);
   PSOCI_HIP(hipMemsetAsync(X.p, 0, (size_t)n * K * sizeof(T), c.stream));
-  double best = 1e300;
-  int bg = 1, br = 1;
-  for (int g : {1, 2, 4})
-    for (int r : {1, 2, 4, 8}) {
-      c.spmm_g[pi] = g;
-      c.spmm_r[pi] = r;
-      double t = 1e300;
-      for (int rep = 0; rep < 3; ++rep) {
-        PSOCI_HIP(hipEventRecord(e0, c.stream));
-        launch_spmm<T, 16>(c, val, X.p, K, Y.p, K);
-        PSOCI_HIP(hipEventRecord(e1, c.stream));
-        PSOCI_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        PSOCI_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (rep > 0) t = std::min(t, (double)ms);
+  const int gs[] = {1, 2, 4}, rs[] = {1, 2, 4, 8};
+  auto best_shape = [&](const int32_t* rows, int64_t nrows, int& bg, int& br) {
+    double best = 1e300;
+    bg = br = 1;
+    for (int g : gs)
+      for (int r : rs) {
+        const double t = time_ms(c, e0, e1, [&] { launch_shape<T, 16>(c, g, r, val, X.p, K, Y.p, K, rows, nrows); });
+        if (t < best) {
+          best = t;
+          bg = g;
+          br = r;
+        }
       }
-      if (t < best) {
-        best = t;
-        bg = g;
-        br = r;
-      }
+    return best;
+  };
+  c.spmm_bucketed[pi] = false;
+  const double single = best_shape(nullptr, n, c.spmm_g[pi], c.spmm_r[pi]);
+  int nonempty = 0;
+  for (int b = 0; b < kSpmmBuckets; ++b) {
+    const int64_t cnt = c.spmm_boff[b + 1] - c.spmm_boff[b];
+    c.spmm_bg[pi][b] = c.spmm_br[pi][b] = 1;
+    if (cnt > 0) {
+      ++nonempty;
+      best_shape(c.d_brows.p + c.spmm_boff[b], cnt, c.spmm_bg[pi][b], c.spmm_br[pi][b]);
     }
-  c.spmm_g[pi] = bg;
-  c.spmm_r[pi] = br;
-  return best;
+  }
+  if (nonempty > 1) {
+    c.spmm_bucketed[pi] = true;
+    const double bucketed = time_ms(c, e0, e1, [&] { launch_spmm<T, 16>(c, val, X.p, K, Y.p, K); });
+    c.spmm_bucketed[pi] = bucketed < 0.97 * single;
+    c.spmm_bucket_ms[pi] = bucketed;
+    // PSOCI_SPMM_BUCKETS=0 / 1 fixes the choice instead
+    if (const char* eb = std::getenv("PSOCI_SPMM_BUCKETS")) c.spmm_bucketed[pi] = eb[0] == '1';
+  } else {
+    c.spmm_bucket_ms[pi] = -1.0;
+  }
+  return c.spmm_bucketed[pi] ? c.spmm_bucket_ms[pi] : single;
+}
+
+// Row-length buckets of the stored matrix (full symmetric rows): lengths
+// < 64, 64-255, 256-1023, 1024-4095, >= 4096; the row lists in ascending
+// order per bucket.  A property of the matrix alone, so a loaded checkpoint
+// gets the same buckets.
+void set_buckets(Ctx& c) {
+  const int64_t n = c.sp.nsef_total;
+  std::vector<int64_t> rp(n + 1);
+  c.d_rowptr.download(rp.data(), n + 1, c.stream);
+  c.sync();
+  std::vector<int32_t> rows;
+  rows.reserve(n);
+  const int64_t lim[kSpmmBuckets] = {64, 256, 1024, 4096, INT64_MAX};
+  c.spmm_boff[0] = 0;
+  for (int b = 0; b < kSpmmBuckets; ++b) {
+    const int64_t lo = b ? lim[b - 1] : 0;
+    for (int64_t r = 0; r < n; ++r) {
+      const int64_t len = rp[r + 1] - rp[r];
+      if (len >= lo && len < lim[b]) rows.push_back((int32_t)r);
+    }
+    c.spmm_boff[b + 1] = (int64_t)rows.size();
+  }
+  c.d_brows.upload(rows, c.stream);
+  c.sync();
 }
 
 void tune_stored(Ctx& c) {
+  set_buckets(c);
+  c.spmm_bucketed[0] = c.spmm_bucketed[1] = false;
   const char* env = std::getenv("PSOCI_SPMM_SHAPE");
   if (env) {
     int g = 1, r = 1;

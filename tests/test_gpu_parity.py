@@ -166,3 +166,36 @@ def test_graph_steps_equal_eager_steps_fci10(gpu_fci10, prec):
     assert a["stats"]["graph_launches"] >= a["stats"]["iterations"] and b["stats"]["graph_launches"] == 0
     assert a["stats"]["iterations"] == b["stats"]["iterations"]
     assert np.array_equal(a["evals"], b["evals"]) and np.array_equal(a["evecs"], b["evecs"])
+
+
+@pytest.mark.parametrize("force", ["0", "1"])
+def test_stored_row_buckets(fci10, oracle_fci10, force, monkeypatch):
+    """Row-length buckets of the stored sigma (fci10: 45 rows shorter than
+    64, the rest 256-1023 long): the bucketed launches and the single one
+    both give the oracle's sigma; a checkpoint carries the choice, so the
+    loaded operator's sigma is the built one's bit for bit."""
+    import os
+    import tempfile
+    monkeypatch.setenv("PSOCI_SPMM_BUCKETS", force)
+    cfg, ints, _ = fci10
+    H = api.Hamiltonian(cfg, ints)
+    H.build()
+    info = H.stored_buckets(api.FP32)
+    assert info["rows"] == [45, 0, 31605, 0, 0]
+    assert info["used"] == (force == "1") and H.stored_buckets(api.FP64)["used"] == (force == "1")
+    X = np.random.default_rng(7).standard_normal((H.n, 12))
+    ref = oracle_fci10.sigma(X)
+    Y = H.sigma(X, api.STORED, api.FP64)
+    assert np.abs(Y - ref).max() <= 1e-12 * np.abs(ref).max()
+    Y32 = H.sigma(X.astype(np.float32), api.STORED, api.FP32)
+    assert np.abs(Y32 - ref).max() <= 2e-5 * np.abs(ref).max()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "h.bin")
+        H.save_h(path)
+        monkeypatch.delenv("PSOCI_SPMM_BUCKETS")
+        G = api.Hamiltonian(cfg, ints)
+        G.load_h(path)
+        assert G.stored_buckets(api.FP32)["used"] == info["used"]
+        assert np.array_equal(G.sigma(X, api.STORED, api.FP64), Y)
+        G.close()
+    H.close()
