@@ -3,10 +3,10 @@ against the all-FP64 GPU run, and the sigma kernel's HBM roofline fraction.
 
 One "step" is one complete mixed-precision block-Davidson solve for the
 lowest 16 roots, with the space, integrals and (for the stored workload) the
-Hamiltonian already resident in HBM; on the direct workload 8 guard roots
-ride in its FP32 phase (a 24-column block).  The all-FP64 solve of the same
-space (16 columns, no guard roots) is run once beside it; every root of
-each timed solve must agree with it to 1e-8 Eh or the run fails.
+Hamiltonian already resident in HBM; on the direct workload the subspace
+holds up to 15 x 16 = 240 vectors.  The all-FP64 solve of the same space,
+with the same solver settings, is run once beside it; every root of each
+timed solve must agree with it to 1e-8 Eh or the run fails.
 
 Prints ONE JSON line (rank 0, N = 1).  Usage:
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mrci10x24]
@@ -44,9 +44,14 @@ def parse():
     ap.add_argument("--workload", default="mrci10x24")
     ap.add_argument("--roots", type=int, default=16)
     ap.add_argument("--tol", type=float, default=1e-5)
-    ap.add_argument("--guard-roots", type=int, default=None,
-                    help="guard roots in the FP32 phase of the mixed solve (default 8 direct, 0 stored)")
+    ap.add_argument("--guard-roots", type=int, default=0,
+                    help="guard roots in the FP32 phase of the mixed solve")
+    ap.add_argument("--max-subspace", type=int, default=None,
+                    help="Davidson subspace cap in vectors (default 15 x roots direct, the library's 5 x roots "
+                         "stored); the same for the FP64 and the mixed solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--graph", type=int, default=0,
+                    help="Davidson steps: 0 stored-sigma steps as captured hipGraphs, 1 eager launches")
     ap.add_argument("--precond", type=int, default=0,
                     help="Davidson preconditioner: 0 the reference's diagonal one, 1 the same with Olsen's "
                          "correction")
@@ -190,7 +195,8 @@ def main():
     args.warmup = max(0, args.warmup)
     mode = api.DIRECT if direct else api.STORED
     k = args.roots
-    guard = args.guard_roots if args.guard_roots is not None else (8 if direct else 0)
+    guard = args.guard_roots
+    msub = args.max_subspace if args.max_subspace is not None else (15 * k if direct else 0)
     n_sef = int(cfg.nsef().sum())
     print(f"[bench] {args.workload}: ncfg={cfg.ncfg} nsef={n_sef} mode={'direct' if direct else 'stored'} k={k}",
           file=sys.stderr, flush=True)
@@ -228,7 +234,8 @@ def main():
         return
 
     # all-FP64 reference solve of the same space
-    ref = H.davidson(k, prec=api.FP64, mode=mode, tol=args.tol, want_vectors=False, precond=args.precond)
+    ref = H.davidson(k, prec=api.FP64, mode=mode, tol=args.tol, want_vectors=False, precond=args.precond, graph=args.graph,
+                     max_subspace=msub)
     if not ref["converged"]:
         raise SystemExit("FP64 Davidson did not converge")
     fp64_s = ref["stats"]["seconds"]
@@ -250,7 +257,7 @@ def main():
     while warm_done < args.warmup and fits(1 + min(args.steps, 2)):
         tw = time.time()
         H.davidson(k, prec=api.MIXED, mode=mode, tol=args.tol, switch_tol=1e-4, want_vectors=False,
-                   extra_roots=guard, precond=args.precond)
+                   extra_roots=guard, precond=args.precond, graph=args.graph, max_subspace=msub)
         torch.cuda.synchronize()
         durations.append(time.time() - tw)
         warm_done += 1
@@ -265,7 +272,7 @@ def main():
         torch.cuda.synchronize()
         ts = time.time()
         out = H.davidson(k, prec=api.MIXED, mode=mode, tol=args.tol, switch_tol=1e-4, want_vectors=False,
-                         extra_roots=guard, precond=args.precond)
+                         extra_roots=guard, precond=args.precond, graph=args.graph, max_subspace=msub)
         torch.cuda.synchronize()
         times.append(time.time() - ts)
         durations.append(times[-1])
@@ -339,11 +346,15 @@ def main():
         "root_diffs_eh": [float(x) for x in np.abs(last["evals"] - ref["evals"])],
         "iterations_mixed": last["stats"]["iterations"],
         "guard_roots_fp32_phase": guard,
+        "max_subspace": msub if msub else 5 * k,
         "preconditioner": ("diagonal (the reference's)" if args.precond == 0 else
                            "diagonal with Olsen's correction"),
         "switch_iter": last["stats"]["switch_iter"],
         "iterations_fp64": ref["stats"]["iterations"],
         "sigma_calls_mixed": last["stats"]["sigma_calls"],
+        "host_eig_seconds_mixed": round(last["stats"]["eig_seconds"], 6),
+        "graph_launches_mixed": last["stats"]["graph_launches"],
+        "sigma_seconds_mixed": round(last["stats"]["sigma_seconds"], 6),
         "setup_seconds": round(setup_s, 4),
         "sigma_fp32_ms": round(ms32, 5),
         "sigma_fp64_ms": round(ms64, 5),
@@ -365,7 +376,7 @@ def main():
         dp = H.direct_det_pairs()
         line["direct"] = {"ordered_configuration_pairs": npairs, "structural_sef_elements_per_sigma": nelems,
                           "connected_determinant_pairs_per_sigma": dp,
-                          "schedule": H.direct_schedule(),
+                          "schedule": {"fp32": H.direct_schedule(api.FP32), "fp64": H.direct_schedule(api.FP64)},
                           f"det_pairs_per_s_fp32_k{k}": round(dp / (ms32 * 1e-3), 1),
                           f"det_pairs_per_s_fp32_k{kr}": round(dp / (msr * 1e-3), 1),
                           f"det_pairs_per_s_fp64_k{k}": round(dp / (ms64 * 1e-3), 1),

@@ -244,12 +244,14 @@ int psoci_direct_info(psoci_ctx* ctx, int64_t* npairs, int64_t* nelems);
 int psoci_direct_det_pairs(psoci_ctx* ctx, int64_t* det_pairs);
 
 /* The direct sigma's launch schedule as chosen by the autotuner at pair
- * setup: row order (0 natural, 1 by occupation string), XCD run length in
- * workgroups, and the tuner's best sample time in ms (-1 when fixed by the
- * PSOCI_DIRECT_ORDER / PSOCI_DIRECT_CHUNK environment).  No reference
- * counterpart (the reference has no launch schedule); any argument may be
- * NULL. */
+ * setup, one per precision: row order (0 natural, 1 by occupation string),
+ * XCD run length in workgroups, and the tuner's best sample time in ms (-1
+ * when fixed by the PSOCI_DIRECT_ORDER / PSOCI_DIRECT_CHUNK environment).
+ * psoci_direct_schedule reports the FP32 one, psoci_direct_schedule_prec
+ * the one of prec (PSOCI_FP64 or PSOCI_FP32).  No reference counterpart
+ * (the reference has no launch schedule); any output may be NULL. */
 int psoci_direct_schedule(psoci_ctx* ctx, int* order, int* chunk, double* tune_ms);
+int psoci_direct_schedule_prec(psoci_ctx* ctx, int prec, int* order, int* chunk, double* tune_ms);
 
 /* The stored sigma's kernel shape for precision prec (PSOCI_FP64 or
  * PSOCI_FP32) as chosen by the autotuner after psoci_build_h: lanes per

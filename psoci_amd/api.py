@@ -105,6 +105,7 @@ def load(path: str | None = None):
         "psoci_direct_info": (I, [V, V, V]),
         "psoci_direct_det_pairs": (I, [V, V]),
         "psoci_direct_schedule": (I, [V, V, V, V]),
+        "psoci_direct_schedule_prec": (I, [V, I, V, V, V]),
         "psoci_stored_shape": (I, [V, I, V, V, V]),
         "psoci_stored_buckets": (I, [V, I, V, V, V, V, V]),
         "psoci_sync": (I, [V]),
@@ -387,10 +388,12 @@ class Hamiltonian:
         self._check(self.L.psoci_direct_det_pairs(self._h, ctypes.byref(v)))
         return v.value
 
-    def direct_schedule(self):
-        """The direct sigma's autotuned schedule: dict(order, chunk, tune_ms)."""
+    def direct_schedule(self, prec: int = FP32):
+        """The direct sigma's autotuned schedule of a precision: dict(order,
+        chunk, tune_ms)."""
         o, ch, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
-        self._check(self.L.psoci_direct_schedule(self._h, ctypes.byref(o), ctypes.byref(ch), ctypes.byref(ms)))
+        self._check(self.L.psoci_direct_schedule_prec(self._h, prec, ctypes.byref(o), ctypes.byref(ch),
+                                                      ctypes.byref(ms)))
         return {"order": o.value, "chunk": ch.value, "tune_ms": ms.value}
 
     def stored_shape(self, prec: int = FP32):

@@ -233,9 +233,22 @@ int psoci_direct_schedule(psoci_ctx* ctx, int* order, int* chunk, double* tune_m
   return guarded(ctx, [&](Ctx& c) {
     PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
     if (!c.have_pairs) psoci::build_pairs(c);
-    if (order) *order = c.direct_order ? 1 : 0;
-    if (chunk) *chunk = c.direct_chunk;
-    if (tune_ms) *tune_ms = c.direct_tune_ms;
+    if (order) *order = c.direct_order[1] ? 1 : 0;
+    if (chunk) *chunk = c.direct_chunk[1];
+    if (tune_ms) *tune_ms = c.direct_tune_ms[1];
+    return PSOCI_OK;
+  });
+}
+
+int psoci_direct_schedule_prec(psoci_ctx* ctx, int prec, int* order, int* chunk, double* tune_ms) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
+    if (!c.have_pairs) psoci::build_pairs(c);
+    PSOCI_CHECK(prec == PSOCI_FP64 || prec == PSOCI_FP32, PSOCI_ERR_ARG, "prec must be PSOCI_FP64 or PSOCI_FP32");
+    const int pi = prec == PSOCI_FP32 ? 1 : 0;
+    if (order) *order = c.direct_order[pi] ? 1 : 0;
+    if (chunk) *chunk = c.direct_chunk[pi];
+    if (tune_ms) *tune_ms = c.direct_tune_ms[pi];
     return PSOCI_OK;
   });
 }

@@ -79,11 +79,12 @@ struct Ctx {
   DevBuf<int32_t> d_nsgl;     // ncfg: single-excitation records (next)
   DevBuf<int32_t> d_order;    // alternative row order of the direct sigma (items when split)
   DevBuf<int32_t> d_items;    // split spaces: the natural order's work items
-  bool direct_order = false;  // use d_order
+  // schedule of the direct sigma per precision ([0] FP64, [1] FP32)
+  bool direct_order[2] = {false, false};  // use d_order
   bool direct_split = false;  // rows are work items (I << 2 | q): SEFs [64q, 64q + 64) of I
   int64_t direct_nrows = 0;   // rows (work items) per direct sigma launch
-  int direct_chunk = 0;       // XCD run length (workgroups) of the direct sigma schedule
-  double direct_tune_ms = -1; // the tuner's best sample time (-1: fixed by the environment)
+  int direct_chunk[2] = {0, 0};         // XCD run length (workgroups)
+  double direct_tune_ms[2] = {-1, -1};  // the tuner's best sample time (-1: fixed by the environment)
   DevBuf<ConfRec> d_conf;     // packed configurations
   DevBuf<double> d_xt;        // XT[e][f][o] = (e o|o f)
   DevBuf<int64_t> d_detoff;   // ncfg

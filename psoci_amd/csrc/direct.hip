@@ -463,7 +463,7 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a, const T* __restric
 }
 
 
-DirectArgs direct_args(Ctx& c) {
+DirectArgs direct_args(Ctx& c, int pi) {
   DirectArgs a;
   a.ncfg = c.sp.ncfg;
   a.recoff = c.d_adjoff.p;
@@ -480,8 +480,8 @@ DirectArgs direct_args(Ctx& c) {
   a.cdiag = c.d_cdiag.p;
   a.ndbl = c.d_ndbl.p;
   a.nsgl = c.d_nsgl.p;
-  a.order = c.direct_order ? c.d_order.p : (c.direct_split ? c.d_items.p : nullptr);
-  a.chunk = c.direct_chunk;
+  a.order = c.direct_order[pi] ? c.d_order.p : (c.direct_split ? c.d_items.p : nullptr);
+  a.chunk = c.direct_chunk[pi];
   a.row0 = 0;
   a.nrows = c.direct_nrows;
   a.split = c.direct_split ? 1 : 0;
@@ -492,7 +492,7 @@ DirectArgs direct_args(Ctx& c) {
 template <typename T, int K>
 void launch_direct(Ctx& c, const T* X, int ldx, T* Y, int ldy, int64_t row0 = 0, int64_t nrows = -1,
                    unsigned long long* count = nullptr) {
-  DirectArgs a = direct_args(c);
+  DirectArgs a = direct_args(c, sizeof(T) == 4 ? 1 : 0);
   a.count = count;
   if (nrows >= 0) {
     a.row0 = row0;
@@ -532,42 +532,33 @@ void direct_any_k(Ctx& c, int k, const T* X, int ldx, T* Y, int ldy, const Stagi
 
 // Schedule autotuner: the row order (natural, or by occupation string from
 // orbital 0) and the XCD run length decide how often column configurations
-// are found in L2, which depends on the space's structure.  Each candidate
-// runs the FP32 16-column kernel on the middle eighth of its row order
+// are found in L2, which depends on the space's structure and on the x-row
+// size, so each precision is tuned on its own: every candidate runs the
+// 16-column kernel of that precision on the middle eighth of its row order
 // (at least 65,536 rows, or all) twice; the fastest is kept.
 // PSOCI_DIRECT_ORDER / PSOCI_DIRECT_CHUNK fix the choice instead.
-void tune_direct(Ctx& c) {
-  const char* eo = std::getenv("PSOCI_DIRECT_ORDER");
-  const char* ec = std::getenv("PSOCI_DIRECT_CHUNK");
-  if (eo || ec) {
-    c.direct_order = eo && eo[0] == '1';
-    c.direct_chunk = ec ? std::max(0, std::atoi(ec)) : 0;
-    c.direct_tune_ms = -1.0;
-    return;
-  }
+template <typename T>
+void tune_direct_prec(Ctx& c, int pi, hipEvent_t e0, hipEvent_t e1) {
   const int64_t nr = c.direct_nrows, n = c.sp.nsef_total;
   const int64_t rows = std::min<int64_t>(nr, std::max<int64_t>(nr / 8, 65536));
   const int64_t row0 = (nr - rows) / 2;
   const int K = 16;
-  DevBuf<float> X, Y;
+  DevBuf<T> X, Y;
   X.alloc((size_t)n * K);
   Y.alloc((size_t)n * K);
-  PSOCI_HIP(hipMemsetAsync(X.p, 0, (size_t)n * K * sizeof(float), c.stream));
-  hipEvent_t e0, e1;
-  PSOCI_HIP(hipEventCreate(&e0));
-  PSOCI_HIP(hipEventCreate(&e1));
-  const int chunks[] = {0, 4, 16};
+  PSOCI_HIP(hipMemsetAsync(X.p, 0, (size_t)n * K * sizeof(T), c.stream));
+  const int chunks[] = {0, 4, 16, 64};
   double best = 1e300;
   bool best_order = false;
   int best_chunk = 0;
   for (int o = 0; o < 2; ++o)
     for (int ch : chunks) {
-      c.direct_order = o == 1;
-      c.direct_chunk = ch;
+      c.direct_order[pi] = o == 1;
+      c.direct_chunk[pi] = ch;
       double t = 1e300;
       for (int r = 0; r < 2; ++r) {
         PSOCI_HIP(hipEventRecord(e0, c.stream));
-        launch_direct<float, 16>(c, X.p, K, Y.p, K, row0, rows);
+        launch_direct<T, 16>(c, X.p, K, Y.p, K, row0, rows);
         PSOCI_HIP(hipEventRecord(e1, c.stream));
         PSOCI_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -576,15 +567,33 @@ void tune_direct(Ctx& c) {
       }
       if (t < best) {
         best = t;
-        best_order = c.direct_order;
+        best_order = c.direct_order[pi];
         best_chunk = ch;
       }
     }
+  c.direct_order[pi] = best_order;
+  c.direct_chunk[pi] = best_chunk;
+  c.direct_tune_ms[pi] = best;
+}
+
+void tune_direct(Ctx& c) {
+  const char* eo = std::getenv("PSOCI_DIRECT_ORDER");
+  const char* ec = std::getenv("PSOCI_DIRECT_CHUNK");
+  if (eo || ec) {
+    for (int pi = 0; pi < 2; ++pi) {
+      c.direct_order[pi] = eo && eo[0] == '1';
+      c.direct_chunk[pi] = ec ? std::max(0, std::atoi(ec)) : 0;
+      c.direct_tune_ms[pi] = -1.0;
+    }
+    return;
+  }
+  hipEvent_t e0, e1;
+  PSOCI_HIP(hipEventCreate(&e0));
+  PSOCI_HIP(hipEventCreate(&e1));
+  tune_direct_prec<float>(c, 1, e0, e1);
+  tune_direct_prec<double>(c, 0, e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  c.direct_order = best_order;
-  c.direct_chunk = best_chunk;
-  c.direct_tune_ms = best;
 }
 
 }  // namespace
