@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--graph", type=int, default=0,
                     help="Davidson steps: 0 stored-sigma steps as captured hipGraphs, 1 eager launches")
+    ap.add_argument("--select-rule", type=int, default=0,
+                    help="Davidson corrections per iteration: 0 the padded kernel width's free columns take more, "
+                         "1 the reference's one per unconverged root")
     ap.add_argument("--precond", type=int, default=0,
                     help="Davidson preconditioner: 0 the reference's diagonal one, 1 the same with Olsen's "
                          "correction")
@@ -234,8 +237,8 @@ def main():
         return
 
     # all-FP64 reference solve of the same space
-    ref = H.davidson(k, prec=api.FP64, mode=mode, tol=args.tol, want_vectors=False, precond=args.precond, graph=args.graph,
-                     max_subspace=msub)
+    ref = H.davidson(k, prec=api.FP64, mode=mode, tol=args.tol, want_vectors=False, precond=args.precond,
+                     graph=args.graph, select_rule=args.select_rule, max_subspace=msub)
     if not ref["converged"]:
         raise SystemExit("FP64 Davidson did not converge")
     fp64_s = ref["stats"]["seconds"]
@@ -257,7 +260,8 @@ def main():
     while warm_done < args.warmup and fits(1 + min(args.steps, 2)):
         tw = time.time()
         H.davidson(k, prec=api.MIXED, mode=mode, tol=args.tol, switch_tol=1e-4, want_vectors=False,
-                   extra_roots=guard, precond=args.precond, graph=args.graph, max_subspace=msub)
+                   extra_roots=guard, precond=args.precond, graph=args.graph, select_rule=args.select_rule,
+                   max_subspace=msub)
         torch.cuda.synchronize()
         durations.append(time.time() - tw)
         warm_done += 1
@@ -272,7 +276,8 @@ def main():
         torch.cuda.synchronize()
         ts = time.time()
         out = H.davidson(k, prec=api.MIXED, mode=mode, tol=args.tol, switch_tol=1e-4, want_vectors=False,
-                         extra_roots=guard, precond=args.precond, graph=args.graph, max_subspace=msub)
+                         extra_roots=guard, precond=args.precond, graph=args.graph,
+                         select_rule=args.select_rule, max_subspace=msub)
         torch.cuda.synchronize()
         times.append(time.time() - ts)
         durations.append(times[-1])
@@ -349,6 +354,7 @@ def main():
         "max_subspace": msub if msub else 5 * k,
         "preconditioner": ("diagonal (the reference's)" if args.precond == 0 else
                            "diagonal with Olsen's correction"),
+        "select_rule": ("padded-width fill" if args.select_rule == 0 else "the reference's (unconverged roots)"),
         "switch_iter": last["stats"]["switch_iter"],
         "iterations_fp64": ref["stats"]["iterations"],
         "sigma_calls_mixed": last["stats"]["sigma_calls"],

@@ -301,6 +301,14 @@ typedef struct {
                             block and its projection V^T H V) runs as a captured hipGraph per
                             block shape, replayed; 1: always launched eagerly.  The results
                             are the same bit for bit */
+  int select_rule;    /* correction vectors added per iteration.  1: the reference's rule,
+                            one per root whose residual is not yet below the tolerance
+                            (PsociGAbasis.C:1371-1470); 0: the same, and the columns by
+                            which the sigma launch pads that block to its kernel width
+                            (1, 2, 4, 8, 16, 32) take the corrections of unconverged guard
+                            roots, then of converged wanted roots with the largest
+                            residuals (above 1e-3 x the tolerance): they cost no sigma
+                            time and cut iterations */
 } psoci_dav_params;
 
 typedef struct {

@@ -41,7 +41,7 @@ class DavParams(ctypes.Structure):
     _fields_ = [("nroots", ctypes.c_int), ("prec", ctypes.c_int), ("sigma_mode", ctypes.c_int),
                 ("max_iter", ctypes.c_int), ("max_subspace", ctypes.c_int), ("guess", ctypes.c_int),
                 ("tol", ctypes.c_double), ("switch_tol", ctypes.c_double), ("extra_roots", ctypes.c_int),
-                ("precond", ctypes.c_int), ("graph", ctypes.c_int)]
+                ("precond", ctypes.c_int), ("graph", ctypes.c_int), ("select_rule", ctypes.c_int)]
 
 
 class DavStats(ctypes.Structure):
@@ -314,7 +314,7 @@ class Hamiltonian:
     def davidson(self, nroots: int, prec: int = FP64, mode: int = STORED, tol: float = 1e-6,
                  switch_tol: float = 1e-4, max_iter: int = 200, max_subspace: int = 0, guess: int = 0,
                  want_vectors: bool = True, guess_vectors=None, extra_roots: int = 0,
-                 restart: str | None = None, precond: int = 0, graph: int = 0):
+                 restart: str | None = None, precond: int = 0, graph: int = 0, select_rule: int = 0):
         """Block Davidson for the nroots lowest roots.  guess: 0 lowest diagonal,
         1 first SEFs (the reference's), 2 guess_vectors ([n][nroots]);
         restart: a CI-vector file (civec_write) read as guess_vectors, missing
@@ -322,11 +322,14 @@ class Hamiltonian:
         extra_roots guard roots ride along in the block (not tested; with
         MIXED only in the FP32 phase); precond 0: the reference's diagonal
         preconditioner, 1: the same with Olsen's correction; graph 0:
-        stored-sigma steps as captured hipGraphs, 1: eager launches."""
+        stored-sigma steps as captured hipGraphs, 1: eager launches;
+        select_rule 0: the padded kernel width's free columns take more
+        corrections, 1: the reference's one per unconverged root."""
         if restart is not None:
             guess_vectors, _ = civec_read(restart, self.n, nroots)
             guess = 2
-        p = DavParams(nroots, prec, mode, max_iter, max_subspace, guess, tol, switch_tol, extra_roots, precond, graph)
+        p = DavParams(nroots, prec, mode, max_iter, max_subspace, guess, tol, switch_tol, extra_roots, precond, graph,
+                      select_rule)
         ev = np.zeros(nroots)
         vec = np.zeros((self.n, nroots)) if want_vectors else None
         if guess == 2:

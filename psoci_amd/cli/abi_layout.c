@@ -21,6 +21,7 @@ int main(void) {
   F(psoci_dav_params, extra_roots);
   F(psoci_dav_params, precond);
   F(psoci_dav_params, graph);
+  F(psoci_dav_params, select_rule);
   printf("psoci_dav_stats %zu\n", sizeof(psoci_dav_stats));
   F(psoci_dav_stats, iterations);
   F(psoci_dav_stats, sigma_calls);

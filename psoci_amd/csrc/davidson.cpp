@@ -60,6 +60,8 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
   PSOCI_CHECK(p.precond == 0 || p.precond == 1, PSOCI_ERR_ARG, "precond must be 0 or 1");
   PSOCI_CHECK(p.graph == 0 || p.graph == 1, PSOCI_ERR_ARG, "graph must be 0 or 1");
   const bool olsen = p.precond == 1;
+  PSOCI_CHECK(p.select_rule == 0 || p.select_rule == 1, PSOCI_ERR_ARG, "select_rule must be 0 or 1");
+  const bool fill = p.select_rule == 0;
   int mmax = p.max_subspace > 0 ? p.max_subspace : 5 * k;
   mmax = std::max(mmax, 3 * k);
   if (mmax > n) mmax = (int)n;
@@ -321,8 +323,30 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     }
     const double sel_tol = (phase32 && p.prec == PSOCI_MIXED) ? switch_tol : p.tol;
     std::vector<int> sel;
-    for (int j = 0; j < k; ++j)
-      if (rn[j] >= sel_tol) sel.push_back(j);
+    if (!fill) {
+      for (int j = 0; j < k; ++j)
+        if (rn[j] >= sel_tol) sel.push_back(j);
+    } else {
+      // the unconverged wanted roots first; the sigma launch pads the block
+      // to its kernel width anyway, so the free columns take the corrections
+      // of unconverged guard roots, then of the converged wanted roots with
+      // the largest residuals
+      for (int j = 0; j < kw && j < k; ++j)
+        if (rn[j] >= sel_tol) sel.push_back(j);
+      int width = 1;  // the kernel width the block is padded to (powers of two up to 32)
+      while (width < (int)sel.size()) width <<= 1;
+      if (sel.empty() || width > 32) width = (int)sel.size();
+      for (int j = kw; j < k && (int)sel.size() < width; ++j)
+        if (rn[j] >= sel_tol) sel.push_back(j);
+      std::vector<int> rest;
+      for (int j = 0; j < kw && j < k; ++j)
+        if (rn[j] < sel_tol && rn[j] > 1e-3 * sel_tol) rest.push_back(j);
+      std::stable_sort(rest.begin(), rest.end(), [&](int x, int y) { return rn[x] > rn[y]; });
+      for (int j : rest) {
+        if ((int)sel.size() >= width) break;
+        sel.push_back(j);
+      }
+    }
     int nsel = (int)sel.size();
     if (olsen) L.precondition_olsen(n, k, c.d_diag.p, w.data(), 1.0e-3, R.p, k, T.p, k);
     else L.precondition(n, k, c.d_diag.p, w.data(), 1.0e-3, R.p, k);
@@ -331,9 +355,9 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
       // the retained Ritz values and vectors are unchanged: w[0..k) still valid
     }
     // a space of fewer than 2k SEFs (mmax = n < kc + k) leaves fewer free
-    // columns than selected roots after the collapse: keep the corrections of
-    // the lowest roots (sel is ascending); none fit when the basis spans the
-    // whole space, whose Ritz pairs are then exact
+    // columns than selected roots after the collapse: keep the first
+    // corrections (the unconverged wanted roots, ascending, come first); none
+    // fit when the basis spans the whole space, whose Ritz pairs are then exact
     if (m + nsel > mmax) nsel = mmax - m;
     if (nsel <= 0) break;
     // new block = selected preconditioned residuals, placed at V[:, m:m+nsel)

@@ -153,6 +153,27 @@ def test_olsen_preconditioner_same_roots_fci10(gpu_fci10):
     assert e.value.code == api.ERR_ARG
 
 
+@pytest.mark.parametrize("mode", [api.STORED, api.DIRECT])
+def test_select_rules_same_roots_fci10(gpu_fci10, mode):
+    """select_rule 0 (the padded block width's free columns take the
+    corrections of guard roots and of converged wanted roots) converges to
+    the roots of the reference's rule 1 (one correction per unconverged
+    root), with and without guard roots, FP64 and mixed."""
+    H = gpu_fci10
+    ref = H.davidson(6, prec=api.FP64, mode=mode, tol=1e-9, select_rule=1, want_vectors=False)
+    assert ref["converged"]
+    for prec, extra in ((api.FP64, 0), (api.FP64, 3), (api.MIXED, 0), (api.MIXED, 3)):
+        a = H.davidson(6, prec=prec, mode=mode, tol=1e-7, select_rule=0, extra_roots=extra, want_vectors=False)
+        b = H.davidson(6, prec=prec, mode=mode, tol=1e-7, select_rule=1, extra_roots=extra, want_vectors=False)
+        assert a["converged"] and b["converged"]
+        tol = 1e-9 if prec == api.FP64 else 1e-8
+        np.testing.assert_allclose(a["evals"], ref["evals"], atol=tol, rtol=0)
+        np.testing.assert_allclose(b["evals"], ref["evals"], atol=tol, rtol=0)
+    with pytest.raises(api.PsociError) as e:
+        H.davidson(2, prec=api.FP64, mode=mode, select_rule=2)
+    assert e.value.code == api.ERR_ARG
+
+
 @pytest.mark.parametrize("prec", [api.FP64, api.MIXED])
 def test_graph_steps_equal_eager_steps_fci10(gpu_fci10, prec):
     """The stored-sigma steps captured as hipGraphs and replayed (many
