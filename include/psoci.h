@@ -233,6 +233,33 @@ int psoci_sigma_noalloc(psoci_ctx* ctx, int mode, int prec, int k, const void* X
  * (PsociGAhamiltonian.C:2822-2839). */
 int psoci_direct_info(psoci_ctx* ctx, int64_t* npairs, int64_t* nelems);
 
+/* Direct sigma engine.  request: -1 leaves the choice as it is; 0 (the
+ * default) the string-driven engine where the space qualifies, else the
+ * configuration-driven one; 1 the configuration-driven engine (pair records
+ * per connected configuration pair, elements regenerated per determinant
+ * pair: the reference's scheme, PsociGAhamiltonian.C:7549-7823); 2 the
+ * string-driven engine only.  The string-driven engine applies to spaces of
+ * one jsym in the EVEN determinant class with an even electron count whose
+ * configuration list is complete RAS: every occupation with at most m
+ * electrons in a set of external orbitals (full CI, MR-CISD); it evaluates
+ * the determinant Hamiltonian by resolutions of the identity over strings
+ * with one and two electrons fewer, as dense products on MFMA, and agrees
+ * with the configuration-driven sigma to rounding.  With engine != NULL the
+ * chosen engine's operator is built (when not yet) and *engine receives 1
+ * or 2; requesting 2 for a space that does not qualify fails with
+ * PSOCI_ERR_UNSUPPORTED and the reason in psoci_last_error, and leaves the
+ * previous choice in place.  No reference
+ * counterpart (the reference has one direct scheme). */
+int psoci_direct_engine(psoci_ctx* ctx, int request, int* engine);
+
+/* The string-driven engine's plan (after psoci_direct_engine selected it):
+ * external electron limit m, external orbital count, spin sectors, dense
+ * products, determinants evaluated, W elements per column, and the MFMA
+ * flops per column of one sigma (2 rows nq^2 summed over the products).
+ * PSOCI_ERR_STATE when the engine is not set up.  Any output may be NULL. */
+int psoci_direct_ras_info(psoci_ctx* ctx, int* m, int* n_external, int* nsectors, int* nblocks, int64_t* ndet,
+                          int64_t* w_per_column, double* flops_per_column);
+
 /* Connected determinant pairs one direct sigma visits: the (row
  * determinant, column determinant) pairs within two spin-orbital
  * replacements over every configuration pair the reference computes a block

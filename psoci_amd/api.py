@@ -104,6 +104,8 @@ def load(path: str | None = None):
         "psoci_get_transition": (I, [V, V, V, V]),
         "psoci_direct_info": (I, [V, V, V]),
         "psoci_direct_det_pairs": (I, [V, V]),
+        "psoci_direct_engine": (I, [V, I, V]),
+        "psoci_direct_ras_info": (I, [V, V, V, V, V, V, V, V]),
         "psoci_direct_schedule": (I, [V, V, V, V]),
         "psoci_direct_schedule_prec": (I, [V, I, V, V, V]),
         "psoci_stored_shape": (I, [V, I, V, V, V]),
@@ -384,6 +386,27 @@ class Hamiltonian:
         a, b = ctypes.c_int64(), ctypes.c_int64()
         self._check(self.L.psoci_direct_info(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def direct_engine(self, request: int = -1, build: bool = True):
+        """Direct sigma engine: request -1 keeps the choice, 0 auto (the
+        string-driven engine where the space qualifies), 1 the
+        configuration-driven records, 2 string-driven only.  Returns the
+        engine in use (1 or 2; the operator is built) or None with
+        build=False."""
+        e = ctypes.c_int()
+        self._check(self.L.psoci_direct_engine(self._h, request, ctypes.byref(e) if build else None))
+        return e.value if build else None
+
+    def direct_ras_info(self):
+        """The string-driven engine's plan: dict(m, n_external, sectors,
+        blocks, ndet, w_per_column, flops_per_column)."""
+        m, ne, ns, nb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        nd, wu, fl = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
+        self._check(self.L.psoci_direct_ras_info(self._h, ctypes.byref(m), ctypes.byref(ne), ctypes.byref(ns),
+                                                 ctypes.byref(nb), ctypes.byref(nd), ctypes.byref(wu),
+                                                 ctypes.byref(fl)))
+        return {"m": m.value, "n_external": ne.value, "sectors": ns.value, "blocks": nb.value, "ndet": nd.value,
+                "w_per_column": wu.value, "flops_per_column": fl.value}
 
     def direct_det_pairs(self) -> int:
         """Connected determinant pairs per direct sigma (counted once)."""

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpsoci_amd.so")
-SOURCES = ["space.cpp", "api.cpp", "davidson.cpp", "sifs.cpp", "configs.cpp", "restart.cpp", "nos.cpp", "hbuild.hip", "sigma.hip", "linalg.hip", "direct.hip", "rdm.hip", "analysis.hip", "noalloc.hip", "transition.hip"]
+SOURCES = ["space.cpp", "api.cpp", "davidson.cpp", "sifs.cpp", "configs.cpp", "restart.cpp", "nos.cpp", "hbuild.hip", "sigma.hip", "linalg.hip", "direct.hip", "rdm.hip", "analysis.hip", "noalloc.hip", "transition.hip", "ras.cpp", "strsig.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

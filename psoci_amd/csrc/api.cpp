@@ -99,6 +99,7 @@ int psoci_set_integrals(psoci_ctx* ctx, int nbf, const double* valone, const dou
     c.ecore = ecore;
     c.have_ints = true;
     c.have_h = c.have_diag = c.have_pairs = c.have_trans = false;
+    c.drop_ras();
     return PSOCI_OK;
   });
 }
@@ -116,6 +117,7 @@ int psoci_set_configs(psoci_ctx* ctx, int nbf, int nelec, int ksym, int64_t ncfg
                                      psoci::hash_bytes(occ, ncfg * nbf, psoci::hash_bytes(dims, sizeof(dims))));
     c.have_space = true;
     c.have_h = c.have_diag = c.have_pairs = c.have_trans = false;
+    c.drop_ras();
     return PSOCI_OK;
   });
 }
@@ -220,6 +222,45 @@ int psoci_direct_info(psoci_ctx* ctx, int64_t* npairs, int64_t* nelems) {
   });
 }
 
+int psoci_direct_engine(psoci_ctx* ctx, int request, int* engine) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(request >= -1 && request <= 2, PSOCI_ERR_ARG, "request must be -1, 0, 1 or 2");
+    const int before = c.direct_engine_req;
+    if (request >= 0) {
+      c.direct_engine_req = request;
+      if (request == 1 && c.have_ras) c.drop_ras();
+      if (request == 2 && !c.have_ras) c.ras_tried = false;
+    }
+    if (engine) {
+      PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
+      try {
+        psoci::build_direct(c);
+      } catch (...) {
+        c.direct_engine_req = before;  // a request that cannot be met leaves the choice as it was
+        throw;
+      }
+      *engine = psoci::direct_engine(c);
+    }
+    return PSOCI_OK;
+  });
+}
+
+int psoci_direct_ras_info(psoci_ctx* ctx, int* m, int* n_external, int* nsectors, int* nblocks, int64_t* ndet,
+                          int64_t* w_per_column, double* flops_per_column) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(c.have_ras, PSOCI_ERR_STATE, "the string-driven direct sigma is not set up");
+    const psoci::RasPlan& P = c.ras;
+    if (m) *m = P.m;
+    if (n_external) *n_external = psoci::popc(P.emask);
+    if (nsectors) *nsectors = (int)P.sec.size();
+    if (nblocks) *nblocks = (int)P.blk.size();
+    if (ndet) *ndet = P.ndet;
+    if (w_per_column) *w_per_column = P.wunit;
+    if (flops_per_column) *flops_per_column = P.gemm_flops_per_col;
+    return PSOCI_OK;
+  });
+}
+
 int psoci_direct_det_pairs(psoci_ctx* ctx, int64_t* det_pairs) {
   return guarded(ctx, [&](Ctx& c) {
     PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");
@@ -300,7 +341,7 @@ int psoci_sigma(psoci_ctx* ctx, int mode, int prec, int k, const void* X, void* 
     PSOCI_CHECK(prec == PSOCI_FP64 || prec == PSOCI_FP32, PSOCI_ERR_ARG, "sigma precision must be FP64 or FP32");
     PSOCI_CHECK(mode == PSOCI_SIGMA_STORED || mode == PSOCI_SIGMA_DIRECT, PSOCI_ERR_ARG, "bad sigma mode");
     if (mode == PSOCI_SIGMA_STORED) PSOCI_CHECK(c.have_h, PSOCI_ERR_STATE, "stored sigma needs psoci_build_h first");
-    if (mode == PSOCI_SIGMA_DIRECT && !c.have_pairs) psoci::build_pairs(c);
+    if (mode == PSOCI_SIGMA_DIRECT && psoci::direct_needs_build(c)) psoci::build_direct(c);
     const int64_t n = c.sp.nsef_total;
     const size_t es = prec == PSOCI_FP32 ? 4 : 8;
     const void* dx = X;
@@ -329,8 +370,9 @@ int psoci_sigma_noalloc(psoci_ctx* ctx, int mode, int prec, int k, const void* X
     // building the operator allocates: it must exist already
     if (mode == PSOCI_SIGMA_STORED) PSOCI_CHECK(c.have_h, PSOCI_ERR_STATE, "stored sigma needs psoci_build_h first");
     if (mode == PSOCI_SIGMA_DIRECT)
-      PSOCI_CHECK(c.have_pairs, PSOCI_ERR_STATE,
-                  "the direct sigma's pair records are not built (psoci_direct_info or psoci_sigma builds them)");
+      PSOCI_CHECK(psoci::direct_engine(c), PSOCI_ERR_STATE,
+                  "the direct sigma's operator is not built (psoci_direct_engine, psoci_direct_info or psoci_sigma "
+                  "builds it)");
     psoci::Staging st;
     psoci::noalloc_scratch(c.device, &st.p, &st.bytes);
     const size_t need = psoci::sigma_staging_bytes(c, mode, prec, k, X, k, Y, k);

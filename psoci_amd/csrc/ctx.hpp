@@ -2,6 +2,7 @@
 #pragma once
 #include "common.hpp"
 #include "elem.hpp"
+#include "ras.hpp"
 
 namespace psoci {
 
@@ -91,6 +92,42 @@ struct Ctx {
   DevBuf<uint64_t> d_detao;   // ndet_total
   DevBuf<int16_t> d_didx;     // determinant index by open-shell pattern (Space::didx)
 
+  // direct sigma engine: requested (0 auto: the string-driven one where the
+  // space qualifies, else the configuration records; 1 configuration
+  // records; 2 string-driven only) and the string-driven plan (strsig.hip)
+  int direct_engine_req = 0;
+  bool have_ras = false, ras_tried = false;
+  std::string ras_why;
+  RasPlan ras;
+  DevBuf<uint64_t> d_rasmask;
+  DevBuf<int8_t> d_rascls;
+  DevBuf<uint32_t> d_rasadd;
+  DevBuf<int32_t> d_rasrem;
+  DevBuf<RasBlockDev> d_rasblk;
+  DevBuf<int16_t> d_rasq, d_raspidx;
+  DevBuf<RasSef> d_rassef;
+  DevBuf<double> d_rasg64;
+  DevBuf<float> d_rasg32;
+  DevBuf<RasView> d_rasview;
+  DevBuf<char> d_rasc, d_rasw;  // determinant vectors [ndet][16 FP32 | 8 FP64], W [wunit][16 | 8]
+  void drop_ras() {
+    have_ras = ras_tried = false;
+    ras = RasPlan();
+    d_rasmask.release();
+    d_rascls.release();
+    d_rasadd.release();
+    d_rasrem.release();
+    d_rasblk.release();
+    d_rasq.release();
+    d_raspidx.release();
+    d_rassef.release();
+    d_rasg64.release();
+    d_rasg32.release();
+    d_rasview.release();
+    d_rasc.release();
+    d_rasw.release();
+  }
+
   // transition Hamiltonian (rows: this list's SEFs; columns: a second
   // list's SEFs placed at this list's entry points), lower by index
   bool have_trans = false;
@@ -122,7 +159,27 @@ void compute_diag(Ctx& c);              // diagonal only
 void block_on_device(Ctx& c, int64_t I, int64_t J, double* host_out, int* status);
 void sigma_stored(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy, const Staging& st = Staging{});
 void sigma_direct(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy, const Staging& st = Staging{});
-int64_t direct_det_pairs(Ctx& c);  // connected determinant pairs per direct sigma (one counting launch, cached)
+int64_t direct_det_pairs(Ctx& c);
+// string-driven direct sigma (strsig.hip): the plan and its device tables
+// (false when the space does not qualify; c.ras_why says why), and the sigma
+bool build_ras(Ctx& c);
+void sigma_ras(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy);
+// the direct sigma's operator: the string-driven plan where requested and
+// possible, else the configuration pair records (throws when engine 2 is
+// requested and the space does not qualify)
+void build_direct(Ctx& c);
+// the engine a direct sigma uses with what is built now (0: none yet)
+inline int direct_engine(const Ctx& c) {
+  if (c.direct_engine_req != 1 && c.have_ras) return 2;
+  if (c.direct_engine_req != 2 && c.have_pairs) return 1;
+  return 0;
+}
+// whether build_direct would build something (the string plan not tried yet
+// where it may apply, or no operator at all)
+inline bool direct_needs_build(const Ctx& c) {
+  if (c.direct_engine_req != 1 && !c.have_ras && !c.ras_tried) return true;
+  return direct_engine(c) == 0;
+}  // connected determinant pairs per direct sigma (one counting launch, cached)
 // staging bytes a sigma call of k columns at strides ldx, ldy needs (0: none)
 size_t sigma_staging_bytes(const Ctx& c, int mode, int prec, int k, const void* X, int ldx, const void* Y, int ldy);
 // the allocation-free scratch array of the current device (noalloc.hip):

@@ -69,7 +69,7 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
   const int maxit = p.max_iter > 0 ? p.max_iter : 200;
   if (p.sigma_mode == PSOCI_SIGMA_STORED)
     PSOCI_CHECK(c.have_h, PSOCI_ERR_STATE, "stored sigma needs psoci_build_h first");
-  if (p.sigma_mode == PSOCI_SIGMA_DIRECT && !c.have_pairs) build_pairs(c);
+  if (p.sigma_mode == PSOCI_SIGMA_DIRECT && direct_needs_build(c)) build_direct(c);
   if (!c.have_diag) compute_diag(c);
 
   Linalg L;

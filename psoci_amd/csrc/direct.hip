@@ -723,9 +723,11 @@ int64_t direct_det_pairs(Ctx& c) {
 }
 
 void sigma_direct(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy, const Staging& st) {
-  if (!c.have_pairs) {
-    PSOCI_CHECK(!st.p, PSOCI_ERR_STATE, "the direct sigma's pair records are not built");
-    build_pairs(c);
+  if (st.p) PSOCI_CHECK(direct_engine(c), PSOCI_ERR_STATE, "the direct sigma's operator is not built");
+  else if (direct_needs_build(c)) build_direct(c);
+  if (direct_engine(c) == 2) {
+    sigma_ras(c, prec, k, X, ldx, Y, ldy);
+    return;
   }
   if (prec == PSOCI_FP32)
     direct_any_k<float>(c, k, static_cast<const float*>(X), ldx, static_cast<float*>(Y), ldy, st);

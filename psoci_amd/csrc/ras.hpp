@@ -1,0 +1,267 @@
+// String-driven direct sigma for complete RAS spaces (host plan and the
+// index arithmetic shared by the GPU kernels and the host check).
+//
+// A configuration list that holds every occupation of N electrons with at
+// most m electrons in a set E of "external" orbitals (and any occupation of
+// the others) -- a full CI (E empty) or an MR-CISD list such as the
+// reference's benchmark spaces -- has a determinant space that factors into
+// alpha and beta strings: within each spin sector (n_alpha, n_beta), every
+// (alpha string, beta string) pair with ext(alpha) + ext(beta) <= m is a
+// determinant of exactly one configuration.  The reference's determinants
+// are (-1)^phase times the canonical product (alpha orbitals ascending, then
+// beta ascending), its SEFs the Kramers combinations (D_2k + mult
+// D_2k+1)/sqrt(2) (PsociHamiltonian.C:1304-1322, 1350-1454), so the SEF
+// sigma is C^T H_det C x with H_det the spin-free Hamiltonian over canonical
+// determinants.
+//
+// H_det x is evaluated by resolutions of the identity over strings with
+// fewer electrons, which turns element generation into dense products:
+//   alpha-beta: sum_pqrs (pq|rs) E^a_pq E^b_rs through (N-1, N-1) string
+//     pairs (ka, kb): D[(ka,kb)][(q,s)] = eps eps c(ka+q, kb+s),
+//     W = D G^T with G[(p,r),(q,s)] = (pq|rs) (+ the one-electron integrals
+//     folded in as h_pq delta_rs / n_b + h_rs delta_pq / n_a),
+//     sigma(ka+p, kb+r) += eps eps W[(ka,kb)][(p,r)];
+//   same spin: 1/2 sum (pq|rs) a+_p a+_r a_s a_q through (N-2) strings k2 of
+//     that spin and the other spin's N strings, pairs p > r, q > s,
+//     V[(p,r),(q,s)] = (pq|rs) - (ps|rq) (one-electron integrals folded in
+//     here when the other spin has no electrons).
+// The pair sets are restricted per class of the row strings (their external
+// electron counts) to the pairs that keep ext <= m, so every product is a
+// dense GEMM over the class pair.  Only sectors with n_alpha >= n_beta are
+// evaluated: a sector with n_alpha > n_beta is the spin flip of its mirror,
+// whose sigma is mult times its own.
+//
+// The reference keeps no such structure (it regenerates every block with
+// generateSOCIblocks, PsociHamiltonian.C:401-558); the plan is used only
+// where it is exact: EVEN-class spaces of one jsym with an even electron
+// count, and no configuration pair whose single-excitation sum vnt[0] is
+// exactly zero while its exchange terms are not (the reference skips such a
+// block, processSingleExcitation PsociHamiltonian.C:190-261; checked on the
+// GPU at setup).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "elem.hpp"
+
+namespace psoci {
+
+constexpr int kRasMaxCls = 7;    // external-electron classes 0..6
+constexpr int kRasMaxN = 32;     // electrons
+constexpr uint32_t kRasNone = 0xffffffffu;
+
+// add-table entry: sign << 31 | class << 28 | rank within the class
+PSOCI_HD uint32_t ras_pack(int sign_neg, int cls, int64_t rank) {
+  return ((uint32_t)sign_neg << 31) | ((uint32_t)cls << 28) | (uint32_t)rank;
+}
+PSOCI_HD int ras_cls(uint32_t e) { return (int)((e >> 28) & 7u); }
+PSOCI_HD int64_t ras_rank(uint32_t e) { return (int64_t)(e & 0x0fffffffu); }
+PSOCI_HD int ras_neg(uint32_t e) { return (int)(e >> 31); }
+
+struct RasLevelDev {
+  int64_t mask_off;                 // first string of the level in mask[] / cls[]
+  int64_t add_off;                  // add[]: [count][nbf]
+  int64_t rem_off;                  // rem[]: [count][n]
+  int64_t cls_off[kRasMaxCls + 1];  // global index (within the level) of each class's first string
+};
+
+struct RasSectorDev {
+  int na, nb;
+  int64_t det0;                          // first determinant of the sector
+  int64_t boff[kRasMaxCls][kRasMaxCls];  // block (alpha class, beta class) offset in the sector, -1 none
+};
+
+// One dense product: rows (u, w), u of level lu and class cu, w of level lw
+// and class cw (global indices u0 + row / nw, w0 + row % nw); pair list of
+// shape (0: ordered (q, s); 1: q > s) and budget (external electrons the
+// pairs may add), nq pairs padded to nqp.
+struct RasBlockDev {
+  int kind;  // 0 alpha-beta, 1 alpha-alpha, 2 beta-beta
+  int sector;
+  int lu, lw, cu, cw;
+  int budget, nq, nqp;
+  int pad0;
+  int64_t u0, nu, w0, nw, nrows;
+  int64_t qoff;   // first pair in qlist[]
+  int64_t goff;   // first element of G (nqp x nqp, row-major [p-pair][q-pair])
+  int64_t wbase;  // W offset per column: W element (wbase + row * nqp + pidx) * KC + v
+};
+
+// The determinants of one SEF (one or two): alpha / beta global string
+// indices, sector, and flags (bit 0 negative coefficient, bit 1 closed
+// shell: coefficient +-1 instead of +-1/sqrt(2)); n = 1 or 2.
+struct RasSef {
+  int32_t a[2], b[2];
+  int8_t sec[2];
+  uint8_t flag[2];
+  int32_t n;
+};
+
+struct RasView {
+  int nbf, m, nsec, nblk;
+  const uint64_t* mask;
+  const int8_t* cls;
+  const uint32_t* add;
+  const int32_t* rem;
+  RasLevelDev lev[kRasMaxN + 1];
+  RasSectorDev sec[kRasMaxN / 4 + 1];
+  const RasBlockDev* blk;
+  const int16_t* qlist;   // q | s << 8, -1 padding
+  const int16_t* pidx;    // [shape][budget][nbf][nbf] position in the pair list, -1 none
+  int32_t blockid[kRasMaxN / 4 + 1][3][kRasMaxCls][kRasMaxCls];  // -1 none
+  const RasSef* sef;
+  int64_t nsef;
+  int64_t ndet, wunit;  // determinants and W elements per column (bounds of every index)
+};
+
+PSOCI_HD int64_t ras_det(const RasView& v, int s, int ca, int64_t ra, int cb, int64_t rb) {
+  const RasSectorDev& S = v.sec[s];
+  const int64_t nbw = v.lev[S.nb].cls_off[cb + 1] - v.lev[S.nb].cls_off[cb];
+  return S.det0 + S.boff[ca][cb] + ra * nbw + rb;
+}
+
+// class of global string index g of level n
+PSOCI_HD int ras_cls_of(const RasView& v, int n, int64_t g) { return v.cls[v.lev[n].mask_off + g]; }
+
+// the determinant (and sign) of D-entry (row, pair j) of block B: -1 when
+// the pair cannot be added to the row strings
+PSOCI_HD int64_t ras_d_entry(const RasView& v, const RasBlockDev& B, int64_t row, int j, int& neg) {
+  const int16_t qs = v.qlist[B.qoff + j];
+  if (qs < 0) return -1;
+  const int q = qs & 0xff, s = (qs >> 8) & 0xff;
+  const int64_t u = B.u0 + row / B.nw, w = B.w0 + row % B.nw;
+  const RasSectorDev& S = v.sec[B.sector];
+  if (B.kind == 0) {
+    const uint32_t ea = v.add[v.lev[B.lu].add_off + u * v.nbf + q];
+    const uint32_t eb = v.add[v.lev[B.lw].add_off + w * v.nbf + s];
+    if (ea == kRasNone || eb == kRasNone) return -1;
+    neg = ras_neg(ea) ^ ras_neg(eb);
+    return ras_det(v, B.sector, ras_cls(ea), ras_rank(ea), ras_cls(eb), ras_rank(eb));
+  }
+  // same spin: add s, then q (q > s), to the N-2 string; the other spin's
+  // string is the row's own
+  const int l2 = B.kind == 1 ? B.lu : B.lw;
+  const int64_t k2 = B.kind == 1 ? u : w;
+  const uint32_t e1 = v.add[v.lev[l2].add_off + k2 * v.nbf + s];
+  if (e1 == kRasNone) return -1;
+  const int64_t g1 = v.lev[l2 + 1].cls_off[ras_cls(e1)] + ras_rank(e1);
+  const uint32_t e2 = v.add[v.lev[l2 + 1].add_off + g1 * v.nbf + q];
+  if (e2 == kRasNone) return -1;
+  neg = ras_neg(e1) ^ ras_neg(e2);
+  if (B.kind == 1) {
+    const int cb = ras_cls_of(v, S.nb, w);
+    return ras_det(v, B.sector, ras_cls(e2), ras_rank(e2), cb, w - v.lev[S.nb].cls_off[cb]);
+  }
+  const int ca = ras_cls_of(v, S.na, u);
+  return ras_det(v, B.sector, ca, u - v.lev[S.na].cls_off[ca], ras_cls(e2), ras_rank(e2));
+}
+
+// Calls f(block index, W element offset per column (wbase + row * nqp +
+// pidx), negative sign) for every term of sigma at determinant (alpha
+// string a, beta string b) of sector s: alpha-beta removals (p in a, r in
+// b), then alpha-alpha and beta-beta pairs (p > r).  A term without a block
+// or pair position (never, for a determinant of the plan) is passed as
+// f(-1, -1, 0).
+template <class F>
+PSOCI_HD void ras_terms(const RasView& v, int s, int64_t a, int64_t b, F&& f) {
+  const RasSectorDev& S = v.sec[s];
+  const uint64_t am = v.mask[v.lev[S.na].mask_off + a];
+  const uint64_t bm = v.mask[v.lev[S.nb].mask_off + b];
+  const int ca = ras_cls_of(v, S.na, a), cb = ras_cls_of(v, S.nb, b);
+  const int nbf = v.nbf;
+  const int nb2 = nbf * nbf;
+  if (S.na >= 1 && S.nb >= 1) {
+    int tp = 0;
+    for (uint64_t x = am; x; x &= x - 1, ++tp) {
+      const int p = ctz(x);
+      const int64_t u = v.rem[v.lev[S.na].rem_off + a * S.na + tp];
+      const int cu = ras_cls_of(v, S.na - 1, u);
+      int tr = 0;
+      for (uint64_t y = bm; y; y &= y - 1, ++tr) {
+        const int r = ctz(y);
+        const int64_t w = v.rem[v.lev[S.nb].rem_off + b * S.nb + tr];
+        const int cw = ras_cls_of(v, S.nb - 1, w);
+        const int bi = v.blockid[s][0][cu][cw];
+        if (bi < 0) {
+          f(-1, (int64_t)-1, 0);
+          continue;
+        }
+        const RasBlockDev& B = v.blk[bi];
+        const int pi = v.pidx[(0 * (v.m + 1) + B.budget) * nb2 + p * nbf + r];
+        if (pi < 0) {
+          f(-1, (int64_t)-1, 0);
+          continue;
+        }
+        const int64_t row = (u - B.u0) * B.nw + (w - B.w0);
+        f(bi, B.wbase + row * B.nqp + pi, (tp + tr) & 1);
+      }
+    }
+  }
+  for (int spin = 0; spin < 2; ++spin) {
+    const int n = spin == 0 ? S.na : S.nb;
+    if (n < 2) continue;
+    const uint64_t mk = spin == 0 ? am : bm;
+    const int64_t g = spin == 0 ? a : b;
+    const int64_t o = spin == 0 ? b : a;
+    const int co = spin == 0 ? cb : ca;
+    int tp = 0;
+    for (uint64_t x = mk; x; x &= x - 1, ++tp) {
+      const int p = ctz(x);
+      const int64_t u1 = v.rem[v.lev[n].rem_off + g * n + tp];
+      int tr = 0;
+      for (uint64_t y = mk & ((1ull << p) - 1ull); y; y &= y - 1, ++tr) {
+        const int r = ctz(y);
+        const int64_t k2 = v.rem[v.lev[n - 1].rem_off + u1 * (n - 1) + tr];
+        const int ck = ras_cls_of(v, n - 2, k2);
+        const int bi = spin == 0 ? v.blockid[s][1][ck][co] : v.blockid[s][2][co][ck];
+        if (bi < 0) {
+          f(-1, (int64_t)-1, 0);
+          continue;
+        }
+        const RasBlockDev& B = v.blk[bi];
+        const int pi = v.pidx[(1 * (v.m + 1) + B.budget) * nb2 + p * nbf + r];
+        if (pi < 0) {
+          f(-1, (int64_t)-1, 0);
+          continue;
+        }
+        const int64_t row = spin == 0 ? (k2 - B.u0) * B.nw + (o - B.w0) : (o - B.u0) * B.nw + (k2 - B.w0);
+        f(bi, B.wbase + row * B.nqp + pi, (tp + tr) & 1);
+      }
+    }
+  }
+}
+
+// Host plan: the tables above in host memory, the per-block G matrices
+// (FP64), and the sizes the device buffers need.
+struct RasPlan {
+  bool ok = false;
+  std::string why;  // when !ok: why the space does not qualify
+  int nbf = 0, N = 0, m = 0;
+  uint64_t emask = 0;
+  double mult = 1.0;
+  std::vector<uint64_t> mask;
+  std::vector<int8_t> cls;
+  std::vector<uint32_t> add;
+  std::vector<int32_t> rem;
+  std::vector<RasLevelDev> lev;  // kRasMaxN + 1
+  std::vector<int> lev_cnt_n;    // electrons per built level (diagnostics)
+  std::vector<RasSectorDev> sec;
+  std::vector<RasBlockDev> blk;
+  std::vector<int16_t> qlist, pidx;
+  std::vector<double> G;
+  std::vector<int32_t> blockid;  // flattened [sector][kind][cu][cw]
+  std::vector<RasSef> sef;
+  int64_t ndet = 0;   // evaluated determinants (all sectors with n_alpha >= n_beta)
+  int64_t wunit = 0;  // W elements per column
+  double gemm_flops_per_col = 0.0;  // 2 * sum over blocks of rows * nq^2
+};
+
+// Builds the plan for the space and integrals; plan.ok = false (with a
+// reason) when the space is not a complete RAS space of the supported kind.
+struct Space;
+void ras_plan(RasPlan& P, const Space& sp, const double* one, const double* two);
+// A view of the plan's host tables (host pointers), e.g. for a host check.
+RasView ras_host_view(const RasPlan& P);
+
+}  // namespace psoci

@@ -284,6 +284,7 @@ size_t sigma_staging_bytes(const Ctx& c, int mode, int prec, int k, const void* 
   // the stored sigma has widths 1..32 (powers of two), the direct one also 24
   const bool w24 = mode == PSOCI_SIGMA_DIRECT;
   const int64_t n = c.sp.nsef_total;
+  if (mode == PSOCI_SIGMA_DIRECT && direct_engine(c) == 2) return 0;  // reads X and writes Y at their strides
   if (prec == PSOCI_FP32)
     return pad::staging_bytes<float>(n, k, 32, static_cast<const float*>(X), ldx, static_cast<const float*>(Y), ldy,
                                      w24);

@@ -1,0 +1,114 @@
+// Test harness: the string-driven direct sigma's host plan (ras.cpp) and the
+// index arithmetic its GPU kernels use (ras.hpp: ras_d_entry, ras_terms),
+// evaluated with plain loops on the host CPU, so the tables, signs and SEF
+// map can be compared with the oracle on machines without a GPU.  Not part
+// of the product library.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../psoci_amd/csrc/common.hpp"
+#include "../../psoci_amd/csrc/ras.hpp"
+
+using namespace psoci;
+
+extern "C" {
+
+// Plan summary: returns 1 when the space qualifies (0 otherwise, with the
+// reason in why[256]); info[0..5] = m, external orbital count, sectors,
+// blocks, determinants, W elements per column.
+int hr_plan(int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, const int32_t* jsym, const double* one,
+            const double* two, int64_t* info, char* why) {
+  Space sp;
+  RasPlan P;
+  try {
+    build_space(sp, nbf, nelec, ksym, ncfg, occ, jsym);
+    ras_plan(P, sp, one, two);
+  } catch (const Error& e) {
+    std::strncpy(why, e.what(), 255);
+    return -1;
+  }
+  std::strncpy(why, P.why.c_str(), 255);
+  if (!P.ok) return 0;
+  info[0] = P.m;
+  info[1] = popc(P.emask);
+  info[2] = (int64_t)P.sec.size();
+  info[3] = (int64_t)P.blk.size();
+  info[4] = P.ndet;
+  info[5] = P.wunit;
+  return 1;
+}
+
+// Y[n][k] = H X[n][k] by the plan, on the host; returns 0 on success.
+int hr_sigma(int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, const int32_t* jsym, const double* one,
+             const double* two, int k, const double* X, double* Y) {
+  Space sp;
+  RasPlan P;
+  try {
+    build_space(sp, nbf, nelec, ksym, ncfg, occ, jsym);
+    ras_plan(P, sp, one, two);
+  } catch (const Error&) {
+    return -1;
+  }
+  if (!P.ok) return 1;
+  const RasView v = ras_host_view(P);
+  const double r = 0x1.6a09e667f3bcdp-1;
+  auto coef = [&](uint8_t f) { return ((f & 2) ? 1.0 : r) * ((f & 1) ? -1.0 : 1.0); };
+  std::vector<double> c((size_t)P.ndet * k, 0.0);
+  std::vector<char> seen(P.ndet, 0);
+  for (int64_t s = 0; s < v.nsef; ++s) {
+    const RasSef& e = v.sef[s];
+    for (int t = 0; t < e.n; ++t) {
+      const RasSectorDev& S = v.sec[e.sec[t]];
+      const int ca = ras_cls_of(v, S.na, e.a[t]), cb = ras_cls_of(v, S.nb, e.b[t]);
+      const int64_t d = ras_det(v, e.sec[t], ca, e.a[t] - v.lev[S.na].cls_off[ca], cb, e.b[t] - v.lev[S.nb].cls_off[cb]);
+      if (d < 0 || d >= P.ndet || seen[d]) return 2;
+      seen[d] = 1;
+      for (int j = 0; j < k; ++j) c[d * k + j] = coef(e.flag[t]) * X[s * k + j];
+    }
+  }
+  for (int64_t d = 0; d < P.ndet; ++d)
+    if (!seen[d]) return 3;
+  std::vector<double> W((size_t)P.wunit * k, 0.0);
+  std::vector<double> D;
+  for (int bi = 0; bi < v.nblk; ++bi) {
+    const RasBlockDev& B = v.blk[bi];
+    const double* G = P.G.data() + B.goff;
+    D.assign((size_t)B.nqp * k, 0.0);
+    for (int64_t row = 0; row < B.nrows; ++row) {
+      for (int j = 0; j < B.nqp; ++j) {
+        int neg = 0;
+        const int64_t d = ras_d_entry(v, B, row, j, neg);
+        if (d >= P.ndet) return 4;
+        for (int q = 0; q < k; ++q) D[(size_t)j * k + q] = d < 0 ? 0.0 : (neg ? -c[d * k + q] : c[d * k + q]);
+      }
+      for (int i = 0; i < B.nqp; ++i)
+        for (int q = 0; q < k; ++q) {
+          double acc = 0.0;
+          for (int j = 0; j < B.nqp; ++j) acc += G[(int64_t)i * B.nqp + j] * D[(size_t)j * k + q];
+          W[(B.wbase + row * B.nqp + i) * k + q] = acc;
+        }
+    }
+  }
+  for (int64_t s = 0; s < v.nsef; ++s) {
+    const RasSef& e = v.sef[s];
+    for (int q = 0; q < k; ++q) Y[s * k + q] = 0.0;
+    for (int t = 0; t < e.n; ++t) {
+      const double w = coef(e.flag[t]) * (e.sec[t] > 0 ? 2.0 : 1.0);
+      std::vector<double> sig(k, 0.0);
+      bool bad = false;
+      ras_terms(v, e.sec[t], e.a[t], e.b[t], [&](int bi, int64_t off, int neg) {
+        if (bi < 0 || bi >= v.nblk || off < 0 || off >= P.wunit) {
+          bad = true;
+          return;
+        }
+        for (int q = 0; q < k; ++q) sig[q] += neg ? -W[off * k + q] : W[off * k + q];
+      });
+      if (bad) return 5;
+      for (int q = 0; q < k; ++q) Y[s * k + q] += w * sig[q];
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"

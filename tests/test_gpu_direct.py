@@ -1,7 +1,8 @@
-"""Direct sigma (elements regenerated on the fly) through the C ABI: against
+"""Direct sigma (elements regenerated on the fly) through the C ABI, both
+engines (1: configuration pair records, 2: string-driven products): against
 the stored Hamiltonian on the FCI spaces, and against the oracle's own
-configuration-pair blocks, row by row, on the 92,680-configuration MR-CISD
-space (sampled rows: a size-independent check)."""
+configuration-pair blocks, row by row, on the MR-CISD spaces (sampled rows:
+a size-independent check)."""
 import numpy as np
 import pytest
 
@@ -36,11 +37,12 @@ def oracle_rows(sp, cfg, I, X):
     return y
 
 
-@pytest.fixture(scope="module")
-def fci10_gpu(fci10):
+@pytest.fixture(scope="module", params=[1, 2], ids=["records", "strings"])
+def fci10_gpu(fci10, request):
     cfg, ints, _ = fci10
     H = api.Hamiltonian(cfg, ints)
     H.build()
+    assert H.direct_engine(request.param) == request.param
     yield H
     H.close()
 
@@ -66,12 +68,14 @@ def test_direct_davidson_matches_stored_fci10(fci10_gpu):
     np.testing.assert_allclose(b["evals"], a["evals"], atol=1e-9, rtol=0)
 
 
+@pytest.mark.parametrize("engine", [1, 2])
 @pytest.mark.parametrize("k", [8, 24])
-def test_direct_rows_match_oracle_mrci8x24(k):
-    """k = 24 is the width of the FP32 phase of the benchmark solve (16 roots
-    + 8 guard roots)."""
+def test_direct_rows_match_oracle_mrci8x24(k, engine):
+    """k = 24 is the width of an FP32 solve phase with 16 roots and 8 guard
+    roots."""
     cfg, ints, _ = inputs.workload("mrci8x24")
     H = api.Hamiltonian(cfg, ints)
+    assert H.direct_engine(engine) == engine
     sp = oracle.OracleSpace(cfg, ints)
     rng = np.random.default_rng(3)
     X = rng.standard_normal((H.n, k))
@@ -126,12 +130,14 @@ def test_direct_mixed_guard_roots_mrci8x24():
     H.close()
 
 
-def test_direct_full_size_mrci10x24():
+@pytest.mark.parametrize("engine", [1, 2])
+def test_direct_full_size_mrci10x24(engine):
     """The 47.6M-determinant workload at full size: sampled rows of the FP64
     and FP32 direct sigma against the oracle's blocks, and the symmetry of
     the operator (u^T H v = v^T H u) over all 11,894,370 SEFs."""
     cfg, ints, _ = inputs.workload("mrci10x24")
     H = api.Hamiltonian(cfg, ints)
+    assert H.direct_engine(engine) == engine
     sp = oracle.OracleSpace(cfg, ints)
     rng = np.random.default_rng(17)
     X = rng.standard_normal((H.n, 4))

@@ -41,12 +41,13 @@ def run_pair(H, mode, k, prec, seed):
     assert torch.equal(Ya, Yb), (mode, k, prec)
 
 
-@pytest.mark.parametrize("mode", [api.STORED, api.DIRECT])
-def test_noalloc_equals_sigma_fci10(fci10, mode):
+@pytest.mark.parametrize("mode,engine", [(api.STORED, 0), (api.DIRECT, 1), (api.DIRECT, 2)])
+def test_noalloc_equals_sigma_fci10(fci10, mode, engine):
     cfg, ints, _ = fci10
     H = api.Hamiltonian(cfg, ints)
     H.build()
-    H.direct_info()  # builds the direct sigma's pair records
+    if mode == api.DIRECT:
+        assert H.direct_engine(engine) == engine  # builds the direct sigma's operator
     # staged blocks (12 -> width 16, then 3 -> width 4), both precisions,
     # then an in-place block of a kernel width
     run_pair(H, mode, 12, api.FP64, 1)
