@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}  # MI355X dense MFMA
 METRIC = "time to 16 roots at 1e-8 Eh, FP32-sigma vs FP64 on GPU; sigma HBM GB/s"
 
 
@@ -42,6 +43,9 @@ def parse():
                          "only if the longest one so far still fits, with the tail (sigma timings, CPU leg) "
                          "reserved; at least one timed solve always runs")
     ap.add_argument("--workload", default="mrci10x24")
+    ap.add_argument("--engine", type=int, default=0,
+                    help="direct sigma engine: 0 auto (string-driven where the space is complete RAS), "
+                         "1 configuration pair records, 2 string-driven")
     ap.add_argument("--roots", type=int, default=16)
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--guard-roots", type=int, default=0,
@@ -219,8 +223,13 @@ def main():
     os.remove(sifs_path)
     os.remove(cfg_path)
     os.rmdir(in_dir)
+    engine, ras = None, None
     if direct:
-        npairs, nelems = H.direct_info()
+        engine = H.direct_engine(args.engine)
+        if engine == 1:
+            npairs, nelems = H.direct_info()
+        else:
+            ras = H.direct_ras_info()
         nnz = None
     else:
         nnz = H.build()
@@ -307,6 +316,9 @@ def main():
         bytes64 = k * n * (8 + 8) + 16 * cfg.ncfg + 8 * n2
         kernel = (f"k_direct<float,{kr},true,false> (direct FP32 sigma, {kr} vectors: "
                   f"{k} roots + {guard} guard roots)")
+        if engine == 2:
+            t32 = H.direct_ras_timing(api.FP32, kr, 3)
+            t64 = H.direct_ras_timing(api.FP64, k, 3)
     else:
         bytes_r = nnz * (4 + 4) + (n + 1) * 8 + kr * n * (4 + 4)
         bytes64 = nnz * (8 + 4) + (n + 1) * 8 + k * n * (8 + 8)
@@ -322,9 +334,14 @@ def main():
         else:
             cpu = cpu_stored(args, cfg, ints, k, ref)
 
-    desc = (f"{args.workload}: PSOCI SEF space n={n}, ncfg={cfg.ncfg}, "
-            + (f"direct sigma ({npairs} ordered configuration pairs, {nelems} structural elements per sigma)"
-               if direct else f"stored H nnz_lower={nnz}")
+    if direct and engine == 2:
+        sig_desc = (f"direct sigma, string-driven engine ({ras['ndet']} determinants in {ras['sectors']} spin "
+                    f"sectors, {ras['blocks']} MFMA products, {ras['flops_per_column']:.4g} flops per column)")
+    elif direct:
+        sig_desc = f"direct sigma ({npairs} ordered configuration pairs, {nelems} structural elements per sigma)"
+    else:
+        sig_desc = f"stored H nnz_lower={nnz}"
+    desc = (f"{args.workload}: PSOCI SEF space n={n}, ncfg={cfg.ncfg}, " + sig_desc
             + f", {k} roots, mixed-precision block Davidson, residual tol {args.tol:g}")
     line = {
         "metric": METRIC,
@@ -378,7 +395,36 @@ def main():
         line["roofline"]["traffic_gbs"] = round(traffic["bytes_per_launch"] / (msr * 1e-3) / 1e9, 1)
     if not direct:
         line["stored_shape"] = {"fp32": H.stored_shape(api.FP32), "fp64": H.stored_shape(api.FP64)}
-    if direct:
+    if direct and engine == 2:
+        g32_ms, g32_fl = sum(t32["gemm_ms"]), sum(t32["gemm_flops"])
+        g64_ms, g64_fl = sum(t64["gemm_ms"]), sum(t64["gemm_flops"])
+        ach = g32_fl / (g32_ms * 1e-3) / 1e12
+        ach64f = g64_fl / (g64_ms * 1e-3) / 1e12
+        traffic = measured_traffic(args.workload, f"ras{kr}")
+        line["roofline"] = {
+            "bound": "mfma", "achieved": round(ach, 2), "peak": MFMA_PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
+            "frac": round(ach / MFMA_PEAK_TFLOPS["fp32"], 4),
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
+            "kernel": (f"k_ras_gemm<float,16,*> (the string-driven sigma's dense products: all "
+                       f"{ras['blocks']} launches of one {kr}-column FP32 sigma)"),
+            "algorithmic_flops_per_sigma": g32_fl, "gemm_ms_per_sigma": round(g32_ms, 4),
+            "fp64": {"achieved": round(ach64f, 2), "peak": MFMA_PEAK_TFLOPS["fp64"],
+                     "frac": round(ach64f / MFMA_PEAK_TFLOPS["fp64"], 4), "gemm_ms_per_sigma": round(g64_ms, 4),
+                     "algorithmic_flops_per_sigma": g64_fl}}
+        if traffic:
+            line["roofline"]["traffic_source"] = traffic["source"]
+        line["direct"] = {
+            "engine": "string-driven (resolutions of the identity over N-1 / N-2 strings, MFMA products)",
+            "plan": ras,
+            "timing_fp32": {kk: (round(v, 4) if isinstance(v, float) else [round(x, 4) for x in v])
+                            for kk, v in t32.items() if kk != "gemm_flops"},
+            "timing_fp64": {kk: (round(v, 4) if isinstance(v, float) else [round(x, 4) for x in v])
+                            for kk, v in t64.items() if kk != "gemm_flops"},
+            "sigma_hbm_gbs_algorithmic": {"fp32": round(ach32, 1), "fp64": round(ach64, 1)},
+            "note": ("time per sigma call by launch group: SEF -> determinant scatter, dense products by kind "
+                     "(alpha-beta, alpha-alpha, beta-beta), SEF gather; the products run on MFMA, so the "
+                     "roofline is the FP32 MFMA peak")}
+    elif direct:
         dp = H.direct_det_pairs()
         line["direct"] = {"ordered_configuration_pairs": npairs, "structural_sef_elements_per_sigma": nelems,
                           "connected_determinant_pairs_per_sigma": dp,

@@ -260,6 +260,16 @@ int psoci_direct_engine(psoci_ctx* ctx, int request, int* engine);
 int psoci_direct_ras_info(psoci_ctx* ctx, int* m, int* n_external, int* nsectors, int* nblocks, int64_t* ndet,
                           int64_t* w_per_column, double* flops_per_column);
 
+/* Times the string-driven engine: reps sigma calls of k columns in
+ * precision prec on internal device buffers, with HIP events on the context
+ * stream around every launch; ms[5] receives the average time per call of
+ * the SEF -> determinant scatter (ms[0]), the dense products of kind
+ * alpha-beta, alpha-alpha and beta-beta (ms[1..3], all blocks and column
+ * chunks of the call) and the SEF gather (ms[4]); flops[3] (may be NULL) the
+ * products' algorithmic flops per call by kind (2 rows nq^2 k).  No
+ * reference counterpart (a measurement entry point). */
+int psoci_direct_ras_timing(psoci_ctx* ctx, int prec, int k, int reps, double* ms, double* flops);
+
 /* Connected determinant pairs one direct sigma visits: the (row
  * determinant, column determinant) pairs within two spin-orbital
  * replacements over every configuration pair the reference computes a block

@@ -106,6 +106,7 @@ def load(path: str | None = None):
         "psoci_direct_det_pairs": (I, [V, V]),
         "psoci_direct_engine": (I, [V, I, V]),
         "psoci_direct_ras_info": (I, [V, V, V, V, V, V, V, V]),
+        "psoci_direct_ras_timing": (I, [V, I, I, I, V, V]),
         "psoci_direct_schedule": (I, [V, V, V, V]),
         "psoci_direct_schedule_prec": (I, [V, I, V, V, V]),
         "psoci_stored_shape": (I, [V, I, V, V, V]),
@@ -407,6 +408,15 @@ class Hamiltonian:
                                                  ctypes.byref(fl)))
         return {"m": m.value, "n_external": ne.value, "sectors": ns.value, "blocks": nb.value, "ndet": nd.value,
                 "w_per_column": wu.value, "flops_per_column": fl.value}
+
+    def direct_ras_timing(self, prec: int, k: int, reps: int = 3):
+        """Average ms per string-driven sigma call by launch group (scatter,
+        alpha-beta, alpha-alpha, beta-beta products, gather) and the
+        products' flops per call by kind."""
+        ms = np.zeros(5)
+        fl = np.zeros(3)
+        self._check(self.L.psoci_direct_ras_timing(self._h, prec, k, reps, _p(ms), _p(fl)))
+        return {"scatter_ms": ms[0], "gemm_ms": list(ms[1:4]), "gather_ms": ms[4], "gemm_flops": list(fl)}
 
     def direct_det_pairs(self) -> int:
         """Connected determinant pairs per direct sigma (counted once)."""

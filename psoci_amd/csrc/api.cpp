@@ -261,6 +261,15 @@ int psoci_direct_ras_info(psoci_ctx* ctx, int* m, int* n_external, int* nsectors
   });
 }
 
+int psoci_direct_ras_timing(psoci_ctx* ctx, int prec, int k, int reps, double* ms, double* flops) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(prec == PSOCI_FP64 || prec == PSOCI_FP32, PSOCI_ERR_ARG, "prec must be PSOCI_FP64 or PSOCI_FP32");
+    PSOCI_CHECK(ms, PSOCI_ERR_ARG, "null output");
+    psoci::ras_timing(c, prec, k, reps, ms, flops);
+    return PSOCI_OK;
+  });
+}
+
 int psoci_direct_det_pairs(psoci_ctx* ctx, int64_t* det_pairs) {
   return guarded(ctx, [&](Ctx& c) {
     PSOCI_CHECK(c.have_space && c.have_ints, PSOCI_ERR_STATE, "configurations and integrals are required");

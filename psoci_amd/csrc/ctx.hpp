@@ -110,6 +110,10 @@ struct Ctx {
   DevBuf<float> d_rasg32;
   DevBuf<RasView> d_rasview;
   DevBuf<char> d_rasc, d_rasw;  // determinant vectors [ndet][16 FP32 | 8 FP64], W [wunit][16 | 8]
+  DevBuf<int32_t> d_rasdidx;    // [wunit]: determinant (| negative << 31) of every D entry
+  DevBuf<int64_t> d_rastoff;    // nsef + 1: first term of each SEF
+  DevBuf<int32_t> d_rasterm;    // signed W offsets (bit 31: negative)
+  DevBuf<int8_t> d_rasscl;      // per SEF: 0 1/sqrt(2), 1 sqrt(2), 2 one
   void drop_ras() {
     have_ras = ras_tried = false;
     ras = RasPlan();
@@ -126,6 +130,10 @@ struct Ctx {
     d_rasview.release();
     d_rasc.release();
     d_rasw.release();
+    d_rasdidx.release();
+    d_rastoff.release();
+    d_rasterm.release();
+    d_rasscl.release();
   }
 
   // transition Hamiltonian (rows: this list's SEFs; columns: a second
@@ -164,6 +172,9 @@ int64_t direct_det_pairs(Ctx& c);
 // (false when the space does not qualify; c.ras_why says why), and the sigma
 bool build_ras(Ctx& c);
 void sigma_ras(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy);
+// one timed sigma of k columns on internal buffers, averaged over reps: ms[5]
+// per launch group (scatter, products by kind, gather), flops[3] per kind
+void ras_timing(Ctx& c, int prec, int k, int reps, double* ms, double* flops);
 // the direct sigma's operator: the string-driven plan where requested and
 // possible, else the configuration pair records (throws when engine 2 is
 // requested and the space does not qualify)

@@ -263,7 +263,7 @@ void ras_plan(RasPlan& P, const Space& sp, const double* one, const double* two)
           ++cnt;
         }
       qn[id] = cnt;
-      qnp[id] = (cnt + 15) / 16 * 16;
+      qnp[id] = (cnt + 15) / 16 * 16;  // the products take chunks of 32 pairs and a last one of 16
       for (int64_t t = cnt; t < qnp[id]; ++t) P.qlist.push_back((int16_t)-1);
     }
   // blocks
@@ -364,6 +364,7 @@ void ras_plan(RasPlan& P, const Space& sp, const double* one, const double* two)
     }
   }
   (void)nb2;
+  if (wunit >= ((int64_t)1 << 31)) return fail("more than 2^31 W elements per column");
   P.wunit = wunit;
   P.gemm_flops_per_col = flops;
   // SEF -> determinants: SEF k of a configuration with open shells is

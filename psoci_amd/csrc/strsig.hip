@@ -10,12 +10,14 @@
 //   k_ras_x2c   SEF vectors -> determinant vectors c[det][KC] (each
 //               determinant belongs to one SEF: a scatter without collisions);
 //   k_ras_gemm  per block: W[row][p-pair][KC] = sum over q-pairs of
-//               G[p-pair][q-pair] D[row][q-pair][KC], D gathered from c
-//               through the string addition tables into LDS, the product on
+//               G[p-pair][q-pair] D[row][q-pair][KC], D gathered from c into
+//               LDS through the determinant of each entry (built once from
+//               the string addition tables by k_ras_didx), the product on
 //               MFMA (v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64);
-//   k_ras_y     per SEF and column: y = sum over its determinants of
-//               coefficient * (sum over the RI terms of +-W), written once
-//               (a gather: deterministic, no atomics).
+//   k_ras_y     per SEF and column: y = scale * sum of +-W over the RI
+//               terms of its determinants (the term list -- signed W offsets
+//               -- is built once at setup by k_ras_terms), written once (a
+//               gather: deterministic, no atomics).
 // HBM layout: c [ndet][KC] (64 B per determinant at KC = 16 FP32), W
 // [block rows][nqp][KC] so that a term of k_ras_y reads KC contiguous values.
 #include <algorithm>
@@ -28,9 +30,40 @@ namespace psoci {
 
 namespace {
 
-constexpr int kPT = 64;   // pair rows (p-pairs) per GEMM tile
-constexpr int kMT = 128;  // (row, column) values per GEMM tile
-constexpr int kQC = 16;   // q-pairs per LDS chunk
+// GEMM tile shape, LDS layout and schedule (compile-time tuning knobs; the
+// defaults are the fastest measured on mrci10x24: FP32 16-pair chunks
+// without register prefetch, which keeps 6 wavefronts per SIMD, and FP64
+// 16-pair chunks with the next chunk's loads issued before the MFMA steps)
+#ifndef PSOCI_RAS_QC
+#define PSOCI_RAS_QC 16
+#endif
+#ifndef PSOCI_RAS_QC64
+#define PSOCI_RAS_QC64 16
+#endif
+#ifndef PSOCI_RAS_GPAD
+#define PSOCI_RAS_GPAD 1
+#endif
+#ifndef PSOCI_RAS_DPAD
+#define PSOCI_RAS_DPAD 4
+#endif
+#ifndef PSOCI_RAS_VSTORE
+#define PSOCI_RAS_VSTORE 1
+#endif
+#ifndef PSOCI_RAS_WIDE
+#define PSOCI_RAS_WIDE 0
+#endif
+#ifndef PSOCI_RAS_PIPE
+#define PSOCI_RAS_PIPE 0
+#endif
+#ifndef PSOCI_RAS_PIPE64
+#define PSOCI_RAS_PIPE64 1
+#endif
+constexpr int kMT = 128;            // (row, column) values per GEMM tile
+// q-pairs per LDS chunk (FP32, FP64)
+template <typename T>
+constexpr int ras_qc() {
+  return sizeof(T) == 8 ? PSOCI_RAS_QC64 : PSOCI_RAS_QC;
+}
 
 inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
@@ -92,164 +125,177 @@ struct VecN<double, 2> {
   typedef double type __attribute__((ext_vector_type(2)));
 };
 
-// One tile of W = D G^T for block bi: kPT p-pairs x kMT (row, column)
-// values.  Four wavefronts, each 64 p-pairs x 32 values (4 x 2 MFMA tiles).
-// Per chunk of kQC q-pairs: G[p][q] rows into LDS, the D entries gathered
-// from c through the addition tables into LDS, then 4 MFMA steps.  Tiles
-// are dealt so that XCD x runs the contiguous tiles [x per, (x + 1) per):
-// the p-tiles of one row tile, which gather the same c rows, share an L2.
-template <typename T, int KC, int KIND>
-__global__ __launch_bounds__(256) void k_ras_gemm(const RasView* __restrict__ vp, int bi, const T* __restrict__ G,
+// Determinant of every D entry (built once): didx[wbase + row * nqp + j]
+// = det | negative << 31 for pair j of block row `row`, kRasNoDet when the
+// pair cannot be added to the row's strings (ras_d_entry).
+constexpr int32_t kRasNoDet = 0x7fffffff;
+__global__ __launch_bounds__(256) void k_ras_didx(const RasView* __restrict__ vp, int bi, int32_t* __restrict__ didx) {
+  const RasView& v = *vp;
+  const RasBlockDev& B = v.blk[bi];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= B.nrows * B.nqp) return;
+  const int64_t row = e / B.nqp;
+  const int j = (int)(e % B.nqp);
+  int neg = 0;
+  const int64_t det = ras_d_entry(v, B, row, j, neg);
+  didx[B.wbase + e] = (det < 0 || det >= v.ndet) ? kRasNoDet : (int32_t)((uint32_t)det | ((uint32_t)neg << 31));
+}
+
+// One tile of W = D G^T for block bi: PT p-pairs x kMT (row, column)
+// values.  Four wavefronts, each PT p-pairs x 32 values (PT / 16 x 2 MFMA
+// tiles); PT = 128 halves the D gathers per flop where the pair count pads
+// no worse than at 64.
+// Per chunk of kQC q-pairs: G[p][q] rows and the D entries (c rows of the
+// determinants in didx) into LDS, then kQC / 4 MFMA steps; the global loads
+// of the next chunk are issued before the MFMA steps of the current one
+// (software pipelining through registers).  Tiles are dealt so that XCD x
+// runs the contiguous tiles [x per, (x + 1) per): the p-tiles of one row
+// tile, which gather the same c rows, share an L2.
+template <typename T, int KC, int PT>
+__global__ __launch_bounds__(256) void k_ras_gemm(const RasBlockDev* __restrict__ blk, int bi,
+                                                  const int32_t* __restrict__ didx, const T* __restrict__ G,
                                                   const T* __restrict__ c, T* __restrict__ W, int64_t ntiles,
                                                   int npt) {
   typedef Mfma<T> M;
+  constexpr int kQC = ras_qc<T>();
   typedef typename M::v4 v4;
-  constexpr int RT = kMT / KC;                  // block rows per tile
-  constexpr int VPT = KC < 8 ? KC : 8;          // consecutive columns one thread gathers
-  constexpr int CPT = 8 / VPT;                  // (row, pair) entries per thread and chunk
-  constexpr int TPC = KC > 8 ? KC / 8 : 1;      // threads per entry
-  __shared__ T Gs[kPT][kQC + 1];
-  __shared__ T Ds[kQC][kMT + 4];
-  __shared__ int64_t s_ra[RT], s_rb[RT];
-  __shared__ int64_t s_boff[kRasMaxCls][kRasMaxCls];
-  __shared__ int64_t s_cnt[kRasMaxCls], s_off1[kRasMaxCls];
-  const RasView& v = *vp;
+  constexpr int RT = kMT / KC;          // block rows per tile
+  constexpr int DPT = kMT * kQC / 256;        // D values per thread and chunk
+  constexpr int VPT = KC < DPT ? KC : DPT;     // consecutive columns one thread gathers per entry
+  constexpr int CPT = DPT / VPT;               // (row, pair) entries per thread and chunk
+  constexpr int TPC = KC / VPT;                // threads per entry
+  constexpr int GPT = PT * kQC / 256;          // G values per thread and chunk
+  constexpr int NI = PT / 16;                  // p-subtiles per wavefront
+  constexpr bool pipe = sizeof(T) == 8 ? PSOCI_RAS_PIPE64 : PSOCI_RAS_PIPE;
+  // row strides of the LDS tiles (compile-time knobs)
+  __shared__ __align__(16) T Gs[PT][kQC + PSOCI_RAS_GPAD];
+  __shared__ __align__(16) T Ds[kQC][kMT + PSOCI_RAS_DPAD];
   const int64_t per = (ntiles + 7) / 8;
   const int64_t tile = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
   if (tile >= ntiles) return;  // uniform per workgroup
   const int64_t mt = tile / npt;
   const int pt = (int)(tile % npt);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const RasBlockDev& B = v.blk[bi];
-  const int64_t nrows = B.nrows, nw = B.nw, u0 = B.u0, w0 = B.w0;
-  const int nqp = B.nqp, nbf = v.nbf;
-  const RasSectorDev& S = v.sec[B.sector];
-  const int64_t det0 = S.det0, ndet = v.ndet;
-  // second-addition level of a same-spin block: l2 + 1
-  const int l2 = KIND == 1 ? B.lu : B.lw;
-  const int64_t add1_off = KIND == 0 ? 0 : v.lev[l2 + 1].add_off;
-  for (int i = tid; i < kRasMaxCls * kRasMaxCls; i += 256) s_boff[i / kRasMaxCls][i % kRasMaxCls] = S.boff[i / kRasMaxCls][i % kRasMaxCls];
-  if (tid < kRasMaxCls) {
-    s_cnt[tid] = v.lev[S.nb].cls_off[tid + 1] - v.lev[S.nb].cls_off[tid];
-    s_off1[tid] = KIND == 0 ? 0 : v.lev[l2 + 1].cls_off[tid];
-  }
-  for (int rl = tid; rl < RT; rl += 256) {
-    const int64_t row = mt * RT + rl;
-    int64_t ra = -1, rb = -1;
-    if (row < nrows) {
-      const int64_t u = u0 + row / nw, w = w0 + row % nw;
-      if (KIND == 0) {
-        ra = v.lev[B.lu].add_off + u * nbf;
-        rb = v.lev[B.lw].add_off + w * nbf;
-      } else if (KIND == 1) {
-        ra = v.lev[B.lu].add_off + u * nbf;
-        const int cb = ras_cls_of(v, S.nb, w);
-        rb = ((int64_t)cb << 40) | (w - v.lev[S.nb].cls_off[cb]);
-      } else {
-        const int ca = ras_cls_of(v, S.na, u);
-        ra = ((int64_t)ca << 40) | (u - v.lev[S.na].cls_off[ca]);
-        rb = v.lev[B.lw].add_off + w * nbf;
-      }
-    }
-    s_ra[rl] = ra;
-    s_rb[rl] = rb;
-  }
+  const RasBlockDev& B = blk[bi];
+  const int64_t nrows = B.nrows;
+  const int nqp = B.nqp;
   const T* __restrict__ Gb = G + B.goff;
-  const int16_t* __restrict__ ql = v.qlist + B.qoff;
-  const uint32_t* __restrict__ add = v.add;
-  v4 acc[4][2];
+  const int32_t* __restrict__ dx = didx + B.wbase;
+  v4 acc[NI][2];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < NI; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = v4{0, 0, 0, 0};
-  __syncthreads();
-  for (int q0 = 0; q0 < nqp; q0 += kQC) {
-    {
-      const int p = tid >> 2, jq = (tid & 3) * 4;
-      const int pg = pt * kPT + p;
+
+  // per-thread staging of one chunk
+  const int gp = tid / (kQC / GPT), gq = (tid % (kQC / GPT)) * GPT;  // G: row gp, q-pairs gq .. gq + GPT
+  const int gpg = pt * PT + gp;
+  T gv[GPT];
+  T dv[CPT][VPT];
+  // chunks of kQC pairs; nqp is a multiple of 16, so the last chunk may
+  // hold 16: its other half is loaded as zeros
+  auto load = [&](int q0) {
+    const int wq = nqp - q0 < kQC ? nqp - q0 : kQC;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) Gs[p][jq + i] = pg < nqp ? Gb[(int64_t)pg * nqp + q0 + jq + i] : (T)0;
-    }
+    for (int i = 0; i < GPT; ++i) gv[i] = (gpg < nqp && gq + i < wq) ? Gb[(int64_t)gpg * nqp + q0 + gq + i] : (T)0;
 #pragma unroll
     for (int ci = 0; ci < CPT; ++ci) {
       const int ent = (tid / TPC) * CPT + ci;  // (row, pair) entry: rl * kQC + jq
       const int v0 = (tid % TPC) * VPT;
       const int rl = ent / kQC, jq = ent % kQC;
-      T xv[VPT];
+      const int64_t row = mt * RT + rl;
 #pragma unroll
-      for (int i = 0; i < VPT; ++i) xv[i] = (T)0;
-      const int64_t ra = s_ra[rl], rb = s_rb[rl];
-      const int16_t qs = ql[q0 + jq];
-      if (ra >= 0 && qs >= 0) {
-        const int q = qs & 0xff, s = (qs >> 8) & 0xff;
-        int64_t det = -1;
-        int neg = 0;
-        if (KIND == 0) {
-          const uint32_t ea = add[ra + q], eb = add[rb + s];
-          if (ea != kRasNone && eb != kRasNone) {
-            neg = ras_neg(ea) ^ ras_neg(eb);
-            const int ca = ras_cls(ea), cb = ras_cls(eb);
-            const int64_t bo = s_boff[ca][cb];
-            det = bo < 0 ? -1 : det0 + bo + ras_rank(ea) * s_cnt[cb] + ras_rank(eb);
+      for (int i = 0; i < VPT; ++i) dv[ci][i] = (T)0;
+      const int32_t e = (row < nrows && jq < wq) ? dx[row * nqp + q0 + jq] : kRasNoDet;
+      if (e != kRasNoDet) {
+        const bool neg = e < 0;
+        const T* __restrict__ src = c + (int64_t)(e & 0x7fffffff) * KC + v0;
+        if constexpr (VPT * sizeof(T) % 16 == 0) {
+          typedef typename VecN<T, 16 / sizeof(T)>::type V;
+          constexpr int W16 = 16 / sizeof(T);
+#pragma unroll
+          for (int i = 0; i < VPT / W16; ++i) {
+            const V x = reinterpret_cast<const V*>(src)[i];
+#pragma unroll
+            for (int w = 0; w < W16; ++w) dv[ci][i * W16 + w] = neg ? -x[w] : x[w];
           }
         } else {
-          const int64_t base = KIND == 1 ? ra : rb;
-          const uint32_t e1 = add[base + s];
-          if (e1 != kRasNone) {
-            const int64_t g1 = s_off1[ras_cls(e1)] + ras_rank(e1);
-            const uint32_t e2 = add[add1_off + g1 * nbf + q];
-            if (e2 != kRasNone) {
-              neg = ras_neg(e1) ^ ras_neg(e2);
-              if (KIND == 1) {
-                const int cb = (int)(rb >> 40);
-                const int64_t bo = s_boff[ras_cls(e2)][cb];
-                det = bo < 0 ? -1 : det0 + bo + ras_rank(e2) * s_cnt[cb] + (rb & 0xffffffffffll);
-              } else {
-                const int ca = (int)(ra >> 40), cb = ras_cls(e2);
-                const int64_t bo = s_boff[ca][cb];
-                det = bo < 0 ? -1 : det0 + bo + (ra & 0xffffffffffll) * s_cnt[cb] + ras_rank(e2);
-              }
-            }
-          }
-        }
-        if (det >= 0 && det < ndet) {
-          const T* __restrict__ src = c + det * KC + v0;
-          if constexpr (VPT * sizeof(T) % 16 == 0) {
-            typedef typename VecN<T, 16 / sizeof(T)>::type V;
-            constexpr int W16 = 16 / sizeof(T);
 #pragma unroll
-            for (int i = 0; i < VPT / W16; ++i) {
-              const V x = reinterpret_cast<const V*>(src)[i];
-#pragma unroll
-              for (int w = 0; w < W16; ++w) xv[i * W16 + w] = neg ? -x[w] : x[w];
-            }
-          } else {
-#pragma unroll
-            for (int i = 0; i < VPT; ++i) xv[i] = neg ? -src[i] : src[i];
-          }
+          for (int i = 0; i < VPT; ++i) dv[ci][i] = neg ? -src[i] : src[i];
         }
       }
-#pragma unroll
-      for (int i = 0; i < VPT; ++i) Ds[jq][rl * KC + v0 + i] = xv[i];
     }
+  };
+  auto store = [&]() {
+    typedef typename VecN<T, 16 / sizeof(T)>::type V;
+    constexpr int W16 = 16 / sizeof(T);
+    constexpr bool vec = PSOCI_RAS_VSTORE && (kQC + PSOCI_RAS_GPAD) % W16 == 0 && (kMT + PSOCI_RAS_DPAD) % W16 == 0;
+    if constexpr (vec && GPT % W16 == 0) {
+#pragma unroll
+      for (int i = 0; i < GPT / W16; ++i) {
+        V x;
+#pragma unroll
+        for (int w = 0; w < W16; ++w) x[w] = gv[i * W16 + w];
+        *reinterpret_cast<V*>(&Gs[gp][gq + i * W16]) = x;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < GPT; ++i) Gs[gp][gq + i] = gv[i];
+    }
+#pragma unroll
+    for (int ci = 0; ci < CPT; ++ci) {
+      const int ent = (tid / TPC) * CPT + ci;
+      const int v0 = (tid % TPC) * VPT;
+      const int rl = ent / kQC, jq = ent % kQC;
+      if constexpr (vec && VPT % W16 == 0) {
+#pragma unroll
+        for (int i = 0; i < VPT / W16; ++i) {
+          V x;
+#pragma unroll
+          for (int w = 0; w < W16; ++w) x[w] = dv[ci][i * W16 + w];
+          *reinterpret_cast<V*>(&Ds[jq][rl * KC + v0 + i * W16]) = x;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < VPT; ++i) Ds[jq][rl * KC + v0 + i] = dv[ci][i];
+      }
+    }
+  };
+  if (pipe) {
+    load(0);
+    store();
     __syncthreads();
+  }
+  for (int q0 = 0; q0 < nqp; q0 += kQC) {
+    const bool more = q0 + kQC < nqp;
+    if (pipe) {
+      if (more) load(q0 + kQC);  // in flight during the MFMA steps below
+    } else {
+      load(q0);
+      store();
+      __syncthreads();
+    }
 #pragma unroll
     for (int kk = 0; kk < kQC / 4; ++kk) {
-      T a[4], b[2];
+      T a[NI], b[2];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = Gs[16 * i + (lane & 15)][4 * kk + (lane >> 4)];
+      for (int i = 0; i < NI; ++i) a[i] = Gs[16 * i + (lane & 15)][4 * kk + (lane >> 4)];
 #pragma unroll
       for (int j = 0; j < 2; ++j) b[j] = Ds[4 * kk + (lane >> 4)][32 * wave + 16 * j + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < NI; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = M::op(a[i], b[j], acc[i][j]);
     }
     __syncthreads();
+    if (pipe) {
+      if (more) store();
+      __syncthreads();
+    }
   }
   const int64_t wb = B.wbase;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < NI; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int m = 32 * wave + 16 * j + (lane & 15);
@@ -257,35 +303,73 @@ __global__ __launch_bounds__(256) void k_ras_gemm(const RasView* __restrict__ vp
       const int vq = m % KC;
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const int p = pt * kPT + 16 * i + M::row(t, lane);
+        const int p = pt * PT + 16 * i + M::row(t, lane);
         if (row < nrows && p < nqp) W[(wb + row * nqp + p) * KC + vq] = acc[i][j][t];
       }
     }
 }
 
-// y[sef][col0 + q] = sum over the SEF's determinants of coefficient (x 2 for
-// an Ms > 0 determinant, whose mirror is not stored) times sum of +-W terms
-template <typename T, int KC>
-__global__ __launch_bounds__(256) void k_ras_y(const RasView* __restrict__ vp, const T* __restrict__ W,
-                                               T* __restrict__ Y, int ldy, int col0, int kw) {
+// The term list of every SEF (built once): for each of its determinants
+// every RI term of ras_terms, as the W offset per column with bit 31 set
+// for a negative sign (the determinant's coefficient sign folded in), and
+// the SEF's scale: 0 -> 1/sqrt(2) (an Ms = 0 pair, both determinants), 1 ->
+// sqrt(2) (an Ms > 0 determinant standing for its mirror too), 2 -> 1 (a
+// closed shell).  *bad = 1 when a term has no W element or a list does not
+// fill its slot exactly (never, for a plan that passes its host checks).
+__global__ __launch_bounds__(256) void k_ras_terms(const RasView* __restrict__ vp, const int64_t* __restrict__ toff,
+                                                   int32_t* __restrict__ term, int8_t* __restrict__ scl,
+                                                   int* __restrict__ bad) {
   const RasView& v = *vp;
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= v.nsef) return;
+  const RasSef e = v.sef[s];
+  int64_t o = toff[s];
+  const int64_t end = toff[s + 1];
+  scl[s] = (int8_t)((e.flag[0] & 2) ? 2 : (e.sec[0] > 0 ? 1 : 0));
+  for (int d = 0; d < e.n; ++d) {
+    const int nd = e.flag[d] & 1;
+    ras_terms(v, e.sec[d], e.a[d], e.b[d], [&](int bi, int64_t off, int neg) {
+      if (bi < 0 || off < 0 || off >= v.wunit || o >= end) {
+        *bad = 1;
+        return;
+      }
+      term[o++] = (int32_t)((uint32_t)off | ((uint32_t)(neg ^ nd) << 31));
+    });
+  }
+  if (o != end) *bad = 1;
+}
+
+// y[sef][col0 + q] = scale * sum of +-W[off][q] over the SEF's terms
+template <typename T, int KC>
+__global__ __launch_bounds__(256) void k_ras_y(const int64_t* __restrict__ toff, const int32_t* __restrict__ term,
+                                               const int8_t* __restrict__ scl, const T* __restrict__ W, int64_t nsef,
+                                               T* __restrict__ Y, int ldy, int col0, int kw) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t s = t / KC;
   const int q = (int)(t % KC);
-  if (s >= v.nsef) return;
-  const RasSef e = v.sef[s];
-  T y = (T)0;
-  for (int d = 0; d < e.n; ++d) {
-    T sig = (T)0;
-    ras_terms(v, e.sec[d], e.a[d], e.b[d], [&](int bi, int64_t off, int neg) {
-      if (bi < 0 || off < 0 || off >= v.wunit) return;
-      const T w = W[off * KC + q];
-      sig += neg ? -w : w;
-    });
-    const T cf = sef_coef<T>(e.flag[d]) * (e.sec[d] > 0 ? (T)2 : (T)1);
-    y += cf * sig;
+  if (s >= nsef) return;
+  const int64_t b = toff[s], e = toff[s + 1];
+  T a0 = (T)0, a1 = (T)0;
+  int64_t i = b;
+  for (; i + 4 <= e; i += 4) {
+    const int32_t t0 = term[i], t1 = term[i + 1], t2 = term[i + 2], t3 = term[i + 3];
+    const T w0 = W[(int64_t)(t0 & 0x7fffffff) * KC + q];
+    const T w1 = W[(int64_t)(t1 & 0x7fffffff) * KC + q];
+    const T w2 = W[(int64_t)(t2 & 0x7fffffff) * KC + q];
+    const T w3 = W[(int64_t)(t3 & 0x7fffffff) * KC + q];
+    a0 += t0 < 0 ? -w0 : w0;
+    a1 += t1 < 0 ? -w1 : w1;
+    a0 += t2 < 0 ? -w2 : w2;
+    a1 += t3 < 0 ? -w3 : w3;
   }
-  if (q < kw) Y[s * (int64_t)ldy + col0 + q] = y;
+  for (; i < e; ++i) {
+    const int32_t t0 = term[i];
+    const T w0 = W[(int64_t)(t0 & 0x7fffffff) * KC + q];
+    a0 += t0 < 0 ? -w0 : w0;
+  }
+  const int8_t sc = scl[s];
+  const T f = sc == 2 ? (T)1 : (sc == 1 ? (T)0x1.6a09e667f3bcdp+0 : (T)0x1.6a09e667f3bcdp-1);
+  if (q < kw) Y[s * (int64_t)ldy + col0 + q] = f * (a0 + a1);
 }
 
 // Flags (*flag = 1) a configuration pair one electron apart whose
@@ -337,45 +421,82 @@ __global__ __launch_bounds__(256) void k_ras_zero_singles(int64_t ncfg, const ui
   }
 }
 
+// Optional per-launch events of a timed sigma: group 0 the SEF ->
+// determinant scatter, 1..3 the products of each kind, 4 the SEF gather.
+struct RasTiming {
+  std::vector<hipEvent_t> ev;
+  std::vector<int> group;  // group of the launch that ends at ev[i + 1]
+  void mark(hipStream_t s, int g) {
+    hipEvent_t e;
+    PSOCI_HIP(hipEventCreate(&e));
+    PSOCI_HIP(hipEventRecord(e, s));
+    ev.push_back(e);
+    if (g >= 0) group.push_back(g);
+  }
+  ~RasTiming() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+};
+
 template <typename T, int KC>
-void ras_chunk(Ctx& c, const T* X, int ldx, T* Y, int ldy, int col0, int kw) {
+void ras_chunk(Ctx& c, const T* X, int ldx, T* Y, int ldy, int col0, int kw, RasTiming* tm = nullptr) {
   const RasPlan& P = c.ras;
   const RasView* vp = c.d_rasview.p;
   T* cv = reinterpret_cast<T*>(c.d_rasc.p);
   T* W = reinterpret_cast<T*>(c.d_rasw.p);
   const T* G = sizeof(T) == 4 ? reinterpret_cast<const T*>(c.d_rasg32.p) : reinterpret_cast<const T*>(c.d_rasg64.p);
   const int64_t nsef = (int64_t)P.sef.size();
+  if (tm && tm->ev.empty()) tm->mark(c.stream, -1);
   hipLaunchKernelGGL((k_ras_x2c<T, KC>), dim3(nblk(nsef * KC, 256)), dim3(256), 0, c.stream, vp, X, ldx, col0, kw,
                      cv);
   PSOCI_HIP(hipGetLastError());
+  if (tm) tm->mark(c.stream, 0);
   for (int bi = 0; bi < (int)P.blk.size(); ++bi) {
     const RasBlockDev& B = P.blk[bi];
-    const int npt = (B.nqp + kPT - 1) / kPT;
+    // 128 p-pairs per tile (FP32) where that pads the pairs at most 7 % more
+    // than 64 does
+    const bool wide = PSOCI_RAS_WIDE && sizeof(T) == 4 &&
+                      (B.nqp + 127) / 128 * 128 <= (B.nqp + 63) / 64 * 64 * 107 / 100;
+    const int pt_w = wide ? 128 : 64;
+    const int npt = (B.nqp + pt_w - 1) / pt_w;
     const int64_t nmt = (B.nrows * KC + kMT - 1) / kMT;
     const int64_t ntiles = nmt * npt;
     const unsigned grid = (unsigned)(8 * ((ntiles + 7) / 8));
-    if (B.kind == 0)
-      hipLaunchKernelGGL((k_ras_gemm<T, KC, 0>), dim3(grid), dim3(256), 0, c.stream, vp, bi, G, cv, W, ntiles, npt);
-    else if (B.kind == 1)
-      hipLaunchKernelGGL((k_ras_gemm<T, KC, 1>), dim3(grid), dim3(256), 0, c.stream, vp, bi, G, cv, W, ntiles, npt);
-    else
-      hipLaunchKernelGGL((k_ras_gemm<T, KC, 2>), dim3(grid), dim3(256), 0, c.stream, vp, bi, G, cv, W, ntiles, npt);
+    bool done = false;
+    if constexpr (sizeof(T) == 4) {
+      if (wide) {
+        hipLaunchKernelGGL((k_ras_gemm<T, KC, 128>), dim3(grid), dim3(256), 0, c.stream, c.d_rasblk.p, bi,
+                           c.d_rasdidx.p, G, cv, W, ntiles, npt);
+        done = true;
+      }
+    }
+    if (!done)
+      hipLaunchKernelGGL((k_ras_gemm<T, KC, 64>), dim3(grid), dim3(256), 0, c.stream, c.d_rasblk.p, bi,
+                         c.d_rasdidx.p, G, cv, W, ntiles, npt);
     PSOCI_HIP(hipGetLastError());
+    if (tm) tm->mark(c.stream, 1 + B.kind);
   }
-  hipLaunchKernelGGL((k_ras_y<T, KC>), dim3(nblk(nsef * KC, 256)), dim3(256), 0, c.stream, vp, W, Y, ldy, col0, kw);
+  hipLaunchKernelGGL((k_ras_y<T, KC>), dim3(nblk(nsef * KC, 256)), dim3(256), 0, c.stream, c.d_rastoff.p,
+                     c.d_rasterm.p, c.d_rasscl.p, W, nsef, Y, ldy, col0, kw);
   PSOCI_HIP(hipGetLastError());
+  if (tm) tm->mark(c.stream, 4);
 }
 
 template <typename T>
-void ras_any_k(Ctx& c, int k, const T* X, int ldx, T* Y, int ldy) {
+void ras_any_k(Ctx& c, int k, const T* X, int ldx, T* Y, int ldy, RasTiming* tm = nullptr) {
   const int kmax = sizeof(T) == 4 ? 16 : 8;
   for (int c0 = 0; c0 < k; c0 += kmax) {
     const int kw = std::min(kmax, k - c0);
-    if (kw > 8) ras_chunk<T, 16>(c, X, ldx, Y, ldy, c0, kw);
-    else if (kw > 4) ras_chunk<T, 8>(c, X, ldx, Y, ldy, c0, kw);
-    else if (kw > 2) ras_chunk<T, 4>(c, X, ldx, Y, ldy, c0, kw);
-    else if (kw > 1) ras_chunk<T, 2>(c, X, ldx, Y, ldy, c0, kw);
-    else ras_chunk<T, 1>(c, X, ldx, Y, ldy, c0, kw);
+    if constexpr (sizeof(T) == 4) {
+      if (kw > 8) {
+        ras_chunk<T, 16>(c, X, ldx, Y, ldy, c0, kw, tm);
+        continue;
+      }
+    }
+    if (kw > 4) ras_chunk<T, 8>(c, X, ldx, Y, ldy, c0, kw, tm);
+    else if (kw > 2) ras_chunk<T, 4>(c, X, ldx, Y, ldy, c0, kw, tm);
+    else if (kw > 1) ras_chunk<T, 2>(c, X, ldx, Y, ldy, c0, kw, tm);
+    else ras_chunk<T, 1>(c, X, ldx, Y, ldy, c0, kw, tm);
   }
 }
 
@@ -396,6 +517,26 @@ bool build_ras(Ctx& c) {
     return false;
   }
   const RasPlan& P = c.ras;
+  {
+    // device memory: W and the determinant vectors (16 FP32 / 8 FP64
+    // columns), the term lists; leave room for the caller's vectors
+    int64_t terms = 0;
+    for (const RasSef& e : P.sef)
+      for (int d = 0; d < e.n; ++d) {
+        const RasSectorDev& S = P.sec[e.sec[d]];
+        terms += (int64_t)S.na * S.nb + (int64_t)S.na * (S.na - 1) / 2 + (int64_t)S.nb * (S.nb - 1) / 2;
+      }
+    const double need =
+        (double)P.wunit * (64 + 4) + (double)P.ndet * 64 + (double)terms * 4 + (double)P.sef.size() * 9;
+    size_t fr = 0, tot = 0;
+    PSOCI_HIP(hipMemGetInfo(&fr, &tot));
+    if (need > 0.6 * (double)fr) {
+      c.ras_why = "needs " + std::to_string((int64_t)(need / 1e9)) + " GB of device memory, " +
+                  std::to_string((int64_t)(fr / 1000000000)) + " GB free";
+      c.ras = RasPlan();
+      return false;
+    }
+  }
   {
     DevBuf<int> flag;
     flag.alloc(1);
@@ -435,6 +576,40 @@ bool build_ras(Ctx& c) {
   c.d_rasview.upload(&v, 1, c.stream);
   // determinant vectors and W: 16 FP32 or 8 FP64 columns per chunk
   c.d_rasc.alloc((size_t)P.ndet * 16 * sizeof(float));
+  // the D entries' determinants
+  c.d_rasdidx.alloc(P.wunit);
+  for (int bi = 0; bi < (int)P.blk.size(); ++bi) {
+    hipLaunchKernelGGL(k_ras_didx, dim3(nblk(P.blk[bi].nrows * P.blk[bi].nqp, 256)), dim3(256), 0, c.stream,
+                       c.d_rasview.p, bi, c.d_rasdidx.p);
+    PSOCI_HIP(hipGetLastError());
+  }
+  // the SEF term lists: each determinant of sector (na, nb) has na nb +
+  // na (na - 1) / 2 + nb (nb - 1) / 2 terms
+  {
+    const int64_t nsef = (int64_t)P.sef.size();
+    std::vector<int64_t> toff(nsef + 1, 0);
+    for (int64_t s = 0; s < nsef; ++s) {
+      int64_t cnt = 0;
+      for (int d = 0; d < P.sef[s].n; ++d) {
+        const RasSectorDev& S = P.sec[P.sef[s].sec[d]];
+        cnt += (int64_t)S.na * S.nb + (int64_t)S.na * (S.na - 1) / 2 + (int64_t)S.nb * (S.nb - 1) / 2;
+      }
+      toff[s + 1] = toff[s] + cnt;
+    }
+    c.d_rastoff.upload(toff, c.stream);
+    c.d_rasterm.alloc(toff[nsef]);
+    c.d_rasscl.alloc(nsef);
+    DevBuf<int> bad;
+    bad.alloc(1);
+    PSOCI_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), c.stream));
+    hipLaunchKernelGGL(k_ras_terms, dim3(nblk(nsef, 256)), dim3(256), 0, c.stream, c.d_rasview.p, c.d_rastoff.p,
+                       c.d_rasterm.p, c.d_rasscl.p, bad.p);
+    PSOCI_HIP(hipGetLastError());
+    int b = 0;
+    bad.download(&b, 1, c.stream);
+    c.sync();
+    PSOCI_CHECK(!b, PSOCI_ERR_STATE, "string-driven sigma: inconsistent term lists");
+  }
   c.d_rasw.alloc((size_t)P.wunit * 16 * sizeof(float));
   c.sync();
   c.have_ras = true;
@@ -449,6 +624,36 @@ void build_direct(Ctx& c) {
   PSOCI_CHECK(c.direct_engine_req != 2 || c.have_ras, PSOCI_ERR_UNSUPPORTED,
               "the string-driven direct sigma does not apply to this space: " + c.ras_why);
   if (!direct_engine(c)) build_pairs(c);
+}
+
+void ras_timing(Ctx& c, int prec, int k, int reps, double* ms, double* flops) {
+  PSOCI_CHECK(c.have_ras, PSOCI_ERR_STATE, "the string-driven direct sigma is not set up");
+  PSOCI_CHECK(k >= 1 && reps >= 1, PSOCI_ERR_ARG, "k and reps must be positive");
+  const int64_t n = c.sp.nsef_total;
+  const size_t es = prec == PSOCI_FP32 ? 4 : 8;
+  DevBuf<char> X, Y;
+  X.alloc((size_t)n * k * es);
+  Y.alloc((size_t)n * k * es);
+  PSOCI_HIP(hipMemsetAsync(X.p, 0, (size_t)n * k * es, c.stream));
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int r = 0; r < reps; ++r) {
+    RasTiming tm;
+    if (prec == PSOCI_FP32)
+      ras_any_k<float>(c, k, reinterpret_cast<const float*>(X.p), k, reinterpret_cast<float*>(Y.p), k, &tm);
+    else
+      ras_any_k<double>(c, k, reinterpret_cast<const double*>(X.p), k, reinterpret_cast<double*>(Y.p), k, &tm);
+    PSOCI_HIP(hipEventSynchronize(tm.ev.back()));
+    for (size_t i = 0; i < tm.group.size(); ++i) {
+      float t = 0.f;
+      PSOCI_HIP(hipEventElapsedTime(&t, tm.ev[i], tm.ev[i + 1]));
+      acc[tm.group[i]] += t;
+    }
+  }
+  for (int g = 0; g < 5; ++g) ms[g] = acc[g] / reps;
+  if (flops) {
+    for (int g = 0; g < 3; ++g) flops[g] = 0.0;
+    for (const RasBlockDev& B : c.ras.blk) flops[B.kind] += 2.0 * (double)B.nrows * B.nq * B.nq * k;
+  }
 }
 
 void sigma_ras(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy) {
