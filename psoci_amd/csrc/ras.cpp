@@ -5,6 +5,7 @@
 #include "ras.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <numeric>
 
@@ -230,6 +231,11 @@ void ras_plan(RasPlan& P, const Space& sp, const double* one, const double* two)
     RasSectorDev S;
     S.na = s.first;
     S.nb = s.second;
+    // the SEF vectors make every Ms = 0 determinant vector symmetric under
+    // the spin flip when mult = +1 (D_2k+1 is D_2k's flip with the same
+    // coefficient; a closed shell is its own flip)
+    S.sym = (S.na == S.nb && sp.mult > 0) ? 1 : 0;
+    S.pad = 0;
     S.det0 = det0;
     int64_t o = 0;
     for (int a = 0; a < kRasMaxCls; ++a)
@@ -299,8 +305,11 @@ void ras_plan(RasPlan& P, const Space& sp, const double* one, const double* two)
       const bool fold_ss = !fold_ab && ((kind == 1 && S.nb == 0) || (kind == 2 && S.na == 0));
       int64_t gbase_by_budget[kRasMaxCls];
       for (int b = 0; b <= m; ++b) gbase_by_budget[b] = -1;
+      // a symmetric sector's mirrored products are not evaluated
+      if (S.sym && kind == 2) continue;
       for (int cu = 0; cu <= m; ++cu)
         for (int cw = 0; cw + cu <= m; ++cw) {
+          if (S.sym && kind == 0 && cu > cw) continue;
           const int64_t nu = ccount(lu, cu), nw = ccount(lw, cw);
           if (!nu || !nw) continue;
           const int bud = m - cu - cw;
@@ -320,7 +329,8 @@ void ras_plan(RasPlan& P, const Space& sp, const double* one, const double* two)
           B.nu = nu;
           B.w0 = P.lev[lw].cls_off[cw];
           B.nw = nw;
-          B.nrows = nu * nw;
+          B.tri = (S.sym && kind == 0 && cu == cw) ? 1 : 0;
+          B.nrows = B.tri ? nu * (nu + 1) / 2 : nu * nw;
           B.qoff = qoff[id];
           B.wbase = wunit;
           wunit += B.nrows * B.nqp;

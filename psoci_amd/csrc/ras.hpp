@@ -68,6 +68,10 @@ struct RasLevelDev {
 
 struct RasSectorDev {
   int na, nb;
+  // 1: an Ms = 0 sector whose vectors are symmetric under the spin flip
+  // (mult = +1): only the alpha-beta products with class(u) <= class(w) and
+  // no beta-beta products are evaluated; the others are their mirrors
+  int sym, pad;
   int64_t det0;                          // first determinant of the sector
   int64_t boff[kRasMaxCls][kRasMaxCls];  // block (alpha class, beta class) offset in the sector, -1 none
 };
@@ -81,7 +85,10 @@ struct RasBlockDev {
   int sector;
   int lu, lw, cu, cw;
   int budget, nq, nqp;
-  int pad0;
+  // 1: an alpha-beta product of a symmetric sector with cu == cw, whose rows
+  // (u, w) with u > w mirror (w, u): only u <= w are evaluated, in the
+  // triangular order ras_tri_row gives (nrows = nu (nu + 1) / 2)
+  int tri;
   int64_t u0, nu, w0, nw, nrows;
   int64_t qoff;   // first pair in qlist[]
   int64_t goff;   // first element of G (nqp x nqp, row-major [p-pair][q-pair])
@@ -115,6 +122,19 @@ struct RasView {
   int64_t ndet, wunit;  // determinants and W elements per column (bounds of every index)
 };
 
+// triangular row of (u, w), u <= w < n (local indices), and its inverse
+PSOCI_HD int64_t ras_tri_row(int64_t u, int64_t w, int64_t n) { return u * n - u * (u - 1) / 2 + (w - u); }
+PSOCI_HD void ras_tri_uw(int64_t row, int64_t n, int64_t& u, int64_t& w) {
+  const double b = 2.0 * (double)n + 1.0;
+  int64_t x = (int64_t)((b - sqrt(b * b - 8.0 * (double)row)) * 0.5);
+  if (x < 0) x = 0;
+  if (x > n - 1) x = n - 1;
+  while (x > 0 && ras_tri_row(x, x, n) > row) --x;
+  while (x < n - 1 && ras_tri_row(x + 1, x + 1, n) <= row) ++x;
+  u = x;
+  w = x + (row - ras_tri_row(x, x, n));
+}
+
 PSOCI_HD int64_t ras_det(const RasView& v, int s, int ca, int64_t ra, int cb, int64_t rb) {
   const RasSectorDev& S = v.sec[s];
   const int64_t nbw = v.lev[S.nb].cls_off[cb + 1] - v.lev[S.nb].cls_off[cb];
@@ -130,7 +150,15 @@ PSOCI_HD int64_t ras_d_entry(const RasView& v, const RasBlockDev& B, int64_t row
   const int16_t qs = v.qlist[B.qoff + j];
   if (qs < 0) return -1;
   const int q = qs & 0xff, s = (qs >> 8) & 0xff;
-  const int64_t u = B.u0 + row / B.nw, w = B.w0 + row % B.nw;
+  int64_t u, w;
+  if (B.tri) {
+    ras_tri_uw(row, B.nu, u, w);
+    u += B.u0;
+    w += B.w0;
+  } else {
+    u = B.u0 + row / B.nw;
+    w = B.w0 + row % B.nw;
+  }
   const RasSectorDev& S = v.sec[B.sector];
   if (B.kind == 0) {
     const uint32_t ea = v.add[v.lev[B.lu].add_off + u * v.nbf + q];
@@ -163,6 +191,12 @@ PSOCI_HD int64_t ras_d_entry(const RasView& v, const RasBlockDev& B, int64_t row
 // b), then alpha-alpha and beta-beta pairs (p > r).  A term without a block
 // or pair position (never, for a determinant of the plan) is passed as
 // f(-1, -1, 0).
+//
+// Spin-flip symmetry (S.sym): with the determinant vector symmetric under
+// exchanging the alpha and beta strings, W[(kb, ka)][(r, p)] of the
+// alpha-beta product equals W[(ka, kb)][(p, r)], and the beta-beta W at
+// row (a, k2) equals the alpha-alpha W at row (k2, a) (same pair); a term
+// of a product that is not evaluated reads its mirror.
 template <class F>
 PSOCI_HD void ras_terms(const RasView& v, int s, int64_t a, int64_t b, F&& f) {
   const RasSectorDev& S = v.sec[s];
@@ -182,18 +216,30 @@ PSOCI_HD void ras_terms(const RasView& v, int s, int64_t a, int64_t b, F&& f) {
         const int r = ctz(y);
         const int64_t w = v.rem[v.lev[S.nb].rem_off + b * S.nb + tr];
         const int cw = ras_cls_of(v, S.nb - 1, w);
-        const int bi = v.blockid[s][0][cu][cw];
+        const bool mir = S.sym && cu > cw;
+        const int bi = mir ? v.blockid[s][0][cw][cu] : v.blockid[s][0][cu][cw];
         if (bi < 0) {
           f(-1, (int64_t)-1, 0);
           continue;
         }
         const RasBlockDev& B = v.blk[bi];
-        const int pi = v.pidx[(0 * (v.m + 1) + B.budget) * nb2 + p * nbf + r];
+        // local row strings and pair of the evaluated product
+        int64_t ul = mir ? w - B.u0 : u - B.u0, wl = mir ? u - B.w0 : w - B.w0;
+        int pp = mir ? r : p, rr = mir ? p : r;
+        if (B.tri && ul > wl) {
+          const int64_t t = ul;
+          ul = wl;
+          wl = t;
+          const int tq = pp;
+          pp = rr;
+          rr = tq;
+        }
+        const int pi = v.pidx[(0 * (v.m + 1) + B.budget) * nb2 + pp * nbf + rr];
         if (pi < 0) {
           f(-1, (int64_t)-1, 0);
           continue;
         }
-        const int64_t row = (u - B.u0) * B.nw + (w - B.w0);
+        const int64_t row = B.tri ? ras_tri_row(ul, wl, B.nu) : ul * B.nw + wl;
         f(bi, B.wbase + row * B.nqp + pi, (tp + tr) & 1);
       }
     }
@@ -214,7 +260,8 @@ PSOCI_HD void ras_terms(const RasView& v, int s, int64_t a, int64_t b, F&& f) {
         const int r = ctz(y);
         const int64_t k2 = v.rem[v.lev[n - 1].rem_off + u1 * (n - 1) + tr];
         const int ck = ras_cls_of(v, n - 2, k2);
-        const int bi = spin == 0 ? v.blockid[s][1][ck][co] : v.blockid[s][2][co][ck];
+        const bool mir = S.sym && spin == 1;  // beta-beta read from the alpha-alpha product
+        const int bi = (spin == 0 || mir) ? v.blockid[s][1][ck][co] : v.blockid[s][2][co][ck];
         if (bi < 0) {
           f(-1, (int64_t)-1, 0);
           continue;
@@ -225,7 +272,8 @@ PSOCI_HD void ras_terms(const RasView& v, int s, int64_t a, int64_t b, F&& f) {
           f(-1, (int64_t)-1, 0);
           continue;
         }
-        const int64_t row = spin == 0 ? (k2 - B.u0) * B.nw + (o - B.w0) : (o - B.u0) * B.nw + (k2 - B.w0);
+        const int64_t row =
+            (spin == 0 || mir) ? (k2 - B.u0) * B.nw + (o - B.w0) : (o - B.u0) * B.nw + (k2 - B.w0);
         f(bi, B.wbase + row * B.nqp + pi, (tp + tr) & 1);
       }
     }
