@@ -4,7 +4,7 @@ against the all-FP64 GPU run, and the sigma kernel's HBM roofline fraction.
 One "step" is one complete mixed-precision block-Davidson solve for the
 lowest 16 roots, with the space, integrals and (for the stored workload) the
 Hamiltonian already resident in HBM; on the direct workload the subspace
-holds up to 15 x 16 = 240 vectors.  The all-FP64 solve of the same space,
+holds up to 8 x 16 = 128 vectors.  The all-FP64 solve of the same space,
 with the same solver settings, is run once beside it; every root of each
 timed solve must agree with it to 1e-8 Eh or the run fails.
 
@@ -51,7 +51,7 @@ def parse():
     ap.add_argument("--guard-roots", type=int, default=0,
                     help="guard roots in the FP32 phase of the mixed solve")
     ap.add_argument("--max-subspace", type=int, default=None,
-                    help="Davidson subspace cap in vectors (default 15 x roots direct, the library's 5 x roots "
+                    help="Davidson subspace cap in vectors (default 8 x roots direct, the library's 5 x roots "
                          "stored); the same for the FP64 and the mixed solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--graph", type=int, default=0,
@@ -203,7 +203,7 @@ def main():
     mode = api.DIRECT if direct else api.STORED
     k = args.roots
     guard = args.guard_roots
-    msub = args.max_subspace if args.max_subspace is not None else (15 * k if direct else 0)
+    msub = args.max_subspace if args.max_subspace is not None else (8 * k if direct else 0)
     n_sef = int(cfg.nsef().sum())
     print(f"[bench] {args.workload}: ncfg={cfg.ncfg} nsef={n_sef} mode={'direct' if direct else 'stored'} k={k}",
           file=sys.stderr, flush=True)

@@ -310,6 +310,9 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
       collapse(k);
       phase32 = false;
       S.switch_iter = it;
+      // the FP64 phase starts from an orthonormal basis to FP64 precision
+      for (int pass = 0; pass < 2; ++pass)
+        PSOCI_CHECK(sym_orth(0, k) == k, PSOCI_ERR_NUMERIC, "Ritz vectors dependent at the switch");
       sigma_update(0, k);
       continue;
     }
@@ -365,9 +368,40 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     for (int j = 0; j < nsel; ++j) Ssel[(size_t)sel[j] * nsel + j] = 1.0;
     L.gemm_nn(n, k, nsel, R.p, k, Ssel.data(), 0.0, V.p + m, mmax);
     int nb = nsel;
-    for (int pass = 0; pass < 2 && nb > 0; ++pass) {
+    {
+      // block Gram-Schmidt against V[:, 0:m) and symmetric orthonormalisation
+      // of the block, twice.  In the FP32 phase (whose sigma carries 1e-7
+      // relative errors anyway) the second pass runs only when the first
+      // removed more than half of a column's squared norm (the
+      // Daniel-Gragg-Kaufman-Stewart criterion) or left the block's columns
+      // nearly dependent (the smallest eigenvalue of their correlation
+      // matrix below 1e-2); the FP64 phase starts from a re-orthonormalised
+      // basis (below, at the switch) and always takes both passes
+      std::vector<double> G0((size_t)nb * nb), G1((size_t)nb * nb);
+      if (phase32 && m > 0) L.gemm_tn_host(n, nb, nb, V.p + m, mmax, V.p + m, mmax, G0.data());
       project_out(m, nb);
+      bool again = m > 0;
+      if (phase32 && m > 0) {
+        L.gemm_tn_host(n, nb, nb, V.p + m, mmax, V.p + m, mmax, G1.data());
+        again = false;
+        for (int j = 0; j < nb; ++j)
+          if (!(G1[(size_t)j * nb + j] >= 0.5 * G0[(size_t)j * nb + j]) || !(G1[(size_t)j * nb + j] > 0.0))
+            again = true;
+        if (!again) {
+          std::vector<double> C((size_t)nb * nb), lam(nb);
+          for (int i = 0; i < nb; ++i)
+            for (int j = 0; j < nb; ++j)
+              C[(size_t)i * nb + j] =
+                  G1[(size_t)i * nb + j] / std::sqrt(G1[(size_t)i * nb + i] * G1[(size_t)j * nb + j]);
+          if (!sym_eig(nb, C.data(), lam.data()) || !(lam[0] >= 1e-2)) again = true;
+        }
+      }
+      if (again) project_out(m, nb);
       nb = sym_orth(m, nb);
+      if (nb > 0 && again) {
+        project_out(m, nb);
+        nb = sym_orth(m, nb);
+      }
     }
     if (nb == 0) break;  // no new directions: stagnation
     sigma_update(m, nb);
