@@ -5,8 +5,8 @@
 // path of the reference (matrixVectorProductsScatterAccumulateLowLevel...
 // DirectFetch, PsociGAhamiltonian.C:7549-7823, with matrixVectorUpdate
 // :13036-13115), which regenerates every configuration-pair block.  One
-// sigma call of K columns is, per chunk of KC <= 16 (FP32) or 8 (FP64)
-// columns:
+// sigma call of K columns is, per chunk of KC <= 16 columns (FP64: 8 when
+// device memory is short):
 //   k_ras_x2c   SEF vectors -> determinant vectors c[det][KC] (each
 //               determinant belongs to one SEF: a scatter without collisions);
 //   k_ras_gemm  per block: W[row][p-pair][KC] = sum over q-pairs of
@@ -48,9 +48,6 @@ namespace {
 #endif
 #ifndef PSOCI_RAS_VSTORE
 #define PSOCI_RAS_VSTORE 1
-#endif
-#ifndef PSOCI_RAS_WIDE
-#define PSOCI_RAS_WIDE 0
 #endif
 #ifndef PSOCI_RAS_PIPE
 #define PSOCI_RAS_PIPE 0
@@ -143,8 +140,8 @@ __global__ __launch_bounds__(256) void k_ras_didx(const RasView* __restrict__ vp
 
 // One tile of W = D G^T for block bi: PT p-pairs x kMT (row, column)
 // values.  Four wavefronts, each PT p-pairs x 32 values (PT / 16 x 2 MFMA
-// tiles); PT = 128 halves the D gathers per flop where the pair count pads
-// no worse than at 64.
+// tiles); PT = 64, or the whole (16-padded) pair count when it is at most
+// 112, so that no tile computes rows past it.
 // Per chunk of kQC q-pairs: G[p][q] rows and the D entries (c rows of the
 // determinants in didx) into LDS, then kQC / 4 MFMA steps; the global loads
 // of the next chunk are issued before the MFMA steps of the current one
@@ -164,7 +161,7 @@ __global__ __launch_bounds__(256) void k_ras_gemm(const RasBlockDev* __restrict_
   constexpr int VPT = KC < DPT ? KC : DPT;     // consecutive columns one thread gathers per entry
   constexpr int CPT = DPT / VPT;               // (row, pair) entries per thread and chunk
   constexpr int TPC = KC / VPT;                // threads per entry
-  constexpr int GPT = PT * kQC / 256;          // G values per thread and chunk
+  constexpr int GN = (PT * kQC + 255) / 256;   // G values per thread and chunk (at most)
   constexpr int NI = PT / 16;                  // p-subtiles per wavefront
   constexpr bool pipe = sizeof(T) == 8 ? PSOCI_RAS_PIPE64 : PSOCI_RAS_PIPE;
   // row strides of the LDS tiles (compile-time knobs)
@@ -187,17 +184,19 @@ __global__ __launch_bounds__(256) void k_ras_gemm(const RasBlockDev* __restrict_
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = v4{0, 0, 0, 0};
 
-  // per-thread staging of one chunk
-  const int gp = tid / (kQC / GPT), gq = (tid % (kQC / GPT)) * GPT;  // G: row gp, q-pairs gq .. gq + GPT
-  const int gpg = pt * PT + gp;
-  T gv[GPT];
+  // per-thread staging of one chunk: G values tid + 256 i (row e / kQC,
+  // pair e % kQC of the tile's G slice), D entries below
+  T gv[GN];
   T dv[CPT][VPT];
   // chunks of kQC pairs; nqp is a multiple of 16, so the last chunk may
   // hold 16: its other half is loaded as zeros
   auto load = [&](int q0) {
     const int wq = nqp - q0 < kQC ? nqp - q0 : kQC;
 #pragma unroll
-    for (int i = 0; i < GPT; ++i) gv[i] = (gpg < nqp && gq + i < wq) ? Gb[(int64_t)gpg * nqp + q0 + gq + i] : (T)0;
+    for (int i = 0; i < GN; ++i) {
+      const int e = tid + 256 * i, gp = e / kQC, gq = e % kQC, gpg = pt * PT + gp;
+      gv[i] = (e < PT * kQC && gpg < nqp && gq < wq) ? Gb[(int64_t)gpg * nqp + q0 + gq] : (T)0;
+    }
 #pragma unroll
     for (int ci = 0; ci < CPT; ++ci) {
       const int ent = (tid / TPC) * CPT + ci;  // (row, pair) entry: rl * kQC + jq
@@ -229,18 +228,11 @@ __global__ __launch_bounds__(256) void k_ras_gemm(const RasBlockDev* __restrict_
   auto store = [&]() {
     typedef typename VecN<T, 16 / sizeof(T)>::type V;
     constexpr int W16 = 16 / sizeof(T);
-    constexpr bool vec = PSOCI_RAS_VSTORE && (kQC + PSOCI_RAS_GPAD) % W16 == 0 && (kMT + PSOCI_RAS_DPAD) % W16 == 0;
-    if constexpr (vec && GPT % W16 == 0) {
+    constexpr bool vec = PSOCI_RAS_VSTORE && (kMT + PSOCI_RAS_DPAD) % W16 == 0;
 #pragma unroll
-      for (int i = 0; i < GPT / W16; ++i) {
-        V x;
-#pragma unroll
-        for (int w = 0; w < W16; ++w) x[w] = gv[i * W16 + w];
-        *reinterpret_cast<V*>(&Gs[gp][gq + i * W16]) = x;
-      }
-    } else {
-#pragma unroll
-      for (int i = 0; i < GPT; ++i) Gs[gp][gq + i] = gv[i];
+    for (int i = 0; i < GN; ++i) {
+      const int e = tid + 256 * i;
+      if (e < PT * kQC) Gs[e / kQC][e % kQC] = gv[i];
     }
 #pragma unroll
     for (int ci = 0; ci < CPT; ++ci) {
@@ -453,26 +445,31 @@ void ras_chunk(Ctx& c, const T* X, int ldx, T* Y, int ldy, int col0, int kw, Ras
   if (tm) tm->mark(c.stream, 0);
   for (int bi = 0; bi < (int)P.blk.size(); ++bi) {
     const RasBlockDev& B = P.blk[bi];
-    // 128 p-pairs per tile (FP32) where that pads the pairs at most 7 % more
-    // than 64 does
-    const bool wide = PSOCI_RAS_WIDE && sizeof(T) == 4 &&
-                      (B.nqp + 127) / 128 * 128 <= (B.nqp + 63) / 64 * 64 * 107 / 100;
-    const int pt_w = wide ? 128 : 64;
+    // one tile over all p-pairs when there are at most 112, else tiles of 64
+    const int pt_w = B.nqp <= 112 ? B.nqp : 64;
     const int npt = (B.nqp + pt_w - 1) / pt_w;
     const int64_t nmt = (B.nrows * KC + kMT - 1) / kMT;
     const int64_t ntiles = nmt * npt;
     const unsigned grid = (unsigned)(8 * ((ntiles + 7) / 8));
-    bool done = false;
-    if constexpr (sizeof(T) == 4) {
-      if (wide) {
-        hipLaunchKernelGGL((k_ras_gemm<T, KC, 128>), dim3(grid), dim3(256), 0, c.stream, c.d_rasblk.p, bi,
-                           c.d_rasdidx.p, G, cv, W, ntiles, npt);
-        done = true;
-      }
+    const RasBlockDev* blk = c.d_rasblk.p;
+    const int32_t* didx = c.d_rasdidx.p;
+    switch (pt_w) {
+#define PSOCI_RAS_GEMM(PTV)                                                                                  \
+  case PTV:                                                                                                 \
+    hipLaunchKernelGGL((k_ras_gemm<T, KC, PTV>), dim3(grid), dim3(256), 0, c.stream, blk, bi, didx, G, cv, W, \
+                       ntiles, npt);                                                                        \
+    break;
+      PSOCI_RAS_GEMM(16)
+      PSOCI_RAS_GEMM(32)
+      PSOCI_RAS_GEMM(48)
+      PSOCI_RAS_GEMM(80)
+      PSOCI_RAS_GEMM(96)
+      PSOCI_RAS_GEMM(112)
+#undef PSOCI_RAS_GEMM
+      default:
+        hipLaunchKernelGGL((k_ras_gemm<T, KC, 64>), dim3(grid), dim3(256), 0, c.stream, blk, bi, didx, G, cv, W,
+                           ntiles, npt);
     }
-    if (!done)
-      hipLaunchKernelGGL((k_ras_gemm<T, KC, 64>), dim3(grid), dim3(256), 0, c.stream, c.d_rasblk.p, bi,
-                         c.d_rasdidx.p, G, cv, W, ntiles, npt);
     PSOCI_HIP(hipGetLastError());
     if (tm) tm->mark(c.stream, 1 + B.kind);
   }
@@ -484,16 +481,12 @@ void ras_chunk(Ctx& c, const T* X, int ldx, T* Y, int ldy, int col0, int kw, Ras
 
 template <typename T>
 void ras_any_k(Ctx& c, int k, const T* X, int ldx, T* Y, int ldy, RasTiming* tm = nullptr) {
-  const int kmax = sizeof(T) == 4 ? 16 : 8;
+  const int kmax = sizeof(T) == 4 ? 16 : c.ras_kc64;
   for (int c0 = 0; c0 < k; c0 += kmax) {
     const int kw = std::min(kmax, k - c0);
-    if constexpr (sizeof(T) == 4) {
-      if (kw > 8) {
-        ras_chunk<T, 16>(c, X, ldx, Y, ldy, c0, kw, tm);
-        continue;
-      }
-    }
-    if (kw > 4) ras_chunk<T, 8>(c, X, ldx, Y, ldy, c0, kw, tm);
+    if (kw > 8)
+      ras_chunk<T, 16>(c, X, ldx, Y, ldy, c0, kw, tm);
+    else if (kw > 4) ras_chunk<T, 8>(c, X, ldx, Y, ldy, c0, kw, tm);
     else if (kw > 2) ras_chunk<T, 4>(c, X, ldx, Y, ldy, c0, kw, tm);
     else if (kw > 1) ras_chunk<T, 2>(c, X, ldx, Y, ldy, c0, kw, tm);
     else ras_chunk<T, 1>(c, X, ldx, Y, ldy, c0, kw, tm);
@@ -518,7 +511,7 @@ bool build_ras(Ctx& c) {
   }
   const RasPlan& P = c.ras;
   {
-    // device memory: W and the determinant vectors (16 FP32 / 8 FP64
+    // device memory: W and the determinant vectors (16 FP32 / 8 or 16 FP64
     // columns), the term lists; leave room for the caller's vectors
     int64_t terms = 0;
     for (const RasSef& e : P.sef)
@@ -530,6 +523,9 @@ bool build_ras(Ctx& c) {
         (double)P.wunit * (64 + 4) + (double)P.ndet * 64 + (double)terms * 4 + (double)P.sef.size() * 9;
     size_t fr = 0, tot = 0;
     PSOCI_HIP(hipMemGetInfo(&fr, &tot));
+    // FP64 in chunks of 16 columns (whole 128-byte W and c pieces) when W and
+    // c fit twice over in 60 % of the free memory, else chunks of 8
+    c.ras_kc64 = need + (double)P.wunit * 64 + (double)P.ndet * 64 <= 0.6 * (double)fr ? 16 : 8;
     if (need > 0.6 * (double)fr) {
       c.ras_why = "needs " + std::to_string((int64_t)(need / 1e9)) + " GB of device memory, " +
                   std::to_string((int64_t)(fr / 1000000000)) + " GB free";
@@ -574,8 +570,8 @@ bool build_ras(Ctx& c) {
   v.pidx = c.d_raspidx.p;
   v.sef = c.d_rassef.p;
   c.d_rasview.upload(&v, 1, c.stream);
-  // determinant vectors and W: 16 FP32 or 8 FP64 columns per chunk
-  c.d_rasc.alloc((size_t)P.ndet * 16 * sizeof(float));
+  // determinant vectors and W: 16 FP32 or ras_kc64 FP64 columns per chunk
+  c.d_rasc.alloc((size_t)P.ndet * (c.ras_kc64 == 16 ? 16 * sizeof(double) : 16 * sizeof(float)));
   // the D entries' determinants
   c.d_rasdidx.alloc(P.wunit);
   for (int bi = 0; bi < (int)P.blk.size(); ++bi) {
@@ -610,7 +606,7 @@ bool build_ras(Ctx& c) {
     c.sync();
     PSOCI_CHECK(!b, PSOCI_ERR_STATE, "string-driven sigma: inconsistent term lists");
   }
-  c.d_rasw.alloc((size_t)P.wunit * 16 * sizeof(float));
+  c.d_rasw.alloc((size_t)P.wunit * (c.ras_kc64 == 16 ? 16 * sizeof(double) : 16 * sizeof(float)));
   c.sync();
   c.have_ras = true;
   return true;

@@ -112,3 +112,28 @@ int hr_sigma(int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, con
 }
 
 }  // extern "C"
+
+extern "C" {
+// Per-block summary of the plan: rows, nq, nqp, kind, sector, classes (9
+// int64 per block); returns the block count (at most cap).
+int hr_blocks(int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, const int32_t* jsym, const double* one,
+              const double* two, int64_t* out, int cap) {
+  Space sp;
+  RasPlan P;
+  try {
+    build_space(sp, nbf, nelec, ksym, ncfg, occ, jsym);
+    ras_plan(P, sp, one, two);
+  } catch (const Error&) {
+    return -1;
+  }
+  if (!P.ok) return 0;
+  int nb = 0;
+  for (const RasBlockDev& B : P.blk) {
+    if (nb >= cap) break;
+    int64_t* o = out + 9 * nb++;
+    o[0] = B.nrows; o[1] = B.nq; o[2] = B.nqp; o[3] = B.kind; o[4] = B.sector; o[5] = B.cu; o[6] = B.cw;
+    o[7] = B.tri; o[8] = B.budget;
+  }
+  return nb;
+}
+}
