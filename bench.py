@@ -405,7 +405,7 @@ def main():
             "bound": "mfma", "achieved": round(ach, 2), "peak": MFMA_PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
             "frac": round(ach / MFMA_PEAK_TFLOPS["fp32"], 4),
             "traffic": traffic["bytes_per_launch"] if traffic else None,
-            "kernel": (f"k_ras_gemm<float,16,*> (the string-driven sigma's dense products: all "
+            "kernel": (f"k_ras_gemm_lds<float,16,*> (the string-driven sigma's dense products: all "
                        f"{ras['blocks']} launches of one {kr}-column FP32 sigma)"),
             "algorithmic_flops_per_sigma": g32_fl, "gemm_ms_per_sigma": round(g32_ms, 4),
             "fp64": {"achieved": round(ach64f, 2), "peak": MFMA_PEAK_TFLOPS["fp64"],

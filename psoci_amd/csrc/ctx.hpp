@@ -109,9 +109,9 @@ struct Ctx {
   DevBuf<double> d_rasg64;
   DevBuf<float> d_rasg32;
   DevBuf<RasView> d_rasview;
-  DevBuf<char> d_rasc, d_rasw;  // determinant vectors [ndet][16 FP32 | ras_kc64 FP64], W [wunit][same]
+  DevBuf<char> d_rasc, d_rasw;  // determinant vectors [2 ndet + 2][16 FP32 | ras_kc64 FP64], W [wunit][same]
   int ras_kc64 = 8;             // FP64 columns per chunk (16 when memory allows, else 8)
-  DevBuf<int32_t> d_rasdidx;    // [wunit]: determinant (| negative << 31) of every D entry
+  DevBuf<int32_t> d_rasdidx;    // [dunit]: signed vector row (2 det + negative) of every D entry
   DevBuf<int64_t> d_rastoff;    // nsef + 1: first term of each SEF
   DevBuf<int32_t> d_rasterm;    // signed W offsets (bit 31: negative)
   DevBuf<int8_t> d_rasscl;      // per SEF: 0 1/sqrt(2), 1 sqrt(2), 2 one

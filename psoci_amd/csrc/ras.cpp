@@ -251,7 +251,9 @@ void ras_plan(RasPlan& P, const Space& sp, const double* one, const double* two)
     P.sec.push_back(S);
   }
   P.ndet = det0;
-  if (P.ndet >= ((int64_t)1 << 31)) return fail("more than 2^31 determinants");
+  // the D index table names signed vector rows 2 det + negative (and the zero
+  // rows 2 ndet, 2 ndet + 1) in 31 bits
+  if (P.ndet >= ((int64_t)1 << 30) - 1) return fail("more than 2^30 determinants");
   // pair lists per (shape, budget): shape 0 ordered (q, s), shape 1 q > s;
   // q-major so that consecutive pairs reach determinants of one alpha string
   std::vector<int64_t> qoff(2 * (m + 1)), qn(2 * (m + 1)), qnp(2 * (m + 1));
@@ -278,7 +280,7 @@ void ras_plan(RasPlan& P, const Space& sp, const double* one, const double* two)
   const int nb2 = nbf * nbf;
   auto g2d = [&](int p, int q, int r, int s) { return g2(two, p, q, r, s); };
   auto h1d = [&](int p, int q) { return h1(one, p, q); };
-  int64_t wunit = 0;
+  int64_t wunit = 0, dunit = 0;
   double flops = 0.0;
   for (size_t si = 0; si < nsec; ++si) {
     const RasSectorDev& S = P.sec[si];
@@ -334,6 +336,8 @@ void ras_plan(RasPlan& P, const Space& sp, const double* one, const double* two)
           B.qoff = qoff[id];
           B.wbase = wunit;
           wunit += B.nrows * B.nqp;
+          B.dbase = dunit;
+          dunit += (B.nrows + 15) / 16 * 16 * B.nqp;
           flops += 2.0 * (double)B.nrows * B.nq * B.nq;
           if (gbase_by_budget[bud] < 0) {
             // the integral matrix of this (sector, kind, budget)
@@ -344,7 +348,10 @@ void ras_plan(RasPlan& P, const Space& sp, const double* one, const double* two)
             for (int i = 0; i < B.nq; ++i) {
               const int16_t pr = P.qlist[B.qoff + i];
               const int p = pr & 0xff, r = (pr >> 8) & 0xff;
-              for (int j = 0; j < B.nq; ++j) {
+              // symmetric: (pq|rs) = (qp|sr); the upper triangle is
+              // computed and mirrored, so G reads the same by rows and
+              // columns bit for bit
+              for (int j = i; j < B.nq; ++j) {
                 const int16_t qs = P.qlist[B.qoff + j];
                 const int q = qs & 0xff, s = (qs >> 8) & 0xff;
                 double val;
@@ -364,6 +371,7 @@ void ras_plan(RasPlan& P, const Space& sp, const double* one, const double* two)
                   val = 0.5 * (vp(p, q, r, s) - vp(r, q, p, s) - vp(p, s, r, q) + vp(r, s, p, q));
                 }
                 G[(int64_t)i * np + j] = val;
+                G[(int64_t)j * np + i] = val;
               }
             }
           }
@@ -376,6 +384,7 @@ void ras_plan(RasPlan& P, const Space& sp, const double* one, const double* two)
   (void)nb2;
   if (wunit >= ((int64_t)1 << 31)) return fail("more than 2^31 W elements per column");
   P.wunit = wunit;
+  P.dunit = dunit;
   P.gemm_flops_per_col = flops;
   // SEF -> determinants: SEF k of a configuration with open shells is
   // (D_2k + mult D_2k+1)/sqrt(2), each D = (-1)^phase times its canonical

@@ -93,6 +93,7 @@ struct RasBlockDev {
   int64_t qoff;   // first pair in qlist[]
   int64_t goff;   // first element of G (nqp x nqp, row-major [p-pair][q-pair])
   int64_t wbase;  // W offset per column: W element (wbase + row * nqp + pidx) * KC + v
+  int64_t dbase;  // first entry in the D index table (rows padded to a multiple of 16)
 };
 
 // The determinants of one SEF (one or two): alpha / beta global string
@@ -302,6 +303,7 @@ struct RasPlan {
   std::vector<RasSef> sef;
   int64_t ndet = 0;   // evaluated determinants (all sectors with n_alpha >= n_beta)
   int64_t wunit = 0;  // W elements per column
+  int64_t dunit = 0;  // D index table entries
   double gemm_flops_per_col = 0.0;  // 2 * sum over blocks of rows * nq^2
 };
 

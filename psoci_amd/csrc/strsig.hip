@@ -7,19 +7,22 @@
 // :13036-13115), which regenerates every configuration-pair block.  One
 // sigma call of K columns is, per chunk of KC <= 16 columns (FP64: 8 when
 // device memory is short):
-//   k_ras_x2c   SEF vectors -> determinant vectors c[det][KC] (each
-//               determinant belongs to one SEF: a scatter without collisions);
+//   k_ras_x2c   SEF vectors -> determinant vectors c[2 det + s][KC] = (-1)^s
+//               c_det (each determinant belongs to one SEF: a scatter
+//               without collisions; both signs, so a product reads an entry
+//               of either sign without arithmetic), rows 2 ndet and 2 ndet +
+//               1 zero;
 //   k_ras_gemm  per block: W[row][p-pair][KC] = sum over q-pairs of
 //               G[p-pair][q-pair] D[row][q-pair][KC], D gathered from c into
-//               LDS through the determinant of each entry (built once from
+//               LDS through the signed row of each entry (built once from
 //               the string addition tables by k_ras_didx), the product on
 //               MFMA (v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64);
 //   k_ras_y     per SEF and column: y = scale * sum of +-W over the RI
 //               terms of its determinants (the term list -- signed W offsets
 //               -- is built once at setup by k_ras_terms), written once (a
 //               gather: deterministic, no atomics).
-// HBM layout: c [ndet][KC] (64 B per determinant at KC = 16 FP32), W
-// [block rows][nqp][KC] so that a term of k_ras_y reads KC contiguous values.
+// HBM layout: c [2 ndet + 2][KC] (64 B per row at KC = 16 FP32), W [block
+// rows][nqp][KC] so that a term of k_ras_y reads KC contiguous values.
 #include <algorithm>
 #include <cstdlib>
 
@@ -54,6 +57,11 @@ namespace {
 #endif
 #ifndef PSOCI_RAS_PIPE64
 #define PSOCI_RAS_PIPE64 1
+#endif
+// 16- and 8-column chunks through the LDS-copy product k_ras_gemm_lds (0:
+// the register-staged k_ras_gemm for every width)
+#ifndef PSOCI_RAS_DMA
+#define PSOCI_RAS_DMA 1
 #endif
 constexpr int kMT = 128;            // (row, column) values per GEMM tile
 // q-pairs per LDS chunk (FP32, FP64)
@@ -91,12 +99,14 @@ __device__ __forceinline__ T sef_coef(uint8_t f) {
   return (f & 1) ? -m : m;
 }
 
-// c[det][KC] = coefficient * x[sef][col0 + q] (zero for q >= kw)
+// c[2 det][KC] = coefficient * x[sef][col0 + q] (zero for q >= kw), c[2 det
+// + 1] its negative; the zero rows 2 ndet, 2 ndet + 1
 template <typename T, int KC>
 __global__ __launch_bounds__(256) void k_ras_x2c(const RasView* __restrict__ vp, const T* __restrict__ X, int ldx,
                                                  int col0, int kw, T* __restrict__ c) {
   const RasView& v = *vp;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < 2 * KC) c[2 * v.ndet * KC + t] = (T)0;
   const int64_t s = t / KC;
   const int q = (int)(t % KC);
   if (s >= v.nsef) return;
@@ -107,7 +117,11 @@ __global__ __launch_bounds__(256) void k_ras_x2c(const RasView* __restrict__ vp,
     const int ca = ras_cls_of(v, S.na, e.a[d]), cb = ras_cls_of(v, S.nb, e.b[d]);
     const int64_t det =
         ras_det(v, e.sec[d], ca, e.a[d] - v.lev[S.na].cls_off[ca], cb, e.b[d] - v.lev[S.nb].cls_off[cb]);
-    if (det >= 0 && det < v.ndet) c[det * KC + q] = sef_coef<T>(e.flag[d]) * x;
+    if (det >= 0 && det < v.ndet) {
+      const T y = sef_coef<T>(e.flag[d]) * x;
+      c[2 * det * KC + q] = y;
+      c[(2 * det + 1) * KC + q] = -y;
+    }
   }
 }
 
@@ -122,20 +136,21 @@ struct VecN<double, 2> {
   typedef double type __attribute__((ext_vector_type(2)));
 };
 
-// Determinant of every D entry (built once): didx[wbase + row * nqp + j]
-// = det | negative << 31 for pair j of block row `row`, kRasNoDet when the
-// pair cannot be added to the row's strings (ras_d_entry).
-constexpr int32_t kRasNoDet = 0x7fffffff;
+// Signed vector row of every D entry (built once): didx[dbase + row * nqp +
+// j] = 2 det + negative for pair j of block row `row`, the zero row 2 ndet
+// when the pair cannot be added to the row's strings (ras_d_entry) and for
+// the padding rows past nrows (up to a multiple of 16, so that whole tiles
+// read their entries without bounds checks).
 __global__ __launch_bounds__(256) void k_ras_didx(const RasView* __restrict__ vp, int bi, int32_t* __restrict__ didx) {
   const RasView& v = *vp;
   const RasBlockDev& B = v.blk[bi];
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= B.nrows * B.nqp) return;
+  if (e >= (B.nrows + 15) / 16 * 16 * B.nqp) return;
   const int64_t row = e / B.nqp;
   const int j = (int)(e % B.nqp);
   int neg = 0;
-  const int64_t det = ras_d_entry(v, B, row, j, neg);
-  didx[B.wbase + e] = (det < 0 || det >= v.ndet) ? kRasNoDet : (int32_t)((uint32_t)det | ((uint32_t)neg << 31));
+  const int64_t det = row < B.nrows ? ras_d_entry(v, B, row, j, neg) : -1;
+  didx[B.dbase + e] = (det < 0 || det >= v.ndet) ? (int32_t)(2 * v.ndet) : (int32_t)(2 * det + neg);
 }
 
 // One tile of W = D G^T for block bi: PT p-pairs x kMT (row, column)
@@ -151,8 +166,8 @@ __global__ __launch_bounds__(256) void k_ras_didx(const RasView* __restrict__ vp
 template <typename T, int KC, int PT>
 __global__ __launch_bounds__(256) void k_ras_gemm(const RasBlockDev* __restrict__ blk, int bi,
                                                   const int32_t* __restrict__ didx, const T* __restrict__ G,
-                                                  const T* __restrict__ c, T* __restrict__ W, int64_t ntiles,
-                                                  int npt) {
+                                                  const T* __restrict__ c, int32_t zrow, T* __restrict__ W,
+                                                  int64_t ntiles, int npt) {
   typedef Mfma<T> M;
   constexpr int kQC = ras_qc<T>();
   typedef typename M::v4 v4;
@@ -177,7 +192,7 @@ __global__ __launch_bounds__(256) void k_ras_gemm(const RasBlockDev* __restrict_
   const int64_t nrows = B.nrows;
   const int nqp = B.nqp;
   const T* __restrict__ Gb = G + B.goff;
-  const int32_t* __restrict__ dx = didx + B.wbase;
+  const int32_t* __restrict__ dx = didx + B.dbase;
   v4 acc[NI][2];
 #pragma unroll
   for (int i = 0; i < NI; ++i)
@@ -203,25 +218,21 @@ __global__ __launch_bounds__(256) void k_ras_gemm(const RasBlockDev* __restrict_
       const int v0 = (tid % TPC) * VPT;
       const int rl = ent / kQC, jq = ent % kQC;
       const int64_t row = mt * RT + rl;
+      // the entry's signed row (the zero row past the block or the pairs)
+      const int32_t e = (row < nrows && jq < wq) ? dx[row * nqp + q0 + jq] : zrow;
+      const T* __restrict__ src = c + (int64_t)e * KC + v0;
+      if constexpr (VPT * sizeof(T) % 16 == 0) {
+        typedef typename VecN<T, 16 / sizeof(T)>::type V;
+        constexpr int W16 = 16 / sizeof(T);
 #pragma unroll
-      for (int i = 0; i < VPT; ++i) dv[ci][i] = (T)0;
-      const int32_t e = (row < nrows && jq < wq) ? dx[row * nqp + q0 + jq] : kRasNoDet;
-      if (e != kRasNoDet) {
-        const bool neg = e < 0;
-        const T* __restrict__ src = c + (int64_t)(e & 0x7fffffff) * KC + v0;
-        if constexpr (VPT * sizeof(T) % 16 == 0) {
-          typedef typename VecN<T, 16 / sizeof(T)>::type V;
-          constexpr int W16 = 16 / sizeof(T);
+        for (int i = 0; i < VPT / W16; ++i) {
+          const V x = reinterpret_cast<const V*>(src)[i];
 #pragma unroll
-          for (int i = 0; i < VPT / W16; ++i) {
-            const V x = reinterpret_cast<const V*>(src)[i];
-#pragma unroll
-            for (int w = 0; w < W16; ++w) dv[ci][i * W16 + w] = neg ? -x[w] : x[w];
-          }
-        } else {
-#pragma unroll
-          for (int i = 0; i < VPT; ++i) dv[ci][i] = neg ? -src[i] : src[i];
+          for (int w = 0; w < W16; ++w) dv[ci][i * W16 + w] = x[w];
         }
+      } else {
+#pragma unroll
+        for (int i = 0; i < VPT; ++i) dv[ci][i] = src[i];
       }
     }
   };
@@ -299,6 +310,199 @@ __global__ __launch_bounds__(256) void k_ras_gemm(const RasBlockDev* __restrict_
         if (row < nrows && p < nqp) W[(wb + row * nqp + p) * KC + vq] = acc[i][j][t];
       }
     }
+}
+
+// The 16- and 8-column product with its operands copied straight from global
+// memory into LDS (global_load_lds_dwordx4: each lane names its own 16-byte
+// source, the wavefront's 64 pieces land contiguously at one LDS address),
+// into two LDS stages: the copies of chunk n + 1 are in flight while the
+// MFMA steps consume chunk n, with one barrier per chunk and no staging
+// registers.  Layouts in LDS:
+//   G stage [q][PT] -- q-pair rows of the tile's p-pairs, read from G's q
+//     rows (G is symmetric, so row q holds column q); at FP32 the 16-byte
+//     pieces of odd rows swap halves of every 128 bytes when PT / 16 is even,
+//     which keeps the two rows one 32-lane operand read touches in separate
+//     banks;
+//   D stage [row][q][KC] -- one KC-value piece per (row, pair) entry, copied
+//     from the entry's signed row of c (the zero row for a missing
+//     determinant or a padding row), so no arithmetic touches the operands
+//     between the copy and the MFMA; wave w copies and reads only its own
+//     rows of the tile (2w, 2w + 1 at 16 columns; 4w .. 4w + 3 at 8).
+// wavefronts per SIMD the LDS-copy product is compiled for (a register
+// budget: the compiler would otherwise hold more k-steps' operands in flight
+// than occupancy can afford; at FP32 and 64 p-pairs the budget of 6 spills a
+// few set-up values outside the chunk loop, measured 2 % faster than 5)
+template <typename T, int PT>
+constexpr int ras_lds_waves() {
+  return sizeof(T) == 8 ? (PT <= 64 ? 3 : 2) : (PT <= 64 ? 6 : 4);
+}
+
+template <int N>
+struct IntC {
+  static constexpr int value = N;
+};
+
+// 16 bytes from this lane's global address to LDS at dst + 16 * lane (dst
+// uniform over the wavefront); completion is counted by vmcnt.  Written as
+// an asm statement so that the compiler, which cannot tell the two LDS
+// stages apart, does not wait for every copy in flight before each LDS read:
+// the kernel waits for its copies itself (vmcnt(0) before the barrier that
+// publishes a stage).  M0 carries the LDS address; nothing else in the
+// kernels that use this keeps a value in M0.
+__device__ __forceinline__ void lds_copy16(const void* src, void* dst) {
+  const uint32_t lds = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) void*)dst;
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src),
+               "s"(__builtin_amdgcn_readfirstlane(lds))
+               : "memory", "m0");
+}
+
+template <typename T, int KC, int PT, int WPE = ras_lds_waves<T, PT>()>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
+void k_ras_gemm_lds(const RasBlockDev* __restrict__ blk, int bi,
+                                                      const int32_t* __restrict__ didx, const T* __restrict__ G,
+                                                      const T* __restrict__ c, T* __restrict__ W, int64_t ntiles,
+                                                      int npt) {
+  typedef Mfma<T> M;
+  typedef typename M::v4 v4;
+  static_assert(KC == 8 || KC == 16, "8 or 16 columns");
+  constexpr int QC = 16, RT = kMT / KC;  // 8 or 16 block rows per tile
+  constexpr int RW = RT / 4;             // rows per wavefront
+  constexpr int NI = PT / 16;            // p-subtiles per wavefront
+  constexpr int SL = 16 / sizeof(T);     // values per 16-byte piece
+  constexpr int SPE = KC / SL;           // pieces per D entry
+  constexpr int EPI = 64 / SPE;          // D entries per copy instruction
+  constexpr int DI = RW * QC / EPI;      // D copies per wavefront and chunk
+  constexpr int GSR = PT / SL;           // pieces per G row
+  constexpr int GI = QC * GSR / 64;      // G copies per chunk
+  constexpr int GW = (GI + 3) / 4;       // G copies per wavefront (at most)
+  constexpr bool SWZ = sizeof(T) == 4 && (PT / 16) % 2 == 0;
+  static_assert(QC * GSR % 64 == 0 && RW * QC % EPI == 0, "copy shapes");
+  __shared__ __align__(16) T Gs[2][QC * PT];
+  __shared__ __align__(16) T Ds[2][RT * QC * KC];
+  const int64_t per = (ntiles + 7) / 8;
+  const int64_t tile = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (tile >= ntiles) return;  // uniform per workgroup
+  const int64_t mt = tile / npt;
+  const int pt = (int)(tile % npt);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const RasBlockDev& B = blk[bi];
+  const int64_t nrows = B.nrows;
+  const int nqp = B.nqp, nch = nqp / QC;
+
+  // this lane's sources as offsets advanced by one chunk per step: its D
+  // entries in the index table (rows padded to whole tiles: no bounds
+  // checks) and its G pieces (rows q of G's symmetric tile; a piece past nqp
+  // repeats its row's first, feeding only output rows that are not stored)
+  const int32_t* __restrict__ dx = didx + B.dbase;
+  const T* __restrict__ Gb = G + B.goff;
+  int dxo[DI];
+#pragma unroll
+  for (int d = 0; d < DI; ++d) {
+    const int en = d * EPI + lane / SPE;  // entry among the wavefront's RW x QC
+    dxo[d] = (int)((mt * RT + RW * wave + en / QC) * nqp + en % QC);
+  }
+  int go[GW];
+#pragma unroll
+  for (int gi = 0; gi < GW; ++gi) {
+    const int sl = 64 * (wave + 4 * gi) + lane, k = sl / GSR;
+    const int r = SWZ ? (sl % GSR) ^ (4 * (k & 1)) : sl % GSR;
+    const int p = pt * PT + r * SL;
+    go[gi] = k * nqp + (p < nqp ? p : 0);
+  }
+  const int v0 = SL * (lane % SPE);
+  auto fetch_idx = [&](int ch, int32_t (&ix)[DI]) {
+#pragma unroll
+    for (int d = 0; d < DI; ++d) ix[d] = dx[dxo[d] + ch * QC];
+  };
+  // the copies of chunk ch into stage st: D pieces first (they use the
+  // indices loaded a chunk earlier, which the compiler waits for), then G
+  auto issue = [&](int ch, int st, const int32_t (&ix)[DI]) {
+#pragma unroll
+    for (int d = 0; d < DI; ++d)
+      lds_copy16(c + (int64_t)ix[d] * KC + v0, &Ds[st][(RW * wave * QC + d * EPI) * KC]);
+    const int qo = ch * QC * nqp;
+#pragma unroll
+    for (int gi = 0; gi < GW; ++gi)
+      if (wave + 4 * gi < GI) lds_copy16(Gb + (go[gi] + qo), &Gs[st][64 * (wave + 4 * gi) * SL]);
+  };
+
+  // lane offsets of the MFMA operand reads (k row kr = lane / 16 within a
+  // step): p-subtile i of G sits at a0[i & 1] + 16 i (with the swap of
+  // 64-byte halves on odd rows: 16 (i ^ 1) = 16 i -+ 16); the B operand's
+  // column lane & 15 is value vl of row RW wave + rl (+ 16 j / KC for
+  // subtile j)
+  const int kr = lane >> 4;
+  const int sw = SWZ ? 16 * (kr & 1) : 0;
+  const int a0[2] = {kr * PT + (lane & 15) + sw, kr * PT + (lane & 15) - sw};
+  const int rl = (lane & 15) / KC, vl = (lane & 15) % KC;
+  const int boff = ((RW * wave + rl) * QC + kr) * KC + vl;
+  // pairs past nq are padding (zero G rows and columns, zero D entries): a
+  // tile's last p-subtiles that hold only padding are skipped
+  const int nq = B.nq;
+  const int niv = ((nq - pt * PT < PT ? nq - pt * PT : PT) + 15) / 16;
+
+  // the whole tile with NV live p-subtiles
+  auto run = [&](auto nvc) {
+    constexpr int NV = decltype(nvc)::value;
+    v4 acc[NV][2];
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = v4{0, 0, 0, 0};
+    // one k-step of 4 pairs on stage st
+    auto step = [&](int st, int kk) {
+      const T* __restrict__ gs = Gs[st];
+      const T* __restrict__ ds = Ds[st];
+      T a[NV], b[2];
+#pragma unroll
+      for (int i = 0; i < NV; ++i) a[i] = gs[a0[i & 1] + 16 * i + 4 * kk * PT];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) b[j] = ds[boff + (16 / KC * j * QC + 4 * kk) * KC];
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = M::op(a[i], b[j], acc[i][j]);
+    };
+    int32_t ixn[DI];
+    fetch_idx(0, ixn);
+    issue(0, 0, ixn);
+    if (nch > 1) fetch_idx(1, ixn);
+    for (int ch = 0; ch < nch; ++ch) {
+      const int st = ch & 1;
+      // this wavefront's copies of chunk ch have landed, every wavefront's
+      // have after the barrier, and stage st ^ 1 is free (its reads of
+      // chunk ch - 1 ended before the barrier)
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __syncthreads();
+      if (ch + 1 < nch) {
+        issue(ch + 1, st ^ 1, ixn);
+        if (ch + 2 < nch) fetch_idx(ch + 2, ixn);
+      }
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) step(st, kk);
+    }
+    const int64_t wb = B.wbase;
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int64_t row = mt * RT + RW * wave + 16 / KC * j + rl;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int p = pt * PT + 16 * i + M::row(t, lane);
+          if (row < nrows && p < nqp) W[(wb + row * nqp + p) * KC + vl] = acc[i][j][t];
+        }
+      }
+  };
+  // tiles cover the pairs exactly except, for some widths, the last one
+  if (niv >= NI) {
+    run(IntC<NI>{});
+  } else if constexpr (NI > 1) {
+    if (niv == 1) run(IntC<1>{});
+    else if (NI > 2 && niv == 2) run(IntC<(NI > 2 ? 2 : 1)>{});
+    else if (NI > 3 && niv == 3) run(IntC<(NI > 3 ? 3 : 1)>{});
+    else run(IntC<NI>{});
+  }
 }
 
 // The term list of every SEF (built once): for each of its determinants
@@ -430,6 +634,16 @@ struct RasTiming {
   }
 };
 
+// p-pairs per product tile: one tile when there are at most 112, else the
+// first of 64, 96, 80, 112 and 48 that divides the (16-padded) pair count,
+// else 64 with a narrower last tile
+inline int ras_tile_width(int nqp) {
+  if (nqp <= 112) return nqp;
+  for (int w : {64, 96, 80, 112, 48})
+    if (nqp % w == 0) return w;
+  return 64;
+}
+
 template <typename T, int KC>
 void ras_chunk(Ctx& c, const T* X, int ldx, T* Y, int ldy, int col0, int kw, RasTiming* tm = nullptr) {
   const RasPlan& P = c.ras;
@@ -445,19 +659,38 @@ void ras_chunk(Ctx& c, const T* X, int ldx, T* Y, int ldy, int col0, int kw, Ras
   if (tm) tm->mark(c.stream, 0);
   for (int bi = 0; bi < (int)P.blk.size(); ++bi) {
     const RasBlockDev& B = P.blk[bi];
-    // one tile over all p-pairs when there are at most 112, else tiles of 64
-    const int pt_w = B.nqp <= 112 ? B.nqp : 64;
+    const int pt_w = ras_tile_width(B.nqp);
     const int npt = (B.nqp + pt_w - 1) / pt_w;
     const int64_t nmt = (B.nrows * KC + kMT - 1) / kMT;
     const int64_t ntiles = nmt * npt;
     const unsigned grid = (unsigned)(8 * ((ntiles + 7) / 8));
     const RasBlockDev* blk = c.d_rasblk.p;
     const int32_t* didx = c.d_rasdidx.p;
+    const int32_t zrow = (int32_t)(2 * P.ndet);  // the zero row of c
+    if constexpr ((KC == 16 || KC == 8) && PSOCI_RAS_DMA) {
+      switch (pt_w) {
+#define PSOCI_RAS_GEMML(PTV)                                                                                \
+  case PTV:                                                                                                \
+    hipLaunchKernelGGL((k_ras_gemm_lds<T, KC, PTV>), dim3(grid), dim3(256), 0, c.stream, blk, bi, didx, G, cv, \
+                       W, ntiles, npt);                                                                    \
+    break;
+        PSOCI_RAS_GEMML(16)
+        PSOCI_RAS_GEMML(32)
+        PSOCI_RAS_GEMML(48)
+        PSOCI_RAS_GEMML(80)
+        PSOCI_RAS_GEMML(96)
+        PSOCI_RAS_GEMML(112)
+#undef PSOCI_RAS_GEMML
+        default:
+          hipLaunchKernelGGL((k_ras_gemm_lds<T, KC, 64>), dim3(grid), dim3(256), 0, c.stream, blk, bi, didx, G, cv,
+                             W, ntiles, npt);
+      }
+    } else
     switch (pt_w) {
 #define PSOCI_RAS_GEMM(PTV)                                                                                  \
   case PTV:                                                                                                 \
-    hipLaunchKernelGGL((k_ras_gemm<T, KC, PTV>), dim3(grid), dim3(256), 0, c.stream, blk, bi, didx, G, cv, W, \
-                       ntiles, npt);                                                                        \
+    hipLaunchKernelGGL((k_ras_gemm<T, KC, PTV>), dim3(grid), dim3(256), 0, c.stream, blk, bi, didx, G, cv, zrow, \
+                       W, ntiles, npt);                                                                     \
     break;
       PSOCI_RAS_GEMM(16)
       PSOCI_RAS_GEMM(32)
@@ -467,8 +700,8 @@ void ras_chunk(Ctx& c, const T* X, int ldx, T* Y, int ldy, int col0, int kw, Ras
       PSOCI_RAS_GEMM(112)
 #undef PSOCI_RAS_GEMM
       default:
-        hipLaunchKernelGGL((k_ras_gemm<T, KC, 64>), dim3(grid), dim3(256), 0, c.stream, blk, bi, didx, G, cv, W,
-                           ntiles, npt);
+        hipLaunchKernelGGL((k_ras_gemm<T, KC, 64>), dim3(grid), dim3(256), 0, c.stream, blk, bi, didx, G, cv, zrow,
+                           W, ntiles, npt);
     }
     PSOCI_HIP(hipGetLastError());
     if (tm) tm->mark(c.stream, 1 + B.kind);
@@ -519,13 +752,13 @@ bool build_ras(Ctx& c) {
         const RasSectorDev& S = P.sec[e.sec[d]];
         terms += (int64_t)S.na * S.nb + (int64_t)S.na * (S.na - 1) / 2 + (int64_t)S.nb * (S.nb - 1) / 2;
       }
-    const double need =
-        (double)P.wunit * (64 + 4) + (double)P.ndet * 64 + (double)terms * 4 + (double)P.sef.size() * 9;
+    const double need = (double)P.wunit * 64 + (double)P.dunit * 4 + (double)P.ndet * 128 + (double)terms * 4 +
+                        (double)P.sef.size() * 9;
     size_t fr = 0, tot = 0;
     PSOCI_HIP(hipMemGetInfo(&fr, &tot));
     // FP64 in chunks of 16 columns (whole 128-byte W and c pieces) when W and
     // c fit twice over in 60 % of the free memory, else chunks of 8
-    c.ras_kc64 = need + (double)P.wunit * 64 + (double)P.ndet * 64 <= 0.6 * (double)fr ? 16 : 8;
+    c.ras_kc64 = need + (double)P.wunit * 64 + (double)P.ndet * 128 <= 0.6 * (double)fr ? 16 : 8;
     if (need > 0.6 * (double)fr) {
       c.ras_why = "needs " + std::to_string((int64_t)(need / 1e9)) + " GB of device memory, " +
                   std::to_string((int64_t)(fr / 1000000000)) + " GB free";
@@ -571,12 +804,12 @@ bool build_ras(Ctx& c) {
   v.sef = c.d_rassef.p;
   c.d_rasview.upload(&v, 1, c.stream);
   // determinant vectors and W: 16 FP32 or ras_kc64 FP64 columns per chunk
-  c.d_rasc.alloc((size_t)P.ndet * (c.ras_kc64 == 16 ? 16 * sizeof(double) : 16 * sizeof(float)));
-  // the D entries' determinants
-  c.d_rasdidx.alloc(P.wunit);
+  c.d_rasc.alloc((size_t)(2 * P.ndet + 2) * (c.ras_kc64 == 16 ? 16 * sizeof(double) : 16 * sizeof(float)));
+  // the D entries' signed rows
+  c.d_rasdidx.alloc(P.dunit);
   for (int bi = 0; bi < (int)P.blk.size(); ++bi) {
-    hipLaunchKernelGGL(k_ras_didx, dim3(nblk(P.blk[bi].nrows * P.blk[bi].nqp, 256)), dim3(256), 0, c.stream,
-                       c.d_rasview.p, bi, c.d_rasdidx.p);
+    const int64_t n = (P.blk[bi].nrows + 15) / 16 * 16 * P.blk[bi].nqp;
+    hipLaunchKernelGGL(k_ras_didx, dim3(nblk(n, 256)), dim3(256), 0, c.stream, c.d_rasview.p, bi, c.d_rasdidx.p);
     PSOCI_HIP(hipGetLastError());
   }
   // the SEF term lists: each determinant of sector (na, nb) has na nb +
