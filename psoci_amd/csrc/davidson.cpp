@@ -143,7 +143,10 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     c.d_pad.reserve((size_t)2 * n * 32 * std::max<size_t>(es, 8));
   }
   auto device_step = [&](int c0, int nb) {
-    if (phase32) {
+    if (phase32 && p.sigma_mode != PSOCI_SIGMA_STORED && direct_engine(c) == 2) {
+      // the string-driven sigma rounds and widens on the fly
+      sigma_ras_f32_io64(c, nb, V.p + c0, mmax, HV.p + c0, mmax);
+    } else if (phase32) {
       L.to_f32(n, nb, V.p + c0, mmax, X32.p, nb);
       if (p.sigma_mode == PSOCI_SIGMA_STORED) sigma_stored(c, PSOCI_FP32, nb, X32.p, nb, Y32.p, nb);
       else sigma_direct(c, PSOCI_FP32, nb, X32.p, nb, Y32.p, nb);

@@ -100,9 +100,10 @@ __device__ __forceinline__ T sef_coef(uint8_t f) {
 }
 
 // c[2 det][KC] = coefficient * x[sef][col0 + q] (zero for q >= kw), c[2 det
-// + 1] its negative; the zero rows 2 ndet, 2 ndet + 1
-template <typename T, int KC>
-__global__ __launch_bounds__(256) void k_ras_x2c(const RasView* __restrict__ vp, const T* __restrict__ X, int ldx,
+// + 1] its negative; the zero rows 2 ndet, 2 ndet + 1.  X may be wider than
+// T (FP64 vectors rounded to the FP32 computation on the fly).
+template <typename T, int KC, typename TX = T>
+__global__ __launch_bounds__(256) void k_ras_x2c(const RasView* __restrict__ vp, const TX* __restrict__ X, int ldx,
                                                  int col0, int kw, T* __restrict__ c) {
   const RasView& v = *vp;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(256) void k_ras_x2c(const RasView* __restrict__ vp,
   const int q = (int)(t % KC);
   if (s >= v.nsef) return;
   const RasSef e = v.sef[s];
-  const T x = q < kw ? X[s * (int64_t)ldx + col0 + q] : (T)0;
+  const T x = q < kw ? (T)X[s * (int64_t)ldx + col0 + q] : (T)0;
   for (int d = 0; d < e.n; ++d) {
     const RasSectorDev& S = v.sec[e.sec[d]];
     const int ca = ras_cls_of(v, S.na, e.a[d]), cb = ras_cls_of(v, S.nb, e.b[d]);
@@ -535,11 +536,12 @@ __global__ __launch_bounds__(256) void k_ras_terms(const RasView* __restrict__ v
   if (o != end) *bad = 1;
 }
 
-// y[sef][col0 + q] = scale * sum of +-W[off][q] over the SEF's terms
-template <typename T, int KC>
+// y[sef][col0 + q] = scale * sum of +-W[off][q] over the SEF's terms (Y may
+// be wider than T: the FP32 result widened exactly)
+template <typename T, int KC, typename TY = T>
 __global__ __launch_bounds__(256) void k_ras_y(const int64_t* __restrict__ toff, const int32_t* __restrict__ term,
                                                const int8_t* __restrict__ scl, const T* __restrict__ W, int64_t nsef,
-                                               T* __restrict__ Y, int ldy, int col0, int kw) {
+                                               TY* __restrict__ Y, int ldy, int col0, int kw) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t s = t / KC;
   const int q = (int)(t % KC);
@@ -565,7 +567,7 @@ __global__ __launch_bounds__(256) void k_ras_y(const int64_t* __restrict__ toff,
   }
   const int8_t sc = scl[s];
   const T f = sc == 2 ? (T)1 : (sc == 1 ? (T)0x1.6a09e667f3bcdp+0 : (T)0x1.6a09e667f3bcdp-1);
-  if (q < kw) Y[s * (int64_t)ldy + col0 + q] = f * (a0 + a1);
+  if (q < kw) Y[s * (int64_t)ldy + col0 + q] = (TY)(f * (a0 + a1));
 }
 
 // Flags (*flag = 1) a configuration pair one electron apart whose
@@ -644,8 +646,8 @@ inline int ras_tile_width(int nqp) {
   return 64;
 }
 
-template <typename T, int KC>
-void ras_chunk(Ctx& c, const T* X, int ldx, T* Y, int ldy, int col0, int kw, RasTiming* tm = nullptr) {
+template <typename T, int KC, typename TIO = T>
+void ras_chunk(Ctx& c, const TIO* X, int ldx, TIO* Y, int ldy, int col0, int kw, RasTiming* tm = nullptr) {
   const RasPlan& P = c.ras;
   const RasView* vp = c.d_rasview.p;
   T* cv = reinterpret_cast<T*>(c.d_rasc.p);
@@ -653,8 +655,8 @@ void ras_chunk(Ctx& c, const T* X, int ldx, T* Y, int ldy, int col0, int kw, Ras
   const T* G = sizeof(T) == 4 ? reinterpret_cast<const T*>(c.d_rasg32.p) : reinterpret_cast<const T*>(c.d_rasg64.p);
   const int64_t nsef = (int64_t)P.sef.size();
   if (tm && tm->ev.empty()) tm->mark(c.stream, -1);
-  hipLaunchKernelGGL((k_ras_x2c<T, KC>), dim3(nblk(nsef * KC, 256)), dim3(256), 0, c.stream, vp, X, ldx, col0, kw,
-                     cv);
+  hipLaunchKernelGGL((k_ras_x2c<T, KC, TIO>), dim3(nblk(nsef * KC, 256)), dim3(256), 0, c.stream, vp, X, ldx, col0,
+                     kw, cv);
   PSOCI_HIP(hipGetLastError());
   if (tm) tm->mark(c.stream, 0);
   for (int bi = 0; bi < (int)P.blk.size(); ++bi) {
@@ -706,23 +708,25 @@ void ras_chunk(Ctx& c, const T* X, int ldx, T* Y, int ldy, int col0, int kw, Ras
     PSOCI_HIP(hipGetLastError());
     if (tm) tm->mark(c.stream, 1 + B.kind);
   }
-  hipLaunchKernelGGL((k_ras_y<T, KC>), dim3(nblk(nsef * KC, 256)), dim3(256), 0, c.stream, c.d_rastoff.p,
+  hipLaunchKernelGGL((k_ras_y<T, KC, TIO>), dim3(nblk(nsef * KC, 256)), dim3(256), 0, c.stream, c.d_rastoff.p,
                      c.d_rasterm.p, c.d_rasscl.p, W, nsef, Y, ldy, col0, kw);
   PSOCI_HIP(hipGetLastError());
   if (tm) tm->mark(c.stream, 4);
 }
 
-template <typename T>
-void ras_any_k(Ctx& c, int k, const T* X, int ldx, T* Y, int ldy, RasTiming* tm = nullptr) {
+// T: the computation's type; TIO: the vectors' (FP64 vectors with an FP32
+// computation: rounded on the way in, widened on the way out)
+template <typename T, typename TIO = T>
+void ras_any_k(Ctx& c, int k, const TIO* X, int ldx, TIO* Y, int ldy, RasTiming* tm = nullptr) {
   const int kmax = sizeof(T) == 4 ? 16 : c.ras_kc64;
   for (int c0 = 0; c0 < k; c0 += kmax) {
     const int kw = std::min(kmax, k - c0);
     if (kw > 8)
-      ras_chunk<T, 16>(c, X, ldx, Y, ldy, c0, kw, tm);
-    else if (kw > 4) ras_chunk<T, 8>(c, X, ldx, Y, ldy, c0, kw, tm);
-    else if (kw > 2) ras_chunk<T, 4>(c, X, ldx, Y, ldy, c0, kw, tm);
-    else if (kw > 1) ras_chunk<T, 2>(c, X, ldx, Y, ldy, c0, kw, tm);
-    else ras_chunk<T, 1>(c, X, ldx, Y, ldy, c0, kw, tm);
+      ras_chunk<T, 16, TIO>(c, X, ldx, Y, ldy, c0, kw, tm);
+    else if (kw > 4) ras_chunk<T, 8, TIO>(c, X, ldx, Y, ldy, c0, kw, tm);
+    else if (kw > 2) ras_chunk<T, 4, TIO>(c, X, ldx, Y, ldy, c0, kw, tm);
+    else if (kw > 1) ras_chunk<T, 2, TIO>(c, X, ldx, Y, ldy, c0, kw, tm);
+    else ras_chunk<T, 1, TIO>(c, X, ldx, Y, ldy, c0, kw, tm);
   }
 }
 
@@ -891,6 +895,11 @@ void sigma_ras(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy
     ras_any_k<float>(c, k, static_cast<const float*>(X), ldx, static_cast<float*>(Y), ldy);
   else
     ras_any_k<double>(c, k, static_cast<const double*>(X), ldx, static_cast<double*>(Y), ldy);
+}
+
+void sigma_ras_f32_io64(Ctx& c, int k, const double* X, int ldx, double* Y, int ldy) {
+  PSOCI_CHECK(c.have_ras, PSOCI_ERR_STATE, "the string-driven direct sigma is not set up");
+  ras_any_k<float, double>(c, k, X, ldx, Y, ldy);
 }
 
 }  // namespace psoci
