@@ -146,6 +146,47 @@ def cpu_stored(args, cfg, ints, k, ref):
             "max_root_diff_vs_gpu_fp64": diff, "valid": valid}
 
 
+def hbuild_measure(api, torch, with_cpu, reps=3):
+    """The stored H build (psoci_build_h: pair screening, element blocks,
+    thresholded packing into CSR) of the 20-spin-orbital / 8-electron full CI
+    space (fci10, 31,650 SEFs): GPU time per build after one warm-up build,
+    elements (lower-triangle nonzeros incl. the diagonal) per second and
+    output bytes nnz_L * 12 + (n + 1) * 8 per second; beside it the oracle's
+    OpenMP build of the same matrix on the host."""
+    from psoci_amd import inputs
+    cfg, ints, _ = inputs.workload("fci10")
+    H = api.Hamiltonian(cfg, ints)
+    nnz = H.build()
+    torch.cuda.synchronize()
+    ts, tt = [], []
+    for _ in range(reps):
+        nnz = H.build()
+        bt = H.build_timing()
+        ts.append(bt["build_s"])
+        tt.append(bt["tune_s"])
+    t = float(np.median(ts))
+    n = H.n
+    out_bytes = nnz * 12 + (n + 1) * 8
+    rec = {"workload": "fci10 (20 spin-orbitals, 8 electrons, 31,650 SEFs)", "nnz_lower": int(nnz),
+           "gpu_ms": round(t * 1e3, 3), "elements_per_s": round(nnz / t, 1),
+           "output_bytes": int(out_bytes), "output_gbs": round(out_bytes / t / 1e9, 2),
+           "tune_ms": round(float(np.median(tt)) * 1e3, 3),
+           "note": ("median of 3 builds after a warm-up build (psoci_build_h_timing): every build screens the "
+                    "pairs, generates the blocks and packs the CSR anew; the stored sigma's shape autotuner that "
+                    "follows each build is tune_ms")}
+    if with_cpu:
+        import oracle
+        threads = min(os.cpu_count() or 1, 16)
+        sp = oracle.OracleSpace(cfg, ints)
+        t0 = time.time()
+        nnz_o = sp.build_h(threads)
+        tc = time.time() - t0
+        rec["cpu"] = {"seconds": round(tc, 3), "threads": threads, "kind": "port",
+                      "nnz_lower": int(nnz_o), "nnz_matches": int(nnz_o) == int(nnz),
+                      "gpu_speedup": round(tc / t, 1)}
+    return rec
+
+
 def cpu_direct(args, cfg, ints, k, sigma_calls, budget_s=15.0):
     """Sampled rows of the reference direct sigma (every block regenerated),
     timed until about budget_s seconds of wall time, scaled to the solve."""
@@ -327,6 +368,8 @@ def main():
     ach64 = bytes64 / (ms64 * 1e-3) / 1e9
     traffic = measured_traffic(args.workload, kr)
 
+    hbuild = hbuild_measure(api, torch, not args.no_cpu)
+
     cpu = None
     if not args.no_cpu:
         if direct:
@@ -388,6 +431,7 @@ def main():
                      "traffic": traffic["bytes_per_launch"] if traffic else None, "kernel": kernel,
                      "algorithmic_bytes_per_launch": int(bytes_r), "launch_ms": round(msr, 4)},
         "cpu_baseline": cpu,
+        "hbuild": hbuild,
         "wall_seconds_timed": round(wall, 3),
     }
     if traffic:

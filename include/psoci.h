@@ -161,6 +161,14 @@ int psoci_block(psoci_ctx* ctx, int64_t I, int64_t J, double* sefsef, int* statu
  * receives the reference's element count. */
 int psoci_build_h(psoci_ctx* ctx, int64_t* nnz_lower);
 
+/* Wall time of the last psoci_build_h on this context, in seconds:
+ * seconds[0] the build proper (pair screening, element blocks, packing into
+ * CSR, the diagonal and the FP32 copy, synchronised), seconds[1] the stored
+ * sigma's shape autotuner that follows it.  PSOCI_ERR_STATE before a build.
+ * No reference counterpart (a measurement entry point; the reference's
+ * build timers are PsociGAhamiltonian.C:2822-2839). */
+int psoci_build_h_timing(psoci_ctx* ctx, double* seconds);
+
 /* Copy the stored lower triangle to host arrays: rowptr[n+1], col[nnz],
  * val[nnz], diag[n] (any may be NULL).  Replaces the GA row fetch
  * fetchLocalHblock (PsociGAhamiltonian.C:11791-11846) and fetchDIAG_SEF. */

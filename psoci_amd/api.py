@@ -107,6 +107,7 @@ def load(path: str | None = None):
         "psoci_direct_engine": (I, [V, I, V]),
         "psoci_direct_ras_info": (I, [V, V, V, V, V, V, V, V]),
         "psoci_direct_ras_timing": (I, [V, I, I, I, V, V]),
+        "psoci_build_h_timing": (I, [V, V]),
         "psoci_direct_schedule": (I, [V, V, V, V]),
         "psoci_direct_schedule_prec": (I, [V, I, V, V, V]),
         "psoci_stored_shape": (I, [V, I, V, V, V]),
@@ -252,6 +253,13 @@ class Hamiltonian:
         self._check(self.L.psoci_build_h(self._h, ctypes.byref(nnz)))
         self.nnz_lower = nnz.value
         return nnz.value
+
+    def build_timing(self) -> dict:
+        """Seconds of the last build(): the build proper and the stored
+        sigma's shape autotuner after it."""
+        sec = np.zeros(2)
+        self._check(self.L.psoci_build_h_timing(self._h, _p(sec)))
+        return {"build_s": float(sec[0]), "tune_s": float(sec[1])}
 
     def transition(self, cfg2):
         """Transition Hamiltonian with this list as rows and cfg2's as

@@ -189,6 +189,16 @@ int psoci_build_h(psoci_ctx* ctx, int64_t* nnz_lower) {
   });
 }
 
+int psoci_build_h_timing(psoci_ctx* ctx, double* seconds) {
+  return guarded(ctx, [&](Ctx& c) {
+    PSOCI_CHECK(seconds, PSOCI_ERR_ARG, "seconds must not be NULL");
+    PSOCI_CHECK(c.have_h && c.hbuild_seconds[0] >= 0.0, PSOCI_ERR_STATE, "no stored Hamiltonian built here");
+    seconds[0] = c.hbuild_seconds[0];
+    seconds[1] = c.hbuild_seconds[1];
+    return PSOCI_OK;
+  });
+}
+
 int psoci_h_save(psoci_ctx* ctx, const char* path) {
   return guarded(ctx, [&](Ctx& c) {
     psoci::save_h(c, path);

@@ -62,6 +62,7 @@ struct Ctx {
   int64_t spmm_boff[kSpmmBuckets + 1] = {0};
   int spmm_bg[2][kSpmmBuckets] = {}, spmm_br[2][kSpmmBuckets] = {};
   bool spmm_bucketed[2] = {false, false};
+  double hbuild_seconds[2] = {-1.0, -1.0};  // last build_h: build proper, shape tuner
   double spmm_bucket_ms[2] = {-1, -1};
   DevBuf<double> d_diag;
   bool have_diag = false;

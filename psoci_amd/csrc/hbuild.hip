@@ -12,6 +12,7 @@
 // The context keeps the symmetric full matrix: row r holds its lower part
 // (the reference's row, diagonal last) followed by the transposed elements of
 // blocks (I, J) with I > J, ascending.
+#include <chrono>
 #include <hipcub/hipcub.hpp>
 
 #include "ctx.hpp"
@@ -398,6 +399,7 @@ void lower_pairs(Ctx& c, PairLists& L) {
 void build_h(Ctx& c) {
   const Space& sp = c.sp;
   const int64_t ncfg = sp.ncfg, n = sp.nsef_total;
+  const auto t0 = std::chrono::steady_clock::now();
   PairLists L;
   lower_pairs(c, L);
   screen(c, 1, L.nu, L.offU, L.rowU, L.nU);
@@ -466,8 +468,11 @@ void build_h(Ctx& c) {
   c.nnz_lower = (nnz + n) / 2;
   c.sync();
   c.have_h = true;
+  const auto t1 = std::chrono::steady_clock::now();
   tune_stored(c);
   c.have_diag = true;
+  c.hbuild_seconds[0] = std::chrono::duration<double>(t1 - t0).count();
+  c.hbuild_seconds[1] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
 }
 
 void compute_diag(Ctx& c) {
@@ -550,6 +555,7 @@ void install_h(Ctx& c, const std::vector<int64_t>& rowptr, const std::vector<int
   c.sync();
   c.have_h = true;
   c.have_diag = true;
+  c.hbuild_seconds[0] = c.hbuild_seconds[1] = -1.0;  // not built here
 }
 
 }  // namespace psoci
