@@ -114,6 +114,7 @@ struct Ctx {
   int ras_kc64 = 8;             // FP64 columns per chunk (16 when memory allows, else 8)
   DevBuf<int32_t> d_rasdidx;    // [dunit]: signed vector row (2 det + negative) of every D entry
   DevBuf<int64_t> d_rastoff;    // nsef + 1: first term of each SEF
+  int64_t ras_nterm = 0;        // terms over all SEFs (after merging mirrored pairs)
   DevBuf<int32_t> d_rasterm;    // signed W offsets (bit 31: negative)
   DevBuf<int8_t> d_rasscl;      // per SEF: 0 1/sqrt(2), 1 sqrt(2), 2 one
   void drop_ras() {

@@ -227,7 +227,10 @@ PSOCI_HD void ras_terms(const RasView& v, int s, int64_t a, int64_t b, F&& f) {
         // local row strings and pair of the evaluated product
         int64_t ul = mir ? w - B.u0 : u - B.u0, wl = mir ? u - B.w0 : w - B.w0;
         int pp = mir ? r : p, rr = mir ? p : r;
-        if (B.tri && ul > wl) {
+        // a triangular product holds rows u <= w; on a diagonal row (u = w)
+        // W[(u,u)][(p,r)] = W[(u,u)][(r,p)] (the flip), read as p <= r so
+        // that a determinant and its flip read the same element
+        if (B.tri && (ul > wl || (ul == wl && pp > rr))) {
           const int64_t t = ul;
           ul = wl;
           wl = t;
@@ -283,6 +286,70 @@ PSOCI_HD void ras_terms(const RasView& v, int s, int64_t a, int64_t b, F&& f) {
 
 // Host plan: the tables above in host memory, the per-block G matrices
 // (FP64), and the sizes the device buffers need.
+// The W terms of SEF e: f(offset, negative) per term; returns the SEF's
+// scale code (0 -> 1/sqrt(2), 1 -> sqrt(2), 2 -> 1).  The two determinants
+// of an open-shell SEF of a spin-flip symmetric sector (D and its flip)
+// read the same W elements -- every term of the flip is the mirror of one of
+// D's, stored once -- so their terms are merged: equal signs give one term
+// of twice the weight (scale 2/sqrt(2) = sqrt(2)), opposite signs cancel.
+// Otherwise (a term without its mirror, more than kRasMergeMax terms, other
+// sectors) the determinants' terms are listed one by one.  *bad = 1 for a
+// term with no W element (never, for a plan that passes its host checks).
+constexpr int kRasMergeMax = 64;
+template <typename F>
+PSOCI_HD int ras_sef_terms(const RasView& v, const RasSef& e, int* bad, F&& f) {
+  if (e.n == 2 && e.sec[0] == e.sec[1] && v.sec[e.sec[0]].sym) {
+    int32_t off[kRasMergeMax];
+    int8_t sg[kRasMergeMax];
+    int n0 = 0;
+    bool ok = true;
+    ras_terms(v, e.sec[0], e.a[0], e.b[0], [&](int bi, int64_t o, int neg) {
+      if (bi < 0 || o < 0 || o >= v.wunit || n0 >= kRasMergeMax) {
+        ok = false;
+        return;
+      }
+      off[n0] = (int32_t)o;
+      sg[n0++] = (int8_t)(neg ^ (e.flag[0] & 1));
+    });
+    if (ok) {
+      uint64_t used = 0;
+      int npair = 0;
+      ras_terms(v, e.sec[1], e.a[1], e.b[1], [&](int bi, int64_t o, int neg) {
+        if (!ok) return;
+        int k = -1;
+        for (int i = 0; i < n0; ++i)
+          if (!((used >> i) & 1) && off[i] == o) {
+            k = i;
+            break;
+          }
+        if (bi < 0 || k < 0) {
+          ok = false;
+          return;
+        }
+        used |= 1ull << k;
+        if (sg[k] != (int8_t)(neg ^ (e.flag[1] & 1))) sg[k] = 2;  // the pair cancels
+        ++npair;
+      });
+      if (ok && npair == n0) {
+        for (int i = 0; i < n0; ++i)
+          if (sg[i] != 2) f((int64_t)off[i], (int)sg[i]);
+        return 1;
+      }
+    }
+  }
+  for (int d = 0; d < e.n; ++d) {
+    const int nd = e.flag[d] & 1;
+    ras_terms(v, e.sec[d], e.a[d], e.b[d], [&](int bi, int64_t o, int neg) {
+      if (bi < 0 || o < 0 || o >= v.wunit) {
+        *bad = 1;
+        return;
+      }
+      f(o, neg ^ nd);
+    });
+  }
+  return (e.flag[0] & 2) ? 2 : (e.sec[0] > 0 ? 1 : 0);
+}
+
 struct RasPlan {
   bool ok = false;
   std::string why;  // when !ok: why the space does not qualify

@@ -506,33 +506,34 @@ void k_ras_gemm_lds(const RasBlockDev* __restrict__ blk, int bi,
   }
 }
 
-// The term list of every SEF (built once): for each of its determinants
-// every RI term of ras_terms, as the W offset per column with bit 31 set
-// for a negative sign (the determinant's coefficient sign folded in), and
-// the SEF's scale: 0 -> 1/sqrt(2) (an Ms = 0 pair, both determinants), 1 ->
-// sqrt(2) (an Ms > 0 determinant standing for its mirror too), 2 -> 1 (a
-// closed shell).  *bad = 1 when a term has no W element or a list does not
-// fill its slot exactly (never, for a plan that passes its host checks).
-__global__ __launch_bounds__(256) void k_ras_terms(const RasView* __restrict__ vp, const int64_t* __restrict__ toff,
-                                                   int32_t* __restrict__ term, int8_t* __restrict__ scl,
-                                                   int* __restrict__ bad) {
+// The term list of every SEF (built once): FILL = false counts each SEF's
+// terms into cnt[s]; FILL = true writes them, as the W offset per column
+// with bit 31 set for a negative sign (the determinants' coefficient signs
+// folded in), at toff[s] (the scanned counts), and the SEF's scale code.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_ras_terms(const RasView* __restrict__ vp, int64_t* __restrict__ cnt,
+                                                   const int64_t* __restrict__ toff, int32_t* __restrict__ term,
+                                                   int8_t* __restrict__ scl, int* __restrict__ bad) {
   const RasView& v = *vp;
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= v.nsef) return;
   const RasSef e = v.sef[s];
+  if (!FILL) {
+    int64_t n = 0;
+    ras_sef_terms(v, e, bad, [&](int64_t, int) { ++n; });
+    cnt[s] = n;
+    return;
+  }
   int64_t o = toff[s];
   const int64_t end = toff[s + 1];
-  scl[s] = (int8_t)((e.flag[0] & 2) ? 2 : (e.sec[0] > 0 ? 1 : 0));
-  for (int d = 0; d < e.n; ++d) {
-    const int nd = e.flag[d] & 1;
-    ras_terms(v, e.sec[d], e.a[d], e.b[d], [&](int bi, int64_t off, int neg) {
-      if (bi < 0 || off < 0 || off >= v.wunit || o >= end) {
-        *bad = 1;
-        return;
-      }
-      term[o++] = (int32_t)((uint32_t)off | ((uint32_t)(neg ^ nd) << 31));
-    });
-  }
+  const int code = ras_sef_terms(v, e, bad, [&](int64_t off, int neg) {
+    if (o >= end) {
+      *bad = 1;
+      return;
+    }
+    term[o++] = (int32_t)((uint32_t)off | ((uint32_t)neg << 31));
+  });
+  scl[s] = (int8_t)code;
   if (o != end) *bad = 1;
 }
 
@@ -816,32 +817,34 @@ bool build_ras(Ctx& c) {
     hipLaunchKernelGGL(k_ras_didx, dim3(nblk(n, 256)), dim3(256), 0, c.stream, c.d_rasview.p, bi, c.d_rasdidx.p);
     PSOCI_HIP(hipGetLastError());
   }
-  // the SEF term lists: each determinant of sector (na, nb) has na nb +
-  // na (na - 1) / 2 + nb (nb - 1) / 2 terms
+  // the SEF term lists (merged where an SEF's two determinants read the
+  // same W elements): count, scan, fill
   {
     const int64_t nsef = (int64_t)P.sef.size();
-    std::vector<int64_t> toff(nsef + 1, 0);
-    for (int64_t s = 0; s < nsef; ++s) {
-      int64_t cnt = 0;
-      for (int d = 0; d < P.sef[s].n; ++d) {
-        const RasSectorDev& S = P.sec[P.sef[s].sec[d]];
-        cnt += (int64_t)S.na * S.nb + (int64_t)S.na * (S.na - 1) / 2 + (int64_t)S.nb * (S.nb - 1) / 2;
-      }
-      toff[s + 1] = toff[s] + cnt;
-    }
-    c.d_rastoff.upload(toff, c.stream);
-    c.d_rasterm.alloc(toff[nsef]);
-    c.d_rasscl.alloc(nsef);
+    DevBuf<int64_t> cnt;
+    cnt.alloc(nsef + 1);
     DevBuf<int> bad;
     bad.alloc(1);
     PSOCI_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), c.stream));
-    hipLaunchKernelGGL(k_ras_terms, dim3(nblk(nsef, 256)), dim3(256), 0, c.stream, c.d_rasview.p, c.d_rastoff.p,
-                       c.d_rasterm.p, c.d_rasscl.p, bad.p);
+    PSOCI_HIP(hipMemsetAsync(cnt.p + nsef, 0, sizeof(int64_t), c.stream));
+    hipLaunchKernelGGL(k_ras_terms<false>, dim3(nblk(nsef, 256)), dim3(256), 0, c.stream, c.d_rasview.p, cnt.p,
+                       nullptr, nullptr, nullptr, bad.p);
+    PSOCI_HIP(hipGetLastError());
+    c.d_rastoff.alloc(nsef + 1);
+    c.scan(cnt.p, c.d_rastoff.p, nsef + 1);
+    int64_t nterm = 0;
+    PSOCI_HIP(hipMemcpyAsync(&nterm, c.d_rastoff.p + nsef, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    c.d_rasterm.alloc(nterm);
+    c.d_rasscl.alloc(nsef);
+    hipLaunchKernelGGL(k_ras_terms<true>, dim3(nblk(nsef, 256)), dim3(256), 0, c.stream, c.d_rasview.p, nullptr,
+                       c.d_rastoff.p, c.d_rasterm.p, c.d_rasscl.p, bad.p);
     PSOCI_HIP(hipGetLastError());
     int b = 0;
     bad.download(&b, 1, c.stream);
     c.sync();
     PSOCI_CHECK(!b, PSOCI_ERR_STATE, "string-driven sigma: inconsistent term lists");
+    c.ras_nterm = nterm;
   }
   c.d_rasw.alloc((size_t)P.wunit * (c.ras_kc64 == 16 ? 16 * sizeof(double) : 16 * sizeof(float)));
   c.sync();

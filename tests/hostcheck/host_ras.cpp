@@ -39,9 +39,29 @@ int hr_plan(int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, cons
   return 1;
 }
 
+static int hr_sigma_impl(int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, const int32_t* jsym,
+                         const double* one, const double* two, int k, const double* X, double* Y, int merged,
+                         int64_t* nterm);
+
 // Y[n][k] = H X[n][k] by the plan, on the host; returns 0 on success.
 int hr_sigma(int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, const int32_t* jsym, const double* one,
              const double* two, int k, const double* X, double* Y) {
+  return hr_sigma_impl(nbf, nelec, ksym, ncfg, occ, jsym, one, two, k, X, Y, 0, nullptr);
+}
+
+// The same with the SEF term lists the GPU gather uses (ras_sef_terms: the
+// mirrored terms of an SEF's two determinants merged); nterm[0] receives the
+// number of terms over all SEFs, nterm[1] the determinants' terms unmerged.
+int hr_sigma_merged(int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, const int32_t* jsym,
+                    const double* one, const double* two, int k, const double* X, double* Y, int64_t* nterm) {
+  return hr_sigma_impl(nbf, nelec, ksym, ncfg, occ, jsym, one, two, k, X, Y, 1, nterm);
+}
+
+}  // extern "C"
+
+static int hr_sigma_impl(int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, const int32_t* jsym,
+                         const double* one, const double* two, int k, const double* X, double* Y, int merged,
+                         int64_t* nterm) {
   Space sp;
   RasPlan P;
   try {
@@ -90,6 +110,27 @@ int hr_sigma(int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, con
         }
     }
   }
+  if (merged) {
+    int64_t nt = 0, nt1 = 0;
+    const double sc[3] = {r, 1.0 / r, 1.0};
+    for (int64_t s = 0; s < v.nsef; ++s) {
+      const RasSef& e = v.sef[s];
+      std::vector<double> sig(k, 0.0);
+      int bad = 0;
+      const int code = ras_sef_terms(v, e, &bad, [&](int64_t off, int neg) {
+        ++nt;
+        for (int q = 0; q < k; ++q) sig[q] += neg ? -W[off * k + q] : W[off * k + q];
+      });
+      if (bad) return 5;
+      for (int q = 0; q < k; ++q) Y[s * k + q] = sc[code] * sig[q];
+      for (int t = 0; t < e.n; ++t) ras_terms(v, e.sec[t], e.a[t], e.b[t], [&](int, int64_t, int) { ++nt1; });
+    }
+    if (nterm) {
+      nterm[0] = nt;
+      nterm[1] = nt1;
+    }
+    return 0;
+  }
   for (int64_t s = 0; s < v.nsef; ++s) {
     const RasSef& e = v.sef[s];
     for (int q = 0; q < k; ++q) Y[s * k + q] = 0.0;
@@ -110,8 +151,6 @@ int hr_sigma(int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, con
   }
   return 0;
 }
-
-}  // extern "C"
 
 extern "C" {
 // Per-block summary of the plan: rows, nq, nqp, kind, sector, classes (9

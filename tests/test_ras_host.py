@@ -32,6 +32,8 @@ def hr():
     P = ctypes.c_void_p
     L.hr_plan.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, P, P, P, P, P, ctypes.c_char_p]
     L.hr_sigma.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, P, P, P, P, ctypes.c_int, P, P]
+    L.hr_sigma_merged.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, P, P, P, P, ctypes.c_int,
+                                  P, P, P]
     return L
 
 
@@ -92,6 +94,32 @@ def test_host_ras_sigma_matches_oracle(hr, name, make, m, next_):
     ref = sp.sigma(X)
     Y = sigma(hr, cfg, ints, X)
     np.testing.assert_allclose(Y, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("name,make,m,next_", CASES, ids=[c[0] for c in CASES])
+def test_host_ras_merged_terms_match_oracle(hr, name, make, m, next_):
+    """The gather's term lists (ras_sef_terms): in spin-flip symmetric sectors
+    the mirrored terms of an SEF's two determinants are one W element each,
+    merged into one term (or cancelled); the sigma is unchanged and, for
+    mult = +1, the list is shorter than the determinants' terms."""
+    cfg, ints = make()
+    sp = oracle.OracleSpace(cfg, ints)
+    sp.build_h()
+    X = np.random.default_rng(4).standard_normal((sp.n, 2))
+    ref = sp.sigma(X)
+    occ, js, one, two = _args(cfg, ints)
+    Y = np.zeros_like(X)
+    nt = np.zeros(2, np.int64)
+    rc = hr.hr_sigma_merged(cfg.nbf, cfg.nelec, cfg.ksym, cfg.ncfg, _p(occ), _p(js), _p(one), _p(two), X.shape[1],
+                            _p(X), _p(Y), _p(nt))
+    assert rc == 0
+    np.testing.assert_allclose(Y, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    merged, full = int(nt[0]), int(nt[1])
+    assert 0 < merged <= full
+    if "mult -1" in name:
+        assert merged == full  # no spin-flip symmetric sector: nothing merges
+    elif (cfg.nopen() > 0).any():
+        assert merged < full
 
 
 def test_host_ras_refusals(hr):
