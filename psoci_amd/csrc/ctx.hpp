@@ -106,6 +106,7 @@ struct Ctx {
   DevBuf<int32_t> d_rasrem;
   DevBuf<RasBlockDev> d_rasblk;
   DevBuf<int16_t> d_rasq, d_raspidx;
+  DevBuf<uint64_t> d_rasqm;  // per block and chunk: bits of the first-added orbitals
   DevBuf<RasSef> d_rassef;
   DevBuf<double> d_rasg64;
   DevBuf<float> d_rasg32;
@@ -126,6 +127,7 @@ struct Ctx {
     d_rasrem.release();
     d_rasblk.release();
     d_rasq.release();
+    d_rasqm.release();
     d_raspidx.release();
     d_rassef.release();
     d_rasg64.release();

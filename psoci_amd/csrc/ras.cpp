@@ -338,6 +338,13 @@ void ras_plan(RasPlan& P, const Space& sp, const double* one, const double* two)
           wunit += B.nrows * B.nqp;
           B.dbase = dunit;
           dunit += (B.nrows + 15) / 16 * 16 * B.nqp;
+          B.qmoff = (int64_t)P.qm.size();
+          B.umoff = P.lev[lu].mask_off;
+          for (int c0 = 0; c0 < B.nqp; c0 += 16) {
+            uint64_t bits = 0;
+            for (int j = c0; j < c0 + 16 && j < B.nq; ++j) bits |= 1ull << (P.qlist[B.qoff + j] & 0xff);
+            P.qm.push_back(bits);
+          }
           flops += 2.0 * (double)B.nrows * B.nq * B.nq;
           if (gbase_by_budget[bud] < 0) {
             // the integral matrix of this (sector, kind, budget)

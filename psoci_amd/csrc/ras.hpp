@@ -94,6 +94,11 @@ struct RasBlockDev {
   int64_t goff;   // first element of G (nqp x nqp, row-major [p-pair][q-pair])
   int64_t wbase;  // W offset per column: W element (wbase + row * nqp + pidx) * KC + v
   int64_t dbase;  // first entry in the D index table (rows padded to a multiple of 16)
+  // per chunk of 16 pairs: the bits of the orbitals q the chunk adds first
+  // (qm[qmoff + chunk]); a chunk all of whose q the row string u already
+  // holds has only zero D entries (kinds 0 and 1, where u is the row's
+  // first string: mask[umoff + u])
+  int64_t qmoff, umoff;
 };
 
 // The determinants of one SEF (one or two): alpha / beta global string
@@ -365,6 +370,7 @@ struct RasPlan {
   std::vector<RasSectorDev> sec;
   std::vector<RasBlockDev> blk;
   std::vector<int16_t> qlist, pidx;
+  std::vector<uint64_t> qm;  // per block and chunk of 16 pairs (RasBlockDev::qmoff)
   std::vector<double> G;
   std::vector<int32_t> blockid;  // flattened [sector][kind][cu][cw]
   std::vector<RasSef> sef;

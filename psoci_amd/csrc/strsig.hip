@@ -362,7 +362,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
 void k_ras_gemm_lds(const RasBlockDev* __restrict__ blk, int bi,
                                                       const int32_t* __restrict__ didx, const T* __restrict__ G,
                                                       const T* __restrict__ c, T* __restrict__ W, int64_t ntiles,
-                                                      int npt) {
+                                                      int npt, const uint64_t* __restrict__ qm,
+                                                      const uint64_t* __restrict__ smask) {
   typedef Mfma<T> M;
   typedef typename M::v4 v4;
   static_assert(KC == 8 || KC == 16, "8 or 16 columns");
@@ -441,6 +442,34 @@ void k_ras_gemm_lds(const RasBlockDev* __restrict__ blk, int bi,
   // tile's last p-subtiles that hold only padding are skipped
   const int nq = B.nq;
   const int niv = ((nq - pt * PT < PT ? nq - pt * PT : PT) + 15) / 16;
+  // chunks with only zero D entries are skipped: when every row of the tile
+  // has the same first string u (alpha-beta and alpha-alpha products, rows
+  // u-major), a chunk whose pairs all add an orbital u holds (bit of qm not
+  // in u's mask) contributes nothing.  live[0..1]: bits of the first 128
+  // chunks (later ones are always live); uniform over the workgroup.
+  uint64_t live[2] = {~0ull, ~0ull};
+  if (B.kind != 2) {
+    const int64_t r0 = mt * RT, r1 = (r0 + RT < nrows ? r0 + RT : nrows) - 1;
+    int64_t ua, ub, wdum;
+    if (B.tri) {
+      ras_tri_uw(r0, B.nu, ua, wdum);
+      ras_tri_uw(r1, B.nu, ub, wdum);
+    } else {
+      ua = r0 / B.nw;
+      ub = r1 / B.nw;
+    }
+    if (ua == ub) {
+      const uint64_t um = smask[B.umoff + B.u0 + ua];
+      for (int ch = 0; ch < nch && ch < 128; ++ch)
+        if ((qm[B.qmoff + ch] & ~um) == 0) live[ch >> 6] &= ~(1ull << (ch & 63));
+    }
+  }
+  // first live chunk after chunk x (nch when none)
+  auto next_live = [&](int x) {
+    for (int ch = x + 1; ch < nch; ++ch)
+      if (ch >= 128 || ((live[ch >> 6] >> (ch & 63)) & 1)) return ch;
+    return nch;
+  };
 
   // the whole tile with NV live p-subtiles
   auto run = [&](auto nvc) {
@@ -464,23 +493,33 @@ void k_ras_gemm_lds(const RasBlockDev* __restrict__ blk, int bi,
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = M::op(a[i], b[j], acc[i][j]);
     };
+    // the live chunks in order: c0 (in the stage being read), c1 (copies in
+    // flight), c2 (indices in flight)
     int32_t ixn[DI];
-    fetch_idx(0, ixn);
-    issue(0, 0, ixn);
-    if (nch > 1) fetch_idx(1, ixn);
-    for (int ch = 0; ch < nch; ++ch) {
-      const int st = ch & 1;
-      // this wavefront's copies of chunk ch have landed, every wavefront's
-      // have after the barrier, and stage st ^ 1 is free (its reads of
-      // chunk ch - 1 ended before the barrier)
+    int c0 = next_live(-1);
+    int c1 = c0 < nch ? next_live(c0) : nch;
+    int c2 = c1 < nch ? next_live(c1) : nch;
+    if (c0 < nch) {
+      fetch_idx(c0, ixn);
+      issue(c0, 0, ixn);
+      if (c1 < nch) fetch_idx(c1, ixn);
+    }
+    for (int it = 0; c0 < nch; ++it) {
+      const int st = it & 1;
+      // this wavefront's copies of chunk c0 have landed, every wavefront's
+      // have after the barrier, and stage st ^ 1 is free (its reads of the
+      // chunk before ended before the barrier)
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       __syncthreads();
-      if (ch + 1 < nch) {
-        issue(ch + 1, st ^ 1, ixn);
-        if (ch + 2 < nch) fetch_idx(ch + 2, ixn);
+      if (c1 < nch) {
+        issue(c1, st ^ 1, ixn);
+        if (c2 < nch) fetch_idx(c2, ixn);
       }
 #pragma unroll
       for (int kk = 0; kk < 4; ++kk) step(st, kk);
+      c0 = c1;
+      c1 = c2;
+      c2 = c2 < nch ? next_live(c2) : nch;
     }
     const int64_t wb = B.wbase;
 #pragma unroll
@@ -670,12 +709,14 @@ void ras_chunk(Ctx& c, const TIO* X, int ldx, TIO* Y, int ldy, int col0, int kw,
     const RasBlockDev* blk = c.d_rasblk.p;
     const int32_t* didx = c.d_rasdidx.p;
     const int32_t zrow = (int32_t)(2 * P.ndet);  // the zero row of c
+    const uint64_t* qm = c.d_rasqm.p;
+    const uint64_t* smask = c.d_rasmask.p;
     if constexpr ((KC == 16 || KC == 8) && PSOCI_RAS_DMA) {
       switch (pt_w) {
 #define PSOCI_RAS_GEMML(PTV)                                                                                \
   case PTV:                                                                                                \
     hipLaunchKernelGGL((k_ras_gemm_lds<T, KC, PTV>), dim3(grid), dim3(256), 0, c.stream, blk, bi, didx, G, cv, \
-                       W, ntiles, npt);                                                                    \
+                       W, ntiles, npt, qm, smask);                                                         \
     break;
         PSOCI_RAS_GEMML(16)
         PSOCI_RAS_GEMML(32)
@@ -686,7 +727,7 @@ void ras_chunk(Ctx& c, const TIO* X, int ldx, TIO* Y, int ldy, int col0, int kw,
 #undef PSOCI_RAS_GEMML
         default:
           hipLaunchKernelGGL((k_ras_gemm_lds<T, KC, 64>), dim3(grid), dim3(256), 0, c.stream, blk, bi, didx, G, cv,
-                             W, ntiles, npt);
+                             W, ntiles, npt, qm, smask);
       }
     } else
     switch (pt_w) {
@@ -793,6 +834,7 @@ bool build_ras(Ctx& c) {
   c.d_rasrem.upload(P.rem, c.stream);
   c.d_rasblk.upload(P.blk, c.stream);
   c.d_rasq.upload(P.qlist, c.stream);
+  c.d_rasqm.upload(P.qm, c.stream);
   c.d_raspidx.upload(P.pidx, c.stream);
   c.d_rassef.upload(P.sef, c.stream);
   c.d_rasg64.upload(P.G, c.stream);

@@ -3,6 +3,7 @@
 // evaluated with plain loops on the host CPU, so the tables, signs and SEF
 // map can be compared with the oracle on machines without a GPU.  Not part
 // of the product library.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -50,8 +51,10 @@ int hr_sigma(int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, con
 }
 
 // The same with the SEF term lists the GPU gather uses (ras_sef_terms: the
-// mirrored terms of an SEF's two determinants merged); nterm[0] receives the
-// number of terms over all SEFs, nterm[1] the determinants' terms unmerged.
+// mirrored terms of an SEF's two determinants merged) and the GPU product's
+// chunk skipping; nterm[0] receives the number of terms over all SEFs,
+// nterm[1] the determinants' terms unmerged, nterm[2] the (row, chunk)
+// pairs the skipping left out.
 int hr_sigma_merged(int nbf, int nelec, int ksym, int64_t ncfg, const uint8_t* occ, const int32_t* jsym,
                     const double* one, const double* two, int k, const double* X, double* Y, int64_t* nterm) {
   return hr_sigma_impl(nbf, nelec, ksym, ncfg, occ, jsym, one, two, k, X, Y, 1, nterm);
@@ -91,12 +94,39 @@ static int hr_sigma_impl(int nbf, int nelec, int ksym, int64_t ncfg, const uint8
     if (!seen[d]) return 3;
   std::vector<double> W((size_t)P.wunit * k, 0.0);
   std::vector<double> D;
+  int64_t skipped = 0;  // (row, chunk) pairs left out by the chunk skipping
   for (int bi = 0; bi < v.nblk; ++bi) {
     const RasBlockDev& B = v.blk[bi];
     const double* G = P.G.data() + B.goff;
     D.assign((size_t)B.nqp * k, 0.0);
     for (int64_t row = 0; row < B.nrows; ++row) {
+      // with merged = 1, the GPU product's chunk skipping: in a tile of 8
+      // rows (16 columns) whose rows share their first string u, a chunk of
+      // 16 pairs whose first-added orbitals u all holds is left out
+      uint64_t live[2] = {~0ull, ~0ull};
+      if (merged && B.kind != 2) {
+        const int64_t RT = 8, r0 = row / RT * RT, r1 = std::min(r0 + RT, B.nrows) - 1;
+        int64_t ua, ub, wd;
+        if (B.tri) {
+          ras_tri_uw(r0, B.nu, ua, wd);
+          ras_tri_uw(r1, B.nu, ub, wd);
+        } else {
+          ua = r0 / B.nw;
+          ub = r1 / B.nw;
+        }
+        if (ua == ub) {
+          const uint64_t um = P.mask[B.umoff + B.u0 + ua];
+          for (int ch = 0; ch < B.nqp / 16 && ch < 128; ++ch)
+            if ((P.qm[B.qmoff + ch] & ~um) == 0) live[ch >> 6] &= ~(1ull << (ch & 63));
+        }
+      }
       for (int j = 0; j < B.nqp; ++j) {
+        const int ch = j / 16;
+        if (ch < 128 && !((live[ch >> 6] >> (ch & 63)) & 1)) {
+          for (int q = 0; q < k; ++q) D[(size_t)j * k + q] = 0.0;
+          if (j % 16 == 0) ++skipped;
+          continue;
+        }
         int neg = 0;
         const int64_t d = ras_d_entry(v, B, row, j, neg);
         if (d >= P.ndet) return 4;
@@ -128,6 +158,7 @@ static int hr_sigma_impl(int nbf, int nelec, int ksym, int64_t ncfg, const uint8
     if (nterm) {
       nterm[0] = nt;
       nterm[1] = nt1;
+      nterm[2] = skipped;
     }
     return 0;
   }

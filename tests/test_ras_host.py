@@ -100,8 +100,10 @@ def test_host_ras_sigma_matches_oracle(hr, name, make, m, next_):
 def test_host_ras_merged_terms_match_oracle(hr, name, make, m, next_):
     """The gather's term lists (ras_sef_terms): in spin-flip symmetric sectors
     the mirrored terms of an SEF's two determinants are one W element each,
-    merged into one term (or cancelled); the sigma is unchanged and, for
-    mult = +1, the list is shorter than the determinants' terms."""
+    merged into one term (or cancelled); with the product's skipping of
+    chunks whose pairs all add an orbital the tile's row string holds, the
+    sigma is unchanged and, for mult = +1, the list is shorter than the
+    determinants' terms."""
     cfg, ints = make()
     sp = oracle.OracleSpace(cfg, ints)
     sp.build_h()
@@ -109,13 +111,17 @@ def test_host_ras_merged_terms_match_oracle(hr, name, make, m, next_):
     ref = sp.sigma(X)
     occ, js, one, two = _args(cfg, ints)
     Y = np.zeros_like(X)
-    nt = np.zeros(2, np.int64)
+    nt = np.zeros(3, np.int64)
     rc = hr.hr_sigma_merged(cfg.nbf, cfg.nelec, cfg.ksym, cfg.ncfg, _p(occ), _p(js), _p(one), _p(two), X.shape[1],
                             _p(X), _p(Y), _p(nt))
     assert rc == 0
     np.testing.assert_allclose(Y, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
-    merged, full = int(nt[0]), int(nt[1])
+    merged, full, skipped = int(nt[0]), int(nt[1]), int(nt[2])
     assert 0 < merged <= full
+    # the product's chunk skipping (same result) is exercised where pair
+    # runs are long enough (8- and 9-orbital spaces)
+    if cfg.nbf >= 8 and not name.startswith("mrcisd 4+5"):
+        assert skipped > 0
     if "mult -1" in name:
         assert merged == full  # no spin-flip symmetric sector: nothing merges
     elif (cfg.nopen() > 0).any():
