@@ -233,10 +233,15 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
   auto collapse = [&](int nc) {
     std::vector<double> Yc;
     cols_of(m, 0, nc, Yc);
-    L.gemm_nn(n, m, nc, V.p, mmax, Yc.data(), 0.0, T.p, nc);
-    L.copy_cols(n, nc, T.p, nc, V.p, mmax);
-    L.gemm_nn(n, m, nc, HV.p, mmax, Yc.data(), 0.0, T.p, nc);
-    L.copy_cols(n, nc, T.p, nc, HV.p, mmax);
+    if (nc <= 128) {
+      L.gemm_nn_inplace(n, m, nc, Yc.data(), V.p, mmax);
+      L.gemm_nn_inplace(n, m, nc, Yc.data(), HV.p, mmax);
+    } else {
+      L.gemm_nn(n, m, nc, V.p, mmax, Yc.data(), 0.0, T.p, nc);
+      L.copy_cols(n, nc, T.p, nc, V.p, mmax);
+      L.gemm_nn(n, m, nc, HV.p, mmax, Yc.data(), 0.0, T.p, nc);
+      L.copy_cols(n, nc, T.p, nc, HV.p, mmax);
+    }
     std::fill(A.begin(), A.end(), 0.0);
     for (int i = 0; i < nc; ++i) A[(size_t)i * mmax + i] = w[i];
     m = nc;
@@ -259,8 +264,12 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     std::vector<double> M((size_t)nb * nk);
     for (int i = 0; i < nb; ++i)
       for (int j = 0; j < nk; ++j) M[(size_t)i * nk + j] = G[(size_t)i * nb + keep[j]] / std::sqrt(lam[keep[j]]);
-    L.copy_cols(n, nb, V.p + c0, mmax, T.p, nb);
-    L.gemm_nn(n, nb, nk, T.p, nb, M.data(), 0.0, V.p + c0, mmax);
+    if (nk <= 128) {
+      L.gemm_nn_inplace(n, nb, nk, M.data(), V.p + c0, mmax);
+    } else {
+      L.copy_cols(n, nb, V.p + c0, mmax, T.p, nb);
+      L.gemm_nn(n, nb, nk, T.p, nb, M.data(), 0.0, V.p + c0, mmax);
+    }
     return nk;
   };
   auto project_out = [&](int c0, int nb) {

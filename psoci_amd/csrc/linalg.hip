@@ -93,7 +93,10 @@ __global__ __launch_bounds__(256) void k_reduce_chunks(int64_t nchunks, int mb, 
 // instruction carries t = t0 + 4q + u for both operands.  Accumulator
 // register j of lane l holds row r0 + 4j + q, column c0 + i.  Each A and B
 // row is read from HBM once; Y (m x k, tiny) is read through the caches.
-template <int NCT, bool TWO, bool XOUT>
+// INPLACE: the rows are read from Out itself (A unused) and overwritten by
+// the product (k <= 128: one wavefront reads every input value of its rows
+// before it writes any output of them).
+template <int NCT, bool TWO, bool XOUT, bool INPLACE = false>
 __global__ __launch_bounds__(256) void k_gemm_nn_mfma(int64_t n, int m, int k, const double* __restrict__ A,
                                                       int lda, const double* __restrict__ B, int ldb,
                                                       const double* __restrict__ Y1, const double* __restrict__ Y2,
@@ -107,7 +110,7 @@ __global__ __launch_bounds__(256) void k_gemm_nn_mfma(int64_t n, int m, int k, c
   const int i = lane & 15, q = lane >> 4;
   const int64_t row = r0 + i;
   const bool rv = row < n;
-  const double* __restrict__ a_row = A + (rv ? row : 0) * (int64_t)lda;
+  const double* a_row = INPLACE ? Out + (rv ? row : 0) * (int64_t)ldo : A + (rv ? row : 0) * (int64_t)lda;
   const double* __restrict__ b_row = TWO ? B + (rv ? row : 0) * (int64_t)ldb : nullptr;
   double4_t acc[NCT], acc2[NCT];
 #pragma unroll
@@ -314,14 +317,14 @@ void Linalg::gemm_tn_host(int64_t n, int m, int b, const double* A, int lda, con
 
 namespace {
 
-template <bool TWO, bool XOUT>
+template <bool TWO, bool XOUT, bool INPLACE = false>
 void launch_nn(hipStream_t st, int64_t n, int m, int k, const double* A, int lda, const double* B, int ldb,
                const double* Y1, const double* Y2, double alpha, double* Out, int ldo, double* Out2, int ldo2) {
   const dim3 grid(nblk(n, 64), (unsigned)((k + 127) / 128));
   const int nct = std::min(8, (k + 15) / 16);
-#define PSOCI_NN(NCT)                                                                                             \
-  hipLaunchKernelGGL((k_gemm_nn_mfma<NCT, TWO, XOUT>), grid, dim3(256), 0, st, n, m, k, A, lda, B, ldb, Y1, Y2, \
-                     alpha, Out, ldo, Out2, ldo2)
+#define PSOCI_NN(NCT)                                                                                      \
+  hipLaunchKernelGGL((k_gemm_nn_mfma<NCT, TWO, XOUT, INPLACE>), grid, dim3(256), 0, st, n, m, k, A, lda, B, \
+                     ldb, Y1, Y2, alpha, Out, ldo, Out2, ldo2)
   if (nct <= 1) PSOCI_NN(1);
   else if (nct <= 2) PSOCI_NN(2);
   else if (nct <= 4) PSOCI_NN(4);
@@ -338,6 +341,15 @@ void Linalg::gemm_nn(int64_t n, int m, int k, const double* A, int lda, const do
   ysmall.reserve((size_t)m * k);
   PSOCI_HIP(hipMemcpyAsync(ysmall.p, Y1_host, (size_t)m * k * sizeof(double), hipMemcpyHostToDevice, stream));
   launch_nn<false, false>(stream, n, m, k, A, lda, nullptr, 0, ysmall.p, nullptr, alpha, Out, ldo, nullptr, 0);
+}
+
+void Linalg::gemm_nn_inplace(int64_t n, int m, int k, const double* Y1_host, double* Out, int ldo) {
+  if (k == 0 || n == 0) return;
+  PSOCI_CHECK(k <= 128 && k <= m, PSOCI_ERR_ARG, "in-place product: at most 128 and at most m output columns");
+  ysmall.reserve((size_t)m * k);
+  PSOCI_HIP(hipMemcpyAsync(ysmall.p, Y1_host, (size_t)m * k * sizeof(double), hipMemcpyHostToDevice, stream));
+  launch_nn<false, false, true>(stream, n, m, k, nullptr, 0, nullptr, 0, ysmall.p, nullptr, 0.0, Out, ldo, nullptr,
+                                0);
 }
 
 void Linalg::gemm_nn2(int64_t n, int m, int k, const double* A, int lda, const double* Y1_host, const double* B, int ldb,

@@ -22,6 +22,8 @@ struct Linalg {
                int ldo);
   // Out[:, 0:k] = A Y1 - B Y2 (Y1, Y2 host, m x k); with X set, also X = B Y1
   // in the same pass over A and B (synchronises)
+  // Out[:, 0:k) = Out[:, 0:m) Y (m x k, host), in place (k <= m, k <= 128)
+  void gemm_nn_inplace(int64_t n, int m, int k, const double* Y1_host, double* Out, int ldo);
   void gemm_nn2(int64_t n, int m, int k, const double* A, int lda, const double* Y1_host, const double* B, int ldb,
                 const double* Y2_host, double* Out, int ldo, double* X = nullptr, int ldx = 0);
   // R[:, c] /= (D - theta_c) where |D - theta_c| > dmin
