@@ -63,6 +63,10 @@ namespace {
 #ifndef PSOCI_RAS_DMA
 #define PSOCI_RAS_DMA 1
 #endif
+// FP32 LDS-copy products in chunks of 32 pairs (0: 16)
+#ifndef PSOCI_RAS_LQC32
+#define PSOCI_RAS_LQC32 0
+#endif
 constexpr int kMT = 128;            // (row, column) values per GEMM tile
 // q-pairs per LDS chunk (FP32, FP64)
 template <typename T>
@@ -357,17 +361,16 @@ __device__ __forceinline__ void lds_copy16(const void* src, void* dst) {
                : "memory", "m0");
 }
 
-template <typename T, int KC, int PT, int WPE = ras_lds_waves<T, PT>()>
+template <typename T, int KC, int PT, int QC = 16, int WPE = ras_lds_waves<T, PT>()>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
-void k_ras_gemm_lds(const RasBlockDev* __restrict__ blk, int bi,
-                                                      const int32_t* __restrict__ didx, const T* __restrict__ G,
-                                                      const T* __restrict__ c, T* __restrict__ W, int64_t ntiles,
-                                                      int npt, const uint64_t* __restrict__ qm,
-                                                      const uint64_t* __restrict__ smask) {
+void k_ras_gemm_lds(const RasBlockDev* __restrict__ blk, int bi, const int32_t* __restrict__ didx,
+                    const T* __restrict__ G, const T* __restrict__ c, int32_t zrow, T* __restrict__ W, int64_t ntiles,
+                    int npt, const uint64_t* __restrict__ qm, const uint64_t* __restrict__ smask) {
   typedef Mfma<T> M;
   typedef typename M::v4 v4;
   static_assert(KC == 8 || KC == 16, "8 or 16 columns");
-  constexpr int QC = 16, RT = kMT / KC;  // 8 or 16 block rows per tile
+  static_assert(QC == 16 || QC == 32, "chunks of 16 or 32 pairs");
+  constexpr int RT = kMT / KC;           // 8 or 16 block rows per tile
   constexpr int RW = RT / 4;             // rows per wavefront
   constexpr int NI = PT / 16;            // p-subtiles per wavefront
   constexpr int SL = 16 / sizeof(T);     // values per 16-byte piece
@@ -389,7 +392,9 @@ void k_ras_gemm_lds(const RasBlockDev* __restrict__ blk, int bi,
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const RasBlockDev& B = blk[bi];
   const int64_t nrows = B.nrows;
-  const int nqp = B.nqp, nch = nqp / QC;
+  // chunks of QC pairs; with QC = 32 the last one may hold 16 (nqp is a
+  // multiple of 16): its other half reads the zero row and G row nqp - 1
+  const int nqp = B.nqp, nch = (nqp + QC - 1) / QC;
 
   // this lane's sources as offsets advanced by one chunk per step: its D
   // entries in the index table (rows padded to whole tiles: no bounds
@@ -414,7 +419,10 @@ void k_ras_gemm_lds(const RasBlockDev* __restrict__ blk, int bi,
   const int v0 = SL * (lane % SPE);
   auto fetch_idx = [&](int ch, int32_t (&ix)[DI]) {
 #pragma unroll
-    for (int d = 0; d < DI; ++d) ix[d] = dx[dxo[d] + ch * QC];
+    for (int d = 0; d < DI; ++d) {
+      if (QC == 16 || ch * QC + (d * EPI + lane / SPE) % QC < nqp) ix[d] = dx[dxo[d] + ch * QC];
+      else ix[d] = zrow;
+    }
   };
   // the copies of chunk ch into stage st: D pieces first (they use the
   // indices loaded a chunk earlier, which the compiler waits for), then G
@@ -422,10 +430,14 @@ void k_ras_gemm_lds(const RasBlockDev* __restrict__ blk, int bi,
 #pragma unroll
     for (int d = 0; d < DI; ++d)
       lds_copy16(c + (int64_t)ix[d] * KC + v0, &Ds[st][(RW * wave * QC + d * EPI) * KC]);
-    const int qo = ch * QC * nqp;
+    const int q0 = ch * QC;
 #pragma unroll
-    for (int gi = 0; gi < GW; ++gi)
-      if (wave + 4 * gi < GI) lds_copy16(Gb + (go[gi] + qo), &Gs[st][64 * (wave + 4 * gi) * SL]);
+    for (int gi = 0; gi < GW; ++gi) {
+      if (wave + 4 * gi >= GI) continue;
+      const int k = (64 * (wave + 4 * gi) + lane) / GSR;  // row of the chunk
+      const int qo = (QC == 16 || q0 + k < nqp) ? q0 * nqp : (nqp - 1 - k) * nqp;
+      lds_copy16(Gb + (go[gi] + qo), &Gs[st][64 * (wave + 4 * gi) * SL]);
+    }
   };
 
   // lane offsets of the MFMA operand reads (k row kr = lane / 16 within a
@@ -460,8 +472,13 @@ void k_ras_gemm_lds(const RasBlockDev* __restrict__ blk, int bi,
     }
     if (ua == ub) {
       const uint64_t um = smask[B.umoff + B.u0 + ua];
-      for (int ch = 0; ch < nch && ch < 128; ++ch)
-        if ((qm[B.qmoff + ch] & ~um) == 0) live[ch >> 6] &= ~(1ull << (ch & 63));
+      constexpr int H = QC / 16;  // 16-pair pieces of a chunk (qm is per piece)
+      for (int ch = 0; ch < nch && ch < 128; ++ch) {
+        bool dead = true;
+        for (int h = 0; h < H; ++h)
+          if (ch * H + h < nqp / 16 && (qm[B.qmoff + ch * H + h] & ~um) != 0) dead = false;
+        if (dead) live[ch >> 6] &= ~(1ull << (ch & 63));
+      }
     }
   }
   // first live chunk after chunk x (nch when none)
@@ -516,7 +533,7 @@ void k_ras_gemm_lds(const RasBlockDev* __restrict__ blk, int bi,
         if (c2 < nch) fetch_idx(c2, ixn);
       }
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk) step(st, kk);
+      for (int kk = 0; kk < QC / 4; ++kk) step(st, kk);
       c0 = c1;
       c1 = c2;
       c2 = c2 < nch ? next_live(c2) : nch;
@@ -712,11 +729,12 @@ void ras_chunk(Ctx& c, const TIO* X, int ldx, TIO* Y, int ldy, int col0, int kw,
     const uint64_t* qm = c.d_rasqm.p;
     const uint64_t* smask = c.d_rasmask.p;
     if constexpr ((KC == 16 || KC == 8) && PSOCI_RAS_DMA) {
+      constexpr int LQC = sizeof(T) == 4 && PSOCI_RAS_LQC32 ? 32 : 16;
       switch (pt_w) {
-#define PSOCI_RAS_GEMML(PTV)                                                                                \
-  case PTV:                                                                                                \
-    hipLaunchKernelGGL((k_ras_gemm_lds<T, KC, PTV>), dim3(grid), dim3(256), 0, c.stream, blk, bi, didx, G, cv, \
-                       W, ntiles, npt, qm, smask);                                                         \
+#define PSOCI_RAS_GEMML(PTV)                                                                                    \
+  case PTV:                                                                                                    \
+    hipLaunchKernelGGL((k_ras_gemm_lds<T, KC, PTV, LQC>), dim3(grid), dim3(256), 0, c.stream, blk, bi, didx, G, cv, \
+                       zrow, W, ntiles, npt, qm, smask);                                                       \
     break;
         PSOCI_RAS_GEMML(16)
         PSOCI_RAS_GEMML(32)
@@ -726,8 +744,8 @@ void ras_chunk(Ctx& c, const TIO* X, int ldx, TIO* Y, int ldy, int col0, int kw,
         PSOCI_RAS_GEMML(112)
 #undef PSOCI_RAS_GEMML
         default:
-          hipLaunchKernelGGL((k_ras_gemm_lds<T, KC, 64>), dim3(grid), dim3(256), 0, c.stream, blk, bi, didx, G, cv,
-                             W, ntiles, npt, qm, smask);
+          hipLaunchKernelGGL((k_ras_gemm_lds<T, KC, 64, LQC>), dim3(grid), dim3(256), 0, c.stream, blk, bi, didx, G,
+                             cv, zrow, W, ntiles, npt, qm, smask);
       }
     } else
     switch (pt_w) {
