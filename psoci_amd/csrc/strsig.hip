@@ -427,9 +427,16 @@ void k_ras_gemm_lds(const RasBlockDev* __restrict__ blk, int bi, const int32_t* 
   // the copies of chunk ch into stage st: D pieces first (they use the
   // indices loaded a chunk earlier, which the compiler waits for), then G
   auto issue = [&](int ch, int st, const int32_t (&ix)[DI]) {
+    // every source address is formed (its index load waited for) before
+    // the first copy issues: a wait for an index load placed between two
+    // copies would also wait for the copy before it
+    const T* src[DI];
 #pragma unroll
-    for (int d = 0; d < DI; ++d)
-      lds_copy16(c + (int64_t)ix[d] * KC + v0, &Ds[st][(RW * wave * QC + d * EPI) * KC]);
+    for (int d = 0; d < DI; ++d) src[d] = c + (int64_t)ix[d] * KC + v0;
+#pragma unroll
+    for (int d = 0; d < DI; ++d) asm volatile("" : "+v"(src[d]));
+#pragma unroll
+    for (int d = 0; d < DI; ++d) lds_copy16(src[d], &Ds[st][(RW * wave * QC + d * EPI) * KC]);
     const int q0 = ch * QC;
 #pragma unroll
     for (int gi = 0; gi < GW; ++gi) {
