@@ -347,18 +347,19 @@ struct IntC {
   static constexpr int value = N;
 };
 
-// 16 bytes from this lane's global address to LDS at dst + 16 * lane (dst
-// uniform over the wavefront); completion is counted by vmcnt.  Written as
+// 16 bytes from this lane's global address to LDS at byte address lds + 16
+// * lane (lds uniform over the wavefront); completion is counted by vmcnt.  Written as
 // an asm statement so that the compiler, which cannot tell the two LDS
 // stages apart, does not wait for every copy in flight before each LDS read:
 // the kernel waits for its copies itself (vmcnt(0) before the barrier that
 // publishes a stage).  M0 carries the LDS address; nothing else in the
 // kernels that use this keeps a value in M0.
-__device__ __forceinline__ void lds_copy16(const void* src, void* dst) {
-  const uint32_t lds = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) void*)dst;
-  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src),
-               "s"(__builtin_amdgcn_readfirstlane(lds))
-               : "memory", "m0");
+__device__ __forceinline__ void lds_copy16(const void* src, uint32_t lds) {
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src), "s"(lds) : "memory", "m0");
+}
+// LDS byte address of a __shared__ location, uniform over the wavefront
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+  return __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p);
 }
 
 template <typename T, int KC, int PT, int QC = 16, int WPE = ras_lds_waves<T, PT>()>
@@ -417,6 +418,9 @@ void k_ras_gemm_lds(const RasBlockDev* __restrict__ blk, int bi, const int32_t* 
     go[gi] = k * nqp + (p < nqp ? p : 0);
   }
   const int v0 = SL * (lane % SPE);
+  // LDS addresses of this wavefront's first D copy and first G copy (stage 0)
+  const uint32_t dl0 = lds_addr(&Ds[0][RW * wave * QC * KC]);
+  const uint32_t gl0 = lds_addr(&Gs[0][64 * wave * SL]);
   auto fetch_idx = [&](int ch, int32_t (&ix)[DI]) {
 #pragma unroll
     for (int d = 0; d < DI; ++d) {
@@ -436,14 +440,15 @@ void k_ras_gemm_lds(const RasBlockDev* __restrict__ blk, int bi, const int32_t* 
 #pragma unroll
     for (int d = 0; d < DI; ++d) asm volatile("" : "+v"(src[d]));
 #pragma unroll
-    for (int d = 0; d < DI; ++d) lds_copy16(src[d], &Ds[st][(RW * wave * QC + d * EPI) * KC]);
+    for (int d = 0; d < DI; ++d)
+      lds_copy16(src[d], dl0 + (uint32_t)((st * RT * QC * KC + d * EPI * KC) * sizeof(T)));
     const int q0 = ch * QC;
 #pragma unroll
     for (int gi = 0; gi < GW; ++gi) {
       if (wave + 4 * gi >= GI) continue;
       const int k = (64 * (wave + 4 * gi) + lane) / GSR;  // row of the chunk
       const int qo = (QC == 16 || q0 + k < nqp) ? q0 * nqp : (nqp - 1 - k) * nqp;
-      lds_copy16(Gb + (go[gi] + qo), &Gs[st][64 * (wave + 4 * gi) * SL]);
+      lds_copy16(Gb + (go[gi] + qo), gl0 + (uint32_t)((st * QC * PT + 64 * 4 * gi * SL) * sizeof(T)));
     }
   };
 
@@ -488,11 +493,21 @@ void k_ras_gemm_lds(const RasBlockDev* __restrict__ blk, int bi, const int32_t* 
       }
     }
   }
-  // first live chunk after chunk x (nch when none)
+  // first live chunk after chunk x (nch when none): a find-first-set on
+  // the masks (bits past nch are set)
   auto next_live = [&](int x) {
-    for (int ch = x + 1; ch < nch; ++ch)
-      if (ch >= 128 || ((live[ch >> 6] >> (ch & 63)) & 1)) return ch;
-    return nch;
+    int y = x + 1;
+    if (y < 64) {
+      const uint64_t m = live[0] >> y;
+      if (m) return min(y + (int)__builtin_ctzll(m), nch);
+      y = 64;
+    }
+    if (y < 128) {
+      const uint64_t m = live[1] >> (y - 64);
+      if (m) return min(y + (int)__builtin_ctzll(m), nch);
+      y = 128;
+    }
+    return min(y, nch);
   };
 
   // the whole tile with NV live p-subtiles
