@@ -390,6 +390,7 @@ void ras_plan(RasPlan& P, const Space& sp, const double* one, const double* two)
   }
   (void)nb2;
   if (wunit >= ((int64_t)1 << 31)) return fail("more than 2^31 W elements per column");
+  if (dunit >= ((int64_t)1 << 31)) return fail("more than 2^31 D index entries");
   P.wunit = wunit;
   P.dunit = dunit;
   P.gemm_flops_per_col = flops;
