@@ -118,6 +118,7 @@ struct Ctx {
   int64_t ras_nterm = 0;        // terms over all SEFs (after merging mirrored pairs)
   DevBuf<int32_t> d_rasterm;    // signed W offsets (bit 31: negative)
   DevBuf<int8_t> d_rasscl;      // per SEF: 0 1/sqrt(2), 1 sqrt(2), 2 one
+  DevBuf<int2> d_rasrows;       // per SEF: signed rows of its determinants' coefficients
   void drop_ras() {
     have_ras = ras_tried = false;
     ras = RasPlan();
@@ -139,6 +140,7 @@ struct Ctx {
     d_rastoff.release();
     d_rasterm.release();
     d_rasscl.release();
+    d_rasrows.release();
   }
 
   // transition Hamiltonian (rows: this list's SEFs; columns: a second

@@ -96,37 +96,50 @@ struct Mfma<double> {
   static __device__ __forceinline__ int row(int t, int l) { return 4 * t + (l >> 4); }
 };
 
-template <typename T>
-__device__ __forceinline__ T sef_coef(uint8_t f) {
-  const T r = (T)0x1.6a09e667f3bcdp-1;
-  const T m = (f & 2) ? (T)1 : r;
-  return (f & 1) ? -m : m;
-}
-
-// c[2 det][KC] = coefficient * x[sef][col0 + q] (zero for q >= kw), c[2 det
-// + 1] its negative; the zero rows 2 ndet, 2 ndet + 1.  X may be wider than
-// T (FP64 vectors rounded to the FP32 computation on the fly).
-template <typename T, int KC, typename TX = T>
-__global__ __launch_bounds__(256) void k_ras_x2c(const RasView* __restrict__ vp, const TX* __restrict__ X, int ldx,
-                                                 int col0, int kw, T* __restrict__ c) {
+// The signed rows of every SEF's determinant coefficients (built once):
+// rows.x = 2 det + negative of its first determinant; rows.y the same for
+// the second, or -1 (one determinant, coefficient 1/sqrt(2)) or -2 (a
+// closed shell, coefficient 1); -3 marks a determinant outside the plan
+// (never, for a plan that passes its host checks).
+__global__ __launch_bounds__(256) void k_ras_sefrows(const RasView* __restrict__ vp, int2* __restrict__ rows) {
   const RasView& v = *vp;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < 2 * KC) c[2 * v.ndet * KC + t] = (T)0;
-  const int64_t s = t / KC;
-  const int q = (int)(t % KC);
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= v.nsef) return;
   const RasSef e = v.sef[s];
-  const T x = q < kw ? (T)X[s * (int64_t)ldx + col0 + q] : (T)0;
+  int r[2] = {-3, (e.flag[0] & 2) ? -2 : -1};
   for (int d = 0; d < e.n; ++d) {
     const RasSectorDev& S = v.sec[e.sec[d]];
     const int ca = ras_cls_of(v, S.na, e.a[d]), cb = ras_cls_of(v, S.nb, e.b[d]);
     const int64_t det =
         ras_det(v, e.sec[d], ca, e.a[d] - v.lev[S.na].cls_off[ca], cb, e.b[d] - v.lev[S.nb].cls_off[cb]);
-    if (det >= 0 && det < v.ndet) {
-      const T y = sef_coef<T>(e.flag[d]) * x;
-      c[2 * det * KC + q] = y;
-      c[(2 * det + 1) * KC + q] = -y;
-    }
+    r[d] = (det >= 0 && det < v.ndet) ? (int)(2 * det + (e.flag[d] & 1)) : -3;
+  }
+  rows[s] = make_int2(r[0], r[1]);
+}
+
+// c[row][KC] = (-1)^(row & 1 ^ negative) * magnitude * x[sef][col0 + q]
+// (zero for q >= kw) for both signed rows of each of the SEF's
+// determinants; the zero rows 2 ndet, 2 ndet + 1.  X may be wider than T
+// (FP64 vectors rounded to the FP32 computation on the fly).
+template <typename T, int KC, typename TX = T>
+__global__ __launch_bounds__(256) void k_ras_x2c(int64_t nsef, int64_t ndet, const int2* __restrict__ rows,
+                                                 const TX* __restrict__ X, int ldx, int col0, int kw,
+                                                 T* __restrict__ c) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < 2 * KC) c[2 * ndet * KC + t] = (T)0;
+  const int64_t s = t / KC;
+  const int q = (int)(t % KC);
+  if (s >= nsef) return;
+  const int2 r = rows[s];
+  const T x = q < kw ? (T)X[s * (int64_t)ldx + col0 + q] : (T)0;
+  const T y = (r.y == -2 ? (T)1 : (T)0x1.6a09e667f3bcdp-1) * x;
+  if (r.x >= 0) {
+    c[(int64_t)r.x * KC + q] = y;
+    c[(int64_t)(r.x ^ 1) * KC + q] = -y;
+  }
+  if (r.y >= 0) {
+    c[(int64_t)r.y * KC + q] = y;
+    c[(int64_t)(r.y ^ 1) * KC + q] = -y;
   }
 }
 
@@ -728,14 +741,13 @@ inline int ras_tile_width(int nqp) {
 template <typename T, int KC, typename TIO = T>
 void ras_chunk(Ctx& c, const TIO* X, int ldx, TIO* Y, int ldy, int col0, int kw, RasTiming* tm = nullptr) {
   const RasPlan& P = c.ras;
-  const RasView* vp = c.d_rasview.p;
   T* cv = reinterpret_cast<T*>(c.d_rasc.p);
   T* W = reinterpret_cast<T*>(c.d_rasw.p);
   const T* G = sizeof(T) == 4 ? reinterpret_cast<const T*>(c.d_rasg32.p) : reinterpret_cast<const T*>(c.d_rasg64.p);
   const int64_t nsef = (int64_t)P.sef.size();
   if (tm && tm->ev.empty()) tm->mark(c.stream, -1);
-  hipLaunchKernelGGL((k_ras_x2c<T, KC, TIO>), dim3(nblk(nsef * KC, 256)), dim3(256), 0, c.stream, vp, X, ldx, col0,
-                     kw, cv);
+  hipLaunchKernelGGL((k_ras_x2c<T, KC, TIO>), dim3(nblk(nsef * KC, 256)), dim3(256), 0, c.stream, nsef, P.ndet,
+                     c.d_rasrows.p, X, ldx, col0, kw, cv);
   PSOCI_HIP(hipGetLastError());
   if (tm) tm->mark(c.stream, 0);
   for (int bi = 0; bi < (int)P.blk.size(); ++bi) {
@@ -919,6 +931,9 @@ bool build_ras(Ctx& c) {
     c.sync();
     c.d_rasterm.alloc(nterm);
     c.d_rasscl.alloc(nsef);
+    c.d_rasrows.alloc(nsef);
+    hipLaunchKernelGGL(k_ras_sefrows, dim3(nblk(nsef, 256)), dim3(256), 0, c.stream, c.d_rasview.p, c.d_rasrows.p);
+    PSOCI_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_ras_terms<true>, dim3(nblk(nsef, 256)), dim3(256), 0, c.stream, c.d_rasview.p, nullptr,
                        c.d_rastoff.p, c.d_rasterm.p, c.d_rasscl.p, bad.p);
     PSOCI_HIP(hipGetLastError());
