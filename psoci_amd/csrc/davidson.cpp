@@ -247,9 +247,11 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     m = nc;
   };
   // symmetric orthonormalisation of columns [c0, c0+nb) of V in place (via T)
-  auto sym_orth = [&](int c0, int nb) -> int {
+  // gram: the block's Gram matrix when the caller has it (nullptr: measured)
+  auto sym_orth = [&](int c0, int nb, const double* gram = nullptr) -> int {
     G.resize((size_t)nb * nb);
-    L.gemm_tn_host(n, nb, nb, V.p + c0, mmax, V.p + c0, mmax, G.data());
+    if (gram) std::copy(gram, gram + (size_t)nb * nb, G.begin());
+    else L.gemm_tn_host(n, nb, nb, V.p + c0, mmax, V.p + c0, mmax, G.data());
     std::vector<double> lam(nb);
     Timer te;
     const bool ok = sym_eig(nb, G.data(), lam.data());
@@ -394,7 +396,16 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
       project_out(m, nb);
       bool again = m > 0;
       if (phase32 && m > 0) {
-        L.gemm_tn_host(n, nb, nb, V.p + m, mmax, V.p + m, mmax, G1.data());
+        // V[:, 0:m) is orthonormal, so after x -= V (V^T x) the block's Gram
+        // matrix is G0 - P^T P (P = V^T x, m x nb): exact where the test
+        // below accepts one pass (at most half of a column's squared norm
+        // removed: cancellation costs at most a factor 2 in relative error)
+        for (int i = 0; i < nb; ++i)
+          for (int j = 0; j < nb; ++j) {
+            double g = G0[(size_t)i * nb + j];
+            for (int t = 0; t < m; ++t) g -= P[(size_t)t * nb + i] * P[(size_t)t * nb + j];
+            G1[(size_t)i * nb + j] = g;
+          }
         again = false;
         for (int j = 0; j < nb; ++j)
           if (!(G1[(size_t)j * nb + j] >= 0.5 * G0[(size_t)j * nb + j]) || !(G1[(size_t)j * nb + j] > 0.0))
@@ -409,7 +420,7 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
         }
       }
       if (again) project_out(m, nb);
-      nb = sym_orth(m, nb);
+      nb = sym_orth(m, nb, (phase32 && m > 0 && !again) ? G1.data() : nullptr);
       if (nb > 0 && again) {
         project_out(m, nb);
         nb = sym_orth(m, nb);
