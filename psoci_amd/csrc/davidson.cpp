@@ -366,7 +366,6 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     }
     int nsel = (int)sel.size();
     if (olsen) L.precondition_olsen(n, k, c.d_diag.p, w.data(), 1.0e-3, R.p, k, T.p, k);
-    else L.precondition(n, k, c.d_diag.p, w.data(), 1.0e-3, R.p, k);
     if (m + nsel > mmax) {
       collapse(kc);
       // the retained Ritz values and vectors are unchanged: w[0..k) still valid
@@ -378,9 +377,13 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     if (m + nsel > mmax) nsel = mmax - m;
     if (nsel <= 0) break;
     // new block = selected preconditioned residuals, placed at V[:, m:m+nsel)
-    std::vector<double> Ssel((size_t)k * nsel, 0.0);
-    for (int j = 0; j < nsel; ++j) Ssel[(size_t)sel[j] * nsel + j] = 1.0;
-    L.gemm_nn(n, k, nsel, R.p, k, Ssel.data(), 0.0, V.p + m, mmax);
+    if (olsen) {
+      std::vector<double> Ssel((size_t)k * nsel, 0.0);
+      for (int j = 0; j < nsel; ++j) Ssel[(size_t)sel[j] * nsel + j] = 1.0;
+      L.gemm_nn(n, k, nsel, R.p, k, Ssel.data(), 0.0, V.p + m, mmax);
+    } else {
+      L.precondition_select(n, k, c.d_diag.p, w.data(), 1.0e-3, R.p, k, sel.data(), nsel, V.p + m, mmax);
+    }
     int nb = nsel;
     {
       // block Gram-Schmidt against V[:, 0:m) and symmetric orthonormalisation

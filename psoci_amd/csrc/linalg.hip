@@ -220,6 +220,23 @@ __global__ void k_olsen_apply(int64_t n, int k, const double* __restrict__ D, co
 }
 
 // t[row][c] = r[row][c] / (D[row] - theta[c]) when |D - theta| > dmin
+// Out[row][j] = R[row][sel[j]] / (D[row] - theta[sel[j]]) where that
+// denominator exceeds dmin in magnitude, else R[row][sel[j]]: the diagonal
+// preconditioner applied to the selected columns on their way into the
+// basis (one pass; the same numbers as preconditioning R and then copying)
+__global__ void k_precondition_select(int64_t n, int ns, const double* __restrict__ D,
+                                      const double* __restrict__ theta_sel, const int* __restrict__ sel,
+                                      double dmin, const double* __restrict__ R, int ldr, double* __restrict__ Out,
+                                      int ldo) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t row = g / ns;
+  const int j = (int)(g - row * ns);
+  if (row >= n) return;
+  const double r = R[row * ldr + sel[j]];
+  const double den = D[row] - theta_sel[j];
+  Out[row * ldo + j] = fabs(den) > dmin ? r / den : r;
+}
+
 __global__ void k_precondition(int64_t n, int k, const double* __restrict__ D, const double* __restrict__ theta,
                                double dmin, double* __restrict__ R, int ldr) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -375,6 +392,23 @@ void Linalg::precondition(int64_t n, int k, const double* D, const double* theta
   PSOCI_HIP(hipMemcpyAsync(theta.p, theta_host, k * sizeof(double), hipMemcpyHostToDevice, stream));
   hipLaunchKernelGGL(k_precondition, dim3(nblk(n * k, 256)), dim3(256), 0, stream, n, k, D, theta.p, dmin, R, ldr);
   PSOCI_HIP(hipGetLastError());
+}
+
+void Linalg::precondition_select(int64_t n, int k, const double* D, const double* theta_host, double dmin,
+                                 const double* R, int ldr, const int* sel_host, int ns, double* Out, int ldo) {
+  if (ns == 0 || n == 0) return;
+  std::vector<double> ts(ns);
+  for (int j = 0; j < ns; ++j) ts[j] = theta_host[sel_host[j]];
+  (void)k;
+  theta.reserve(ns);
+  isel.reserve(ns);
+  PSOCI_HIP(hipMemcpyAsync(theta.p, ts.data(), ns * sizeof(double), hipMemcpyHostToDevice, stream));
+  PSOCI_HIP(hipMemcpyAsync(isel.p, sel_host, ns * sizeof(int), hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(k_precondition_select, dim3(nblk(n * ns, 256)), dim3(256), 0, stream, n, ns, D, theta.p,
+                     isel.p, dmin, R, ldr, Out, ldo);
+  PSOCI_HIP(hipGetLastError());
+  // ts is read by the copy above: keep it until the stream has consumed it
+  PSOCI_HIP(hipStreamSynchronize(stream));
 }
 
 void Linalg::precondition_olsen(int64_t n, int k, const double* D, const double* theta_host, double dmin, double* R,

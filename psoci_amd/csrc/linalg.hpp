@@ -7,6 +7,7 @@ namespace psoci {
 struct Linalg {
   hipStream_t stream = nullptr;
   DevBuf<double> part, small, ysmall, theta;
+  DevBuf<int> isel;
   std::vector<double> yneg;
 
   // partial-sum buffer of gemm_tn for (n, m, b); reserve_tn: for every
@@ -28,6 +29,9 @@ struct Linalg {
                 const double* Y2_host, double* Out, int ldo, double* X = nullptr, int ldx = 0);
   // R[:, c] /= (D - theta_c) where |D - theta_c| > dmin
   void precondition(int64_t n, int k, const double* D, const double* theta_host, double dmin, double* R, int ldr);
+  // Out[:, j) = the diagonal preconditioner applied to R[:, sel[j]] (j < ns)
+  void precondition_select(int64_t n, int k, const double* D, const double* theta_host, double dmin, const double* R,
+                           int ldr, const int* sel_host, int ns, double* Out, int ldo);
   // the same with Olsen's correction against the Ritz vectors X:
   // R[:, c] = f (R - eps_c X) with eps_c = (x . f r) / (x . f x)
   void precondition_olsen(int64_t n, int k, const double* D, const double* theta_host, double dmin, double* R, int ldr,
