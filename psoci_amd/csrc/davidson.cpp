@@ -96,14 +96,9 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
   } else if (p.guess == 1) {
     std::iota(idx.begin(), idx.end(), 0);
   } else {
-    std::vector<double> D(n);
-    c.d_diag.download(D.data(), n, c.stream);
-    c.sync();
-    std::vector<int64_t> ord(n);
-    std::iota(ord.begin(), ord.end(), 0);
-    std::partial_sort(ord.begin(), ord.begin() + k, ord.end(),
-                      [&](int64_t a, int64_t b) { return D[a] < D[b] || (D[a] == D[b] && a < b); });
-    std::copy(ord.begin(), ord.begin() + k, idx.begin());
+    // the k lowest diagonal elements: per-slice candidates on the GPU, the
+    // final order of the few candidates on the host
+    L.lowest(n, k, c.d_diag.p, idx.data());
   }
   if (p.guess == 2) {
     PSOCI_CHECK(evecs, PSOCI_ERR_ARG, "guess 2 needs the starting vectors in evecs");
@@ -246,12 +241,11 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     for (int i = 0; i < nc; ++i) A[(size_t)i * mmax + i] = w[i];
     m = nc;
   };
-  // symmetric orthonormalisation of columns [c0, c0+nb) of V in place (via T)
-  // gram: the block's Gram matrix when the caller has it (nullptr: measured)
-  auto sym_orth = [&](int c0, int nb, const double* gram = nullptr) -> int {
-    G.resize((size_t)nb * nb);
-    if (gram) std::copy(gram, gram + (size_t)nb * nb, G.begin());
-    else L.gemm_tn_host(n, nb, nb, V.p + c0, mmax, V.p + c0, mmax, G.data());
+  // the symmetric orthonormaliser of a block with Gram matrix gram (nb x
+  // nb): M (nb x nk) with M^T gram M = I over the eigenvectors whose
+  // eigenvalues are not negligible; returns nk (0: none left or failure)
+  auto orth_matrix = [&](int nb, const double* gram, std::vector<double>& M) -> int {
+    G.assign(gram, gram + (size_t)nb * nb);
     std::vector<double> lam(nb);
     Timer te;
     const bool ok = sym_eig(nb, G.data(), lam.data());
@@ -263,9 +257,17 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
       if (lam[j] > 1e-14 * lmax && lam[j] > 1e-28) keep.push_back(j);
     int nk = (int)keep.size();
     if (nk == 0) return 0;
-    std::vector<double> M((size_t)nb * nk);
+    M.resize((size_t)nb * nk);
     for (int i = 0; i < nb; ++i)
       for (int j = 0; j < nk; ++j) M[(size_t)i * nk + j] = G[(size_t)i * nb + keep[j]] / std::sqrt(lam[keep[j]]);
+    return nk;
+  };
+  // symmetric orthonormalisation of columns [c0, c0+nb) of V in place (via T)
+  auto sym_orth = [&](int c0, int nb) -> int {
+    std::vector<double> Gb((size_t)nb * nb), M;
+    L.gemm_tn_host(n, nb, nb, V.p + c0, mmax, V.p + c0, mmax, Gb.data());
+    const int nk = orth_matrix(nb, Gb.data(), M);
+    if (nk == 0) return 0;
     if (nk <= 128) {
       L.gemm_nn_inplace(n, nb, nk, M.data(), V.p + c0, mmax);
     } else {
@@ -395,10 +397,14 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
       // matrix below 1e-2); the FP64 phase starts from a re-orthonormalised
       // basis (below, at the switch) and always takes both passes
       std::vector<double> G0((size_t)nb * nb), G1((size_t)nb * nb);
-      if (phase32 && m > 0) L.gemm_tn_host(n, nb, nb, V.p + m, mmax, V.p + m, mmax, G0.data());
-      project_out(m, nb);
-      bool again = m > 0;
+      bool again = m > 0, fused = false;
       if (phase32 && m > 0) {
+        // one projection gives P = V^T x (rows 0..m) and the block's Gram
+        // matrix G0 (rows m..m+nb)
+        std::vector<double> Q((size_t)(m + nb) * nb);
+        L.gemm_tn_host(n, m + nb, nb, V.p, mmax, V.p + m, mmax, Q.data());
+        P.assign(Q.begin(), Q.begin() + (size_t)m * nb);
+        std::copy(Q.begin() + (size_t)m * nb, Q.end(), G0.begin());
         // V[:, 0:m) is orthonormal, so after x -= V (V^T x) the block's Gram
         // matrix is G0 - P^T P (P = V^T x, m x nb): exact where the test
         // below accepts one pass (at most half of a column's squared norm
@@ -421,12 +427,41 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
                   G1[(size_t)i * nb + j] / std::sqrt(G1[(size_t)i * nb + i] * G1[(size_t)j * nb + j]);
           if (!sym_eig(nb, C.data(), lam.data()) || !(lam[0] >= 1e-2)) again = true;
         }
-      }
-      if (again) project_out(m, nb);
-      nb = sym_orth(m, nb, (phase32 && m > 0 && !again) ? G1.data() : nullptr);
-      if (nb > 0 && again) {
+        if (!again) {
+          // one pass: the projection and the orthonormalisation together,
+          // x M - V (P M) = [V x] [-P M; M] into V[:, m:m+nk)
+          std::vector<double> M;
+          const int nk = orth_matrix(nb, G1.data(), M);
+          if (nk > 0) {
+            std::vector<double> Yf((size_t)(m + nb) * nk, 0.0);
+            for (int t = 0; t < m; ++t)
+              for (int j = 0; j < nk; ++j) {
+                double acc = 0.0;
+                for (int i = 0; i < nb; ++i) acc += P[(size_t)t * nb + i] * M[(size_t)i * nk + j];
+                Yf[(size_t)t * nk + j] = -acc;
+              }
+            for (int i = 0; i < nb; ++i)
+              for (int j = 0; j < nk; ++j) Yf[(size_t)(m + i) * nk + j] = M[(size_t)i * nk + j];
+            L.gemm_nn_rows(n, m + nb, nk, V.p, mmax, Yf.data(), V.p + m, mmax);
+          }
+          nb = nk;
+          fused = true;
+        } else {
+          // the first pass's update from the P at hand, then the second pass
+          std::vector<double> Pn(P.size());
+          for (size_t e = 0; e < P.size(); ++e) Pn[e] = -P[e];
+          L.gemm_nn(n, m, nb, V.p, mmax, Pn.data(), 1.0, V.p + m, mmax);
+        }
+      } else {
         project_out(m, nb);
+      }
+      if (!fused) {
+        if (again) project_out(m, nb);
         nb = sym_orth(m, nb);
+        if (nb > 0 && again) {
+          project_out(m, nb);
+          nb = sym_orth(m, nb);
+        }
       }
     }
     if (nb == 0) break;  // no new directions: stagnation

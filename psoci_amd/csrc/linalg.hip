@@ -7,6 +7,9 @@
 // precondition :1860-1920).  Vectors are HBM-resident [n][ld] row-major
 // blocks; every product over n is one launch.
 #include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <vector>
 
 #include "linalg.hpp"
 
@@ -17,6 +20,14 @@ namespace {
 constexpr int kWave = 64;
 constexpr size_t kMaxLds = 48 * 1024;  // dynamic LDS per block used without opt-in
 typedef double double4_t __attribute__((ext_vector_type(4)));
+#ifndef PSOCI_TN_U
+#define PSOCI_TN_U 1
+#endif
+constexpr int kTnU = PSOCI_TN_U;  // 16-row steps per iteration of the projection loop
+#ifndef PSOCI_TN_ROUNDS
+#define PSOCI_TN_ROUNDS 1
+#endif
+constexpr int kTnRounds = PSOCI_TN_ROUNDS;  // rounds of resident workgroups per projection
 
 // C_partial[chunk][m][b] = sum over rows of chunk of A[row][a0 + i] * B[row][b0 + j]
 // for one 16x16 output tile per wavefront, on v_mfma_f64_16x16x4f64.
@@ -39,22 +50,25 @@ __global__ __launch_bounds__(256) void k_gemm_tn_mfma(int64_t n, int m, int b, c
   const int cj = tb * 16 + (lane & 15);
   const int kk = lane >> 4;
   const bool vi = ci < m, vj = cj < b;
-  // four independent accumulation chains, 16 rows per step: the loads of a
-  // step are all issued before its MFMAs
+  // four independent accumulation chains, 4 * TN_U rows per chain and step:
+  // the loads of a step are all issued before its MFMAs
   double4_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0, acc2 = acc0, acc3 = acc0;
-  for (int64_t r = r0; r < r1; r += 16) {
-    double a[4], bb[4];
+  for (int64_t r = r0; r < r1; r += 16 * kTnU) {
+    double a[4 * kTnU], bb[4 * kTnU];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 4 * kTnU; ++u) {
       const int64_t row = r + 4 * u + kk;
       const bool in = row < r1;
       a[u] = (in && vi) ? A[row * lda + ci] : 0.0;
       bb[u] = (in && vj) ? B[row * ldb + cj] : 0.0;
     }
-    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], bb[0], acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], bb[1], acc1, 0, 0, 0);
-    acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2], bb[2], acc2, 0, 0, 0);
-    acc3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[3], bb[3], acc3, 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < kTnU; ++u) {
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[4 * u], bb[4 * u], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[4 * u + 1], bb[4 * u + 1], acc1, 0, 0, 0);
+      acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[4 * u + 2], bb[4 * u + 2], acc2, 0, 0, 0);
+      acc3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[4 * u + 3], bb[4 * u + 3], acc3, 0, 0, 0);
+    }
   }
   const double4_t acc = (acc0 + acc1) + (acc2 + acc3);
   double* out = partial + chunk * (int64_t)m * b;
@@ -95,38 +109,50 @@ __global__ __launch_bounds__(256) void k_reduce_chunks(int64_t nchunks, int mb, 
 // row is read from HBM once; Y (m x k, tiny) is read through the caches.
 // INPLACE: the rows are read from Out itself (A unused) and overwritten by
 // the product (k <= 128: one wavefront reads every input value of its rows
-// before it writes any output of them).
-template <int NCT, bool TWO, bool XOUT, bool INPLACE = false>
-__global__ __launch_bounds__(256) void k_gemm_nn_mfma(int64_t n, int m, int k, const double* __restrict__ A,
+// before it writes any output of them).  For the same reason A may hold the
+// same rows as Out at other or overlapping columns when k <= 128.
+template <int NCT, bool TWO, bool XOUT, bool INPLACE = false, int RT = 1>
+__global__ __launch_bounds__(256) void k_gemm_nn_mfma(int64_t n, int m, int k, const double* A,
                                                       int lda, const double* __restrict__ B, int ldb,
                                                       const double* __restrict__ Y1, const double* __restrict__ Y2,
                                                       double alpha, double* Out, int ldo, double* __restrict__ Out2,
                                                       int ldo2) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const int64_t r0 = ((int64_t)blockIdx.x * (blockDim.x / kWave) + wave) * 16;
+  const int64_t r0 = ((int64_t)blockIdx.x * (blockDim.x / kWave) + wave) * 16 * RT;
   if (r0 >= n) return;  // uniform per wavefront
   const int cb = blockIdx.y * 128;
   const int i = lane & 15, q = lane >> 4;
-  const int64_t row = r0 + i;
-  const bool rv = row < n;
-  const double* a_row = INPLACE ? Out + (rv ? row : 0) * (int64_t)ldo : A + (rv ? row : 0) * (int64_t)lda;
-  const double* __restrict__ b_row = TWO ? B + (rv ? row : 0) * (int64_t)ldb : nullptr;
-  double4_t acc[NCT], acc2[NCT];
+  const double* a_row[RT];
+  const double* b_row[RT];
+  bool rv[RT];
 #pragma unroll
-  for (int ct = 0; ct < NCT; ++ct) {
-    acc[ct] = double4_t{0.0, 0.0, 0.0, 0.0};
-    acc2[ct] = acc[ct];
+  for (int rt = 0; rt < RT; ++rt) {
+    const int64_t row = r0 + 16 * rt + i;
+    rv[rt] = row < n;
+    const int64_t rs = rv[rt] ? row : 0;
+    a_row[rt] = INPLACE ? Out + rs * (int64_t)ldo : A + rs * (int64_t)lda;
+    b_row[rt] = TWO ? B + rs * (int64_t)ldb : nullptr;
   }
+  double4_t acc[RT][NCT], acc2[RT][NCT];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) {
+      acc[rt][ct] = double4_t{0.0, 0.0, 0.0, 0.0};
+      acc2[rt][ct] = acc[rt][ct];
+    }
   for (int t0 = 0; t0 < m; t0 += 16) {
     const int tq = t0 + 4 * q;
-    double a[4], b[4];
+    double a[RT][4], b[RT][4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const bool tv = rv && tq + u < m;
-      a[u] = tv ? a_row[tq + u] : 0.0;
-      b[u] = (TWO && tv) ? b_row[tq + u] : 0.0;
-    }
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const bool tv = rv[rt] && tq + u < m;
+        a[rt][u] = tv ? a_row[rt][tq + u] : 0.0;
+        b[rt][u] = (TWO && tv) ? b_row[rt][tq + u] : 0.0;
+      }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int t = tq + u;
@@ -135,28 +161,31 @@ __global__ __launch_bounds__(256) void k_gemm_nn_mfma(int64_t n, int m, int k, c
         const int c = cb + ct * 16 + i;
         const bool yv = t < m && c < k;
         const double y1 = yv ? Y1[(int64_t)t * k + c] : 0.0;
-        acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], y1, acc[ct], 0, 0, 0);
-        if (XOUT) acc2[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[u], y1, acc2[ct], 0, 0, 0);
-        if (TWO) {
-          const double y2 = yv ? Y2[(int64_t)t * k + c] : 0.0;
-          acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[u], y2, acc[ct], 0, 0, 0);
+        const double y2 = (TWO && yv) ? Y2[(int64_t)t * k + c] : 0.0;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+          acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt][u], y1, acc[rt][ct], 0, 0, 0);
+          if (XOUT) acc2[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[rt][u], y1, acc2[rt][ct], 0, 0, 0);
+          if (TWO) acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[rt][u], y2, acc[rt][ct], 0, 0, 0);
         }
       }
     }
   }
 #pragma unroll
-  for (int ct = 0; ct < NCT; ++ct) {
-    const int c = cb + ct * 16 + i;
-    if (c >= k) continue;
+  for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int64_t rr = r0 + 4 * j + q;
-      if (rr >= n) continue;
-      double* o = Out + rr * ldo + c;
-      *o = (alpha == 0.0) ? acc[ct][j] : __builtin_fma(alpha, *o, acc[ct][j]);
-      if (XOUT) Out2[rr * ldo2 + c] = acc2[ct][j];
+    for (int ct = 0; ct < NCT; ++ct) {
+      const int c = cb + ct * 16 + i;
+      if (c >= k) continue;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t rr = r0 + 16 * rt + 4 * j + q;
+        if (rr >= n) continue;
+        double* o = Out + rr * ldo + c;
+        *o = (alpha == 0.0) ? acc[rt][ct][j] : __builtin_fma(alpha, *o, acc[rt][ct][j]);
+        if (XOUT) Out2[rr * ldo2 + c] = acc2[rt][ct][j];
+      }
     }
-  }
 }
 
 // Olsen's correction of the diagonal preconditioner, per column c (root
@@ -281,6 +310,81 @@ __global__ void k_unit_cols(int64_t n, int k, const int64_t* __restrict__ idx, d
   V[row * ldv + c] = (row == idx[c]) ? 1.0 : 0.0;
 }
 
+// The k lowest entries of D (k <= 128), ordered by (value, index), of each
+// slice of 4096 consecutive rows: block b writes its candidates to
+// val[b * k + r], idx[b * k + r] (index -1 past the slice's size).  The k
+// lowest of the whole vector are among the candidates of all slices.
+constexpr int kLowSlice = 4096;
+constexpr int kLowPer = kLowSlice / 256;
+
+__device__ __forceinline__ bool low_less(double a, int64_t ia, double b, int64_t ib) {
+  return a < b || (a == b && ia < ib);
+}
+
+__global__ void __launch_bounds__(256) k_lowest_cands(int64_t n, int k, const double* __restrict__ D,
+                                                      double* __restrict__ val, int64_t* __restrict__ idx) {
+  __shared__ double sv[4];
+  __shared__ int64_t si[4];
+  const int64_t base = (int64_t)blockIdx.x * kLowSlice;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  double v[kLowPer];
+  unsigned left = 0;  // bit e: element base + t + 256 e is in range and not yet taken
+#pragma unroll
+  for (int e = 0; e < kLowPer; ++e) {
+    const int64_t row = base + t + 256 * e;
+    v[e] = row < n ? D[row] : 0.0;
+    if (row < n) left |= 1u << e;
+  }
+  auto local = [&](double& bv, int64_t& bi) {
+    bv = HUGE_VAL;
+    bi = INT64_MAX;
+#pragma unroll
+    for (int e = 0; e < kLowPer; ++e)
+      if (((left >> e) & 1) && low_less(v[e], base + t + 256 * e, bv, bi)) {
+        bv = v[e];
+        bi = base + t + 256 * e;
+      }
+  };
+  double bv;
+  int64_t bi;
+  local(bv, bi);
+  for (int r = 0; r < k; ++r) {
+    double mv = bv;
+    int64_t mi = bi;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      const double ov = __shfl_xor(mv, o);
+      const int64_t oi = __shfl_xor(mi, o);
+      if (low_less(ov, oi, mv, mi)) {
+        mv = ov;
+        mi = oi;
+      }
+    }
+    if (lane == 0) {
+      sv[wv] = mv;
+      si[wv] = mi;
+    }
+    __syncthreads();
+    mv = sv[0];
+    mi = si[0];
+#pragma unroll
+    for (int w = 1; w < 4; ++w)
+      if (low_less(sv[w], si[w], mv, mi)) {
+        mv = sv[w];
+        mi = si[w];
+      }
+    __syncthreads();
+    if (t == 0) {
+      val[(int64_t)blockIdx.x * k + r] = mv;
+      idx[(int64_t)blockIdx.x * k + r] = mi == INT64_MAX ? -1 : mi;
+    }
+    if (mi == bi && mi != INT64_MAX) {
+      left &= ~(1u << (int)((mi - base - t) >> 8));
+      local(bv, bi);
+    }
+  }
+}
+
 inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
 }  // namespace
@@ -288,9 +392,38 @@ inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per
 namespace {
 
 // about 4096 wavefronts in flight: chunks of a multiple of 16 rows
+// Wavefronts per workgroup of k_gemm_tn_mfma for a tile count: the tiles
+// spread evenly over the fewest workgroups of at most 4 wavefronts.
+int tn_waves(int tiles) {
+  const int groups = (tiles + 3) / 4;
+  return (tiles + groups - 1) / groups;
+}
+
+// Workgroups of k_gemm_tn_mfma with w wavefronts that the current device
+// holds at once (compute units x resident workgroups per unit), queried once
+// per device and width.
+int64_t tn_resident(int w) {
+  static int64_t cache[64][5] = {};
+  int dev = 0;
+  PSOCI_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) dev = 0;
+  if (cache[dev][w] == 0) {
+    int per_cu = 0, cus = 0;
+    PSOCI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gemm_tn_mfma, kWave * w, 0));
+    PSOCI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    cache[dev][w] = std::max<int64_t>(1, (int64_t)per_cu * cus);
+  }
+  return cache[dev][w];
+}
+
+// Row chunks of the projection: one chunk per group of workgroups covering
+// the output tiles, as many groups as the device holds at once (a single
+// round of workgroups, no partly filled second round).
 void tn_chunks(int64_t n, int m, int b, int64_t& rows_per_chunk, int64_t& nchunks) {
   const int tiles = ((m + 15) / 16) * ((b + 15) / 16);
-  int64_t want = std::max<int64_t>(1, 4096 / tiles);
+  const int w = tn_waves(tiles);
+  const int64_t blocks_per_chunk = (tiles + w - 1) / w;
+  int64_t want = std::max<int64_t>(1, kTnRounds * tn_resident(w) / blocks_per_chunk);
   want = std::min<int64_t>(want, 65535);
   rows_per_chunk = (n + want - 1) / want;
   rows_per_chunk = std::max<int64_t>(64, (rows_per_chunk + 15) / 16 * 16);
@@ -318,8 +451,10 @@ void Linalg::gemm_tn(int64_t n, int m, int b, const double* A, int lda, const do
   tn_chunks(n, m, b, rows_per_chunk, nchunks);
   part.reserve((size_t)nchunks * m * b);
   const int tiles = ((m + 15) / 16) * ((b + 15) / 16);
-  dim3 grid(nblk(tiles, 4), (unsigned)nchunks);
-  hipLaunchKernelGGL(k_gemm_tn_mfma, grid, dim3(256), 0, stream, n, m, b, A, lda, B, ldb, rows_per_chunk, part.p);
+  const int w = tn_waves(tiles);
+  dim3 grid(nblk(tiles, w), (unsigned)nchunks);
+  hipLaunchKernelGGL(k_gemm_tn_mfma, grid, dim3(kWave * w), 0, stream, n, m, b, A, lda, B, ldb, rows_per_chunk,
+                     part.p);
   PSOCI_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)(m * b)), dim3(256), 0, stream, nchunks, m * b, part.p, out_dev);
   PSOCI_HIP(hipGetLastError());
@@ -334,18 +469,26 @@ void Linalg::gemm_tn_host(int64_t n, int m, int b, const double* A, int lda, con
 
 namespace {
 
+#ifndef PSOCI_NN_RT1
+#define PSOCI_NN_RT1 4
+#endif
+#ifndef PSOCI_NN_RT2
+#define PSOCI_NN_RT2 2
+#endif
+constexpr int kNnRt1 = PSOCI_NN_RT1, kNnRt2 = PSOCI_NN_RT2;
+
 template <bool TWO, bool XOUT, bool INPLACE = false>
 void launch_nn(hipStream_t st, int64_t n, int m, int k, const double* A, int lda, const double* B, int ldb,
                const double* Y1, const double* Y2, double alpha, double* Out, int ldo, double* Out2, int ldo2) {
-  const dim3 grid(nblk(n, 64), (unsigned)((k + 127) / 128));
   const int nct = std::min(8, (k + 15) / 16);
-#define PSOCI_NN(NCT)                                                                                      \
-  hipLaunchKernelGGL((k_gemm_nn_mfma<NCT, TWO, XOUT, INPLACE>), grid, dim3(256), 0, st, n, m, k, A, lda, B, \
-                     ldb, Y1, Y2, alpha, Out, ldo, Out2, ldo2)
-  if (nct <= 1) PSOCI_NN(1);
-  else if (nct <= 2) PSOCI_NN(2);
-  else if (nct <= 4) PSOCI_NN(4);
-  else PSOCI_NN(8);
+  // row tiles per wavefront: each Y value read feeds RT tiles of 16 rows
+#define PSOCI_NN(NCT, RT)                                                                                      \
+  hipLaunchKernelGGL((k_gemm_nn_mfma<NCT, TWO, XOUT, INPLACE, RT>), dim3(nblk(n, 64 * RT), (unsigned)((k + 127) / 128)), \
+                     dim3(256), 0, st, n, m, k, A, lda, B, ldb, Y1, Y2, alpha, Out, ldo, Out2, ldo2)
+  if (nct <= 1) PSOCI_NN(1, kNnRt1);
+  else if (nct <= 2) PSOCI_NN(2, kNnRt2);
+  else if (nct <= 4) PSOCI_NN(4, 1);
+  else PSOCI_NN(8, 1);
 #undef PSOCI_NN
   PSOCI_HIP(hipGetLastError());
 }
@@ -358,6 +501,15 @@ void Linalg::gemm_nn(int64_t n, int m, int k, const double* A, int lda, const do
   ysmall.reserve((size_t)m * k);
   PSOCI_HIP(hipMemcpyAsync(ysmall.p, Y1_host, (size_t)m * k * sizeof(double), hipMemcpyHostToDevice, stream));
   launch_nn<false, false>(stream, n, m, k, A, lda, nullptr, 0, ysmall.p, nullptr, alpha, Out, ldo, nullptr, 0);
+}
+
+void Linalg::gemm_nn_rows(int64_t n, int m, int k, const double* A, int lda, const double* Y1_host, double* Out,
+                          int ldo) {
+  if (k == 0 || n == 0) return;
+  PSOCI_CHECK(k <= 128, PSOCI_ERR_ARG, "row-aliased product: at most 128 output columns");
+  ysmall.reserve((size_t)m * k);
+  PSOCI_HIP(hipMemcpyAsync(ysmall.p, Y1_host, (size_t)m * k * sizeof(double), hipMemcpyHostToDevice, stream));
+  launch_nn<false, false>(stream, n, m, k, A, lda, nullptr, 0, ysmall.p, nullptr, 0.0, Out, ldo, nullptr, 0);
 }
 
 void Linalg::gemm_nn_inplace(int64_t n, int m, int k, const double* Y1_host, double* Out, int ldo) {
@@ -443,6 +595,29 @@ void Linalg::to_f64(int64_t n, int k, const float* A, int lda, double* B, int ld
 void Linalg::copy_cols(int64_t n, int k, const double* A, int lda, double* B, int ldb) {
   hipLaunchKernelGGL(k_copy_cols, dim3(nblk(n * k, 256)), dim3(256), 0, stream, n, k, A, lda, B, ldb);
   PSOCI_HIP(hipGetLastError());
+}
+
+void Linalg::lowest(int64_t n, int k, const double* D, int64_t* idx_host) {
+  PSOCI_CHECK(k >= 1 && k <= 128 && k <= n, PSOCI_ERR_ARG, "lowest: k must be in 1..min(128, n)");
+  const int64_t nb = (n + kLowSlice - 1) / kLowSlice;
+  part.reserve((size_t)nb * k);
+  ipart.reserve((size_t)nb * k);
+  hipLaunchKernelGGL(k_lowest_cands, dim3((unsigned)nb), dim3(256), 0, stream, n, k, D, part.p, ipart.p);
+  PSOCI_HIP(hipGetLastError());
+  std::vector<double> cv((size_t)nb * k);
+  std::vector<int64_t> ci((size_t)nb * k);
+  part.download(cv.data(), cv.size(), stream);
+  ipart.download(ci.data(), ci.size(), stream);
+  PSOCI_HIP(hipStreamSynchronize(stream));
+  std::vector<int64_t> ord;
+  ord.reserve(ci.size());
+  for (size_t j = 0; j < ci.size(); ++j)
+    if (ci[j] >= 0) ord.push_back((int64_t)j);
+  PSOCI_CHECK((int64_t)ord.size() >= k, PSOCI_ERR_ARG, "lowest: too few candidates");
+  std::partial_sort(ord.begin(), ord.begin() + k, ord.end(), [&](int64_t a, int64_t b) {
+    return cv[a] < cv[b] || (cv[a] == cv[b] && ci[a] < ci[b]);
+  });
+  for (int r = 0; r < k; ++r) idx_host[r] = ci[ord[r]];
 }
 
 void Linalg::unit_cols(int64_t n, int k, const int64_t* idx_host, double* V, int ldv) {

@@ -8,6 +8,7 @@ struct Linalg {
   hipStream_t stream = nullptr;
   DevBuf<double> part, small, ysmall, theta;
   DevBuf<int> isel;
+  DevBuf<int64_t> ipart;
   std::vector<double> yneg;
 
   // partial-sum buffer of gemm_tn for (n, m, b); reserve_tn: for every
@@ -25,6 +26,9 @@ struct Linalg {
   // in the same pass over A and B (synchronises)
   // Out[:, 0:k) = Out[:, 0:m) Y (m x k, host), in place (k <= m, k <= 128)
   void gemm_nn_inplace(int64_t n, int m, int k, const double* Y1_host, double* Out, int ldo);
+  // Out[:, 0:k) = A[:, 0:m) Y where A and Out may be the same rows of one
+  // buffer at other or overlapping columns (k <= 128)
+  void gemm_nn_rows(int64_t n, int m, int k, const double* A, int lda, const double* Y1_host, double* Out, int ldo);
   void gemm_nn2(int64_t n, int m, int k, const double* A, int lda, const double* Y1_host, const double* B, int ldb,
                 const double* Y2_host, double* Out, int ldo, double* X = nullptr, int ldx = 0);
   // R[:, c] /= (D - theta_c) where |D - theta_c| > dmin
@@ -40,6 +44,9 @@ struct Linalg {
   void to_f64(int64_t n, int k, const float* A, int lda, double* B, int ldb);
   void copy_cols(int64_t n, int k, const double* A, int lda, double* B, int ldb);
   void unit_cols(int64_t n, int k, const int64_t* idx_host, double* V, int ldv);
+  // idx_host[0..k) = the rows of the k lowest D entries in ascending order,
+  // ties by the lower row (k <= 128; synchronises)
+  void lowest(int64_t n, int k, const double* D, int64_t* idx_host);
 };
 
 // Symmetric eigen-decomposition of a small dense matrix (host, FP64):
