@@ -23,6 +23,10 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 #ifndef PSOCI_TN_U
 #define PSOCI_TN_U 1
 #endif
+#ifndef PSOCI_NN_FULL
+#define PSOCI_NN_FULL 1
+#endif
+constexpr bool kNnFull = PSOCI_NN_FULL;  // unpredicated loads for whole chunks of the basis update
 constexpr int kTnU = PSOCI_TN_U;  // 16-row steps per iteration of the projection loop
 #ifndef PSOCI_TN_ROUNDS
 #define PSOCI_TN_ROUNDS 1
@@ -145,14 +149,26 @@ __global__ __launch_bounds__(256) void k_gemm_nn_mfma(int64_t n, int m, int k, c
   for (int t0 = 0; t0 < m; t0 += 16) {
     const int tq = t0 + 4 * q;
     double a[RT][4], b[RT][4];
+    if (kNnFull && t0 + 16 <= m) {
+      // a whole chunk: unpredicated loads (rows past n read row 0; their
+      // products land in output rows that are not written)
 #pragma unroll
-    for (int rt = 0; rt < RT; ++rt)
+      for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const bool tv = rv[rt] && tq + u < m;
-        a[rt][u] = tv ? a_row[rt][tq + u] : 0.0;
-        b[rt][u] = (TWO && tv) ? b_row[rt][tq + u] : 0.0;
-      }
+        for (int u = 0; u < 4; ++u) {
+          a[rt][u] = a_row[rt][tq + u];
+          b[rt][u] = TWO ? b_row[rt][tq + u] : 0.0;
+        }
+    } else {
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const bool tv = rv[rt] && tq + u < m;
+          a[rt][u] = tv ? a_row[rt][tq + u] : 0.0;
+          b[rt][u] = (TWO && tv) ? b_row[rt][tq + u] : 0.0;
+        }
+    }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int t = tq + u;
@@ -470,10 +486,10 @@ void Linalg::gemm_tn_host(int64_t n, int m, int b, const double* A, int lda, con
 namespace {
 
 #ifndef PSOCI_NN_RT1
-#define PSOCI_NN_RT1 4
+#define PSOCI_NN_RT1 2
 #endif
 #ifndef PSOCI_NN_RT2
-#define PSOCI_NN_RT2 2
+#define PSOCI_NN_RT2 1
 #endif
 constexpr int kNnRt1 = PSOCI_NN_RT1, kNnRt2 = PSOCI_NN_RT2;
 
