@@ -67,6 +67,9 @@ def parse():
                     help="--sigma-only: block width to time (default: the roots)")
     ap.add_argument("--sigma-only", action="store_true",
                     help="time sigma launches only (no Davidson); for profiling runs")
+    ap.add_argument("--subspace-products", action="store_true",
+                    help="time the Davidson subspace products (psoci_project, psoci_combine) over the workload's "
+                         "SEF count at basis widths 16..128 and exit")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain 1 GiB device copy (reference reading for the FETCH/WRITE counters)")
     return ap.parse_args()
@@ -212,6 +215,36 @@ def cpu_direct(args, cfg, ints, k, sigma_calls, budget_s=15.0):
                        f"host {cpu_model()}")}
 
 
+def subspace_products(api, inputs, torch, n, workload):
+    """Wall time of the Davidson subspace products over n rows of a basis of
+    128 FP64 columns (row stride 128, as the solve's V and HV): the
+    projection A^T B (psoci_project, B 16 columns) and the update A Y
+    (psoci_combine, 16 output columns) at basis widths m = 16..128, best of 3
+    synchronised calls; bytes = n (m + 16) 8."""
+    cfg6, ints6, _ = inputs.workload("fci6")
+    H = api.Hamiltonian(cfg6, ints6)  # a context; the products take any n
+    V = torch.randn(n, 128, dtype=torch.float64, device="cuda")
+    W = torch.randn(n, 128, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(7)
+    out = {"workload": workload, "n": n, "ld": 128, "project": [], "combine": []}
+    for m in (16, 32, 48, 64, 80, 96, 112, 128):
+        best = min(_timed(lambda: H.project(V[:, :m], W[:, 64:80])) for _ in range(3))
+        out["project"].append({"m": m, "b": 16, "ms": round(best * 1e3, 4),
+                               "gbs": round(n * (m + 16) * 8 / best / 1e9, 1)})
+        Y = rng.random((m, 16))
+        best = min(_timed(lambda: H.combine(V[:, :m], Y, W[:, 0:16])) for _ in range(3))
+        out["combine"].append({"m": m, "k": 16, "ms": round(best * 1e3, 4),
+                               "gbs": round(n * (m + 16) * 8 / best / 1e9, 1)})
+    print(json.dumps(out))
+
+
+def _timed(fn):
+    t = time.perf_counter()
+    fn()
+    return time.perf_counter() - t
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -235,6 +268,9 @@ def main():
 
     cfg, ints, _ = inputs.workload(args.workload)
     direct = inputs.WORKLOADS[args.workload]["kind"] != "fci"
+    if args.subspace_products:
+        subspace_products(api, inputs, torch, int(cfg.nsef().sum()), args.workload)
+        return
     if args.steps is None:
         args.steps = 3 if direct else 5
     if args.warmup is None:
