@@ -225,17 +225,20 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
       for (int j = 0; j < cnt; ++j) out[(size_t)i * cnt + j] = Y[(size_t)i * mm + first + j];
   };
   // collapse V, HV onto Ritz vectors 0..nc-1 of the current solve (Y, w)
-  auto collapse = [&](int nc) {
+  // (with_hv false: HV is left as it is, for a caller that recomputes it)
+  auto collapse = [&](int nc, bool with_hv = true) {
     std::vector<double> Yc;
     cols_of(m, 0, nc, Yc);
     if (nc <= 128) {
       L.gemm_nn_inplace(n, m, nc, Yc.data(), V.p, mmax);
-      L.gemm_nn_inplace(n, m, nc, Yc.data(), HV.p, mmax);
+      if (with_hv) L.gemm_nn_inplace(n, m, nc, Yc.data(), HV.p, mmax);
     } else {
       L.gemm_nn(n, m, nc, V.p, mmax, Yc.data(), 0.0, T.p, nc);
       L.copy_cols(n, nc, T.p, nc, V.p, mmax);
-      L.gemm_nn(n, m, nc, HV.p, mmax, Yc.data(), 0.0, T.p, nc);
-      L.copy_cols(n, nc, T.p, nc, HV.p, mmax);
+      if (with_hv) {
+        L.gemm_nn(n, m, nc, HV.p, mmax, Yc.data(), 0.0, T.p, nc);
+        L.copy_cols(n, nc, T.p, nc, HV.p, mmax);
+      }
     }
     std::fill(A.begin(), A.end(), 0.0);
     for (int i = 0; i < nc; ++i) A[(size_t)i * mmax + i] = w[i];
@@ -323,7 +326,7 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
       // with the wanted roots (its launches cost more per column)
       k = kw;
       kc = std::min(2 * k, mmax - k > k ? mmax - k : k);
-      collapse(k);
+      collapse(k, false);  // their sigma is recomputed below
       phase32 = false;
       S.switch_iter = it;
       // the FP64 phase starts from an orthonormal basis to FP64 precision
@@ -389,16 +392,15 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     int nb = nsel;
     {
       // block Gram-Schmidt against V[:, 0:m) and symmetric orthonormalisation
-      // of the block, twice.  In the FP32 phase (whose sigma carries 1e-7
-      // relative errors anyway) the second pass runs only when the first
-      // removed more than half of a column's squared norm (the
-      // Daniel-Gragg-Kaufman-Stewart criterion) or left the block's columns
-      // nearly dependent (the smallest eigenvalue of their correlation
-      // matrix below 1e-2); the FP64 phase starts from a re-orthonormalised
-      // basis (below, at the switch) and always takes both passes
+      // of the block.  The second pass runs only when the first removed more
+      // than half of a column's squared norm (the Daniel-Gragg-Kaufman-Stewart
+      // criterion) or left the block's columns nearly dependent (the smallest
+      // eigenvalue of their correlation matrix below 1e-2); otherwise one
+      // pass keeps the block orthogonal to the basis to a few units of
+      // rounding of the arithmetic it runs in
       std::vector<double> G0((size_t)nb * nb), G1((size_t)nb * nb);
       bool again = m > 0, fused = false;
-      if (phase32 && m > 0) {
+      if (m > 0) {
         // one projection gives P = V^T x (rows 0..m) and the block's Gram
         // matrix G0 (rows m..m+nb)
         std::vector<double> Q((size_t)(m + nb) * nb);
