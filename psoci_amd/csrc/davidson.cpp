@@ -19,14 +19,17 @@
 // Mixed precision: sigma runs on the FP32 copy of H with FP32 vectors and
 // FP32 accumulation until every residual norm is below switch_tol; then the
 // basis collapses to the current Ritz vectors, their sigma is recomputed in
-// FP64, and FP64 sigma continues to convergence.  Subspace products,
-// orthogonalisation, residuals and Rayleigh quotients are FP64 throughout.
+// FP64, and FP64 sigma continues to convergence.  The FP32 phase keeps its
+// basis V and HV in FP32 (the sigma it feeds is FP32 anyway); subspace
+// products, orthogonalisation, residuals and Rayleigh quotients are FP64
+// arithmetic throughout.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <numeric>
+#include <type_traits>
 
 #include "ctx.hpp"
 #include "linalg.hpp"
@@ -74,14 +77,20 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
 
   Linalg L;
   L.stream = c.stream;
+  // the basis V and its sigma HV: FP64 for the FP64 phase, FP32 (V32,
+  // HV32) for the FP32 phase; a mixed solve holds both
   DevBuf<double> V, HV, R, T;
-  V.alloc((size_t)n * mmax);
-  HV.alloc((size_t)n * mmax);
+  DevBuf<float> V32, HV32, X32, Y32;
+  const bool want32 = p.prec != PSOCI_FP64;
+  if (p.prec != PSOCI_FP32) {
+    V.alloc((size_t)n * mmax);
+    HV.alloc((size_t)n * mmax);
+  }
   R.alloc((size_t)n * k0);
   T.alloc((size_t)n * std::max(kc, k0));
-  DevBuf<float> X32, Y32;
-  const bool want32 = p.prec != PSOCI_FP64;
   if (want32) {
+    V32.alloc((size_t)n * mmax);
+    HV32.alloc((size_t)n * mmax);
     X32.alloc((size_t)n * k0);
     Y32.alloc((size_t)n * k0);
   }
@@ -100,15 +109,21 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     // final order of the few candidates on the host
     L.lowest(n, k, c.d_diag.p, idx.data());
   }
+  bool phase32 = (p.prec == PSOCI_FP32 || p.prec == PSOCI_MIXED);
+  // f(V, HV) on the basis of the current phase
+  auto on_basis = [&](auto&& f) {
+    if (phase32) f(V32.p, HV32.p);
+    else f(V.p, HV.p);
+  };
   if (p.guess == 2) {
     PSOCI_CHECK(evecs, PSOCI_ERR_ARG, "guess 2 needs the starting vectors in evecs");
     R.upload(evecs, (size_t)n * k, c.stream);
-    L.copy_cols(n, k, R.p, k, V.p, mmax);
+    if (phase32) L.to_f32(n, k, R.p, k, V32.p, mmax);
+    else L.copy_cols(n, k, R.p, k, V.p, mmax);
   } else {
-    L.unit_cols(n, k, idx.data(), V.p, mmax);
+    on_basis([&](auto* Vb, auto*) { L.unit_cols(n, k, idx.data(), Vb, mmax); });
   }
 
-  bool phase32 = (p.prec == PSOCI_FP32 || p.prec == PSOCI_MIXED);
   const double switch_tol = p.switch_tol > 0 ? p.switch_tol : 1e-4;
   // The device work of one step -- sigma of the new block V[:, c0:c0+nb)
   // into HV and its projection V^T HV[:, c0:c0+nb) into L.small -- has no
@@ -139,19 +154,19 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
   }
   auto device_step = [&](int c0, int nb) {
     if (phase32 && p.sigma_mode != PSOCI_SIGMA_STORED && direct_engine(c) == 2) {
-      // the string-driven sigma rounds and widens on the fly
-      sigma_ras_f32_io64(c, nb, V.p + c0, mmax, HV.p + c0, mmax);
+      // the string-driven sigma reads and writes the basis columns in place
+      sigma_ras(c, PSOCI_FP32, nb, V32.p + c0, mmax, HV32.p + c0, mmax);
     } else if (phase32) {
-      L.to_f32(n, nb, V.p + c0, mmax, X32.p, nb);
+      L.copy_cols(n, nb, V32.p + c0, mmax, X32.p, nb);
       if (p.sigma_mode == PSOCI_SIGMA_STORED) sigma_stored(c, PSOCI_FP32, nb, X32.p, nb, Y32.p, nb);
       else sigma_direct(c, PSOCI_FP32, nb, X32.p, nb, Y32.p, nb);
-      L.to_f64(n, nb, Y32.p, nb, HV.p + c0, mmax);
+      L.copy_cols(n, nb, Y32.p, nb, HV32.p + c0, mmax);
     } else {
       if (p.sigma_mode == PSOCI_SIGMA_STORED) sigma_stored(c, PSOCI_FP64, nb, V.p + c0, mmax, HV.p + c0, mmax);
       else sigma_direct(c, PSOCI_FP64, nb, V.p + c0, mmax, HV.p + c0, mmax);
     }
     L.small.reserve((size_t)(c0 + nb) * nb);
-    L.gemm_tn(n, c0 + nb, nb, V.p, mmax, HV.p + c0, mmax, L.small.p);
+    on_basis([&](auto* Vb, auto* HVb) { L.gemm_tn(n, c0 + nb, nb, Vb, mmax, HVb + c0, mmax, L.small.p); });
   };
   auto run_step = [&](int c0, int nb) {
     if (!graphs) {
@@ -225,21 +240,27 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
       for (int j = 0; j < cnt; ++j) out[(size_t)i * cnt + j] = Y[(size_t)i * mm + first + j];
   };
   // collapse V, HV onto Ritz vectors 0..nc-1 of the current solve (Y, w)
-  // (with_hv false: HV is left as it is, for a caller that recomputes it)
-  auto collapse = [&](int nc, bool with_hv = true) {
+  auto collapse = [&](int nc) {
     std::vector<double> Yc;
     cols_of(m, 0, nc, Yc);
-    if (nc <= 128) {
-      L.gemm_nn_inplace(n, m, nc, Yc.data(), V.p, mmax);
-      if (with_hv) L.gemm_nn_inplace(n, m, nc, Yc.data(), HV.p, mmax);
-    } else {
-      L.gemm_nn(n, m, nc, V.p, mmax, Yc.data(), 0.0, T.p, nc);
-      L.copy_cols(n, nc, T.p, nc, V.p, mmax);
-      if (with_hv) {
-        L.gemm_nn(n, m, nc, HV.p, mmax, Yc.data(), 0.0, T.p, nc);
-        L.copy_cols(n, nc, T.p, nc, HV.p, mmax);
+    on_basis([&](auto* Vb, auto* HVb) {
+      if (nc <= 128) {
+        L.gemm_nn_inplace(n, m, nc, Yc.data(), Vb, mmax);
+        L.gemm_nn_inplace(n, m, nc, Yc.data(), HVb, mmax);
+      } else {
+        // via T (FP64): Vb = T rounded to the basis precision
+        auto back = [&](auto* dst) {
+          if constexpr (std::is_same_v<std::remove_pointer_t<decltype(dst)>, float>)
+            L.to_f32(n, nc, T.p, nc, dst, mmax);
+          else
+            L.copy_cols(n, nc, T.p, nc, dst, mmax);
+        };
+        L.gemm_nn(n, m, nc, Vb, mmax, Yc.data(), 0.0, T.p, nc);
+        back(Vb);
+        L.gemm_nn(n, m, nc, HVb, mmax, Yc.data(), 0.0, T.p, nc);
+        back(HVb);
       }
-    }
+    });
     std::fill(A.begin(), A.end(), 0.0);
     for (int i = 0; i < nc; ++i) A[(size_t)i * mmax + i] = w[i];
     m = nc;
@@ -268,23 +289,21 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
   // symmetric orthonormalisation of columns [c0, c0+nb) of V in place (via T)
   auto sym_orth = [&](int c0, int nb) -> int {
     std::vector<double> Gb((size_t)nb * nb), M;
-    L.gemm_tn_host(n, nb, nb, V.p + c0, mmax, V.p + c0, mmax, Gb.data());
+    on_basis([&](auto* Vb, auto*) { L.gemm_tn_host(n, nb, nb, Vb + c0, mmax, Vb + c0, mmax, Gb.data()); });
     const int nk = orth_matrix(nb, Gb.data(), M);
     if (nk == 0) return 0;
-    if (nk <= 128) {
-      L.gemm_nn_inplace(n, nb, nk, M.data(), V.p + c0, mmax);
-    } else {
-      L.copy_cols(n, nb, V.p + c0, mmax, T.p, nb);
-      L.gemm_nn(n, nb, nk, T.p, nb, M.data(), 0.0, V.p + c0, mmax);
-    }
+    // nk <= nb <= k <= 128
+    on_basis([&](auto* Vb, auto*) { L.gemm_nn_inplace(n, nb, nk, M.data(), Vb + c0, mmax); });
     return nk;
   };
   auto project_out = [&](int c0, int nb) {
     if (c0 == 0) return;
     P.resize((size_t)c0 * nb);
-    L.gemm_tn_host(n, c0, nb, V.p, mmax, V.p + c0, mmax, P.data());
-    for (auto& x : P) x = -x;
-    L.gemm_nn(n, c0, nb, V.p, mmax, P.data(), 1.0, V.p + c0, mmax);
+    on_basis([&](auto* Vb, auto*) {
+      L.gemm_tn_host(n, c0, nb, Vb, mmax, Vb + c0, mmax, P.data());
+      for (auto& x : P) x = -x;
+      L.gemm_nn(n, c0, nb, Vb, mmax, P.data(), 1.0, Vb + c0, mmax);
+    });
   };
 
   if (p.guess == 2) {
@@ -308,7 +327,9 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     // residuals R = HV Yk - V Yk diag(w); with Olsen's correction also the
     // Ritz vectors X = V Yk (into T, free until the next collapse) in the
     // same pass over V and HV
-    L.gemm_nn2(n, m, k, HV.p, mmax, Yk.data(), V.p, mmax, Yk2.data(), R.p, k, olsen ? T.p : nullptr, k);
+    on_basis([&](auto* Vb, auto* HVb) {
+      L.gemm_nn2(n, m, k, HVb, mmax, Yk.data(), Vb, mmax, Yk2.data(), R.p, k, olsen ? T.p : nullptr, k);
+    });
     G.resize((size_t)k * k);
     L.gemm_tn_host(n, k, k, R.p, k, R.p, k, G.data());
     double maxr = 0.0;
@@ -324,9 +345,18 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
       // switch: collapse to the Ritz vectors and recompute their sigma in FP64
       // guard roots ride in the FP32 phase only: the FP64 phase continues
       // with the wanted roots (its launches cost more per column)
+      // the Ritz vectors V32 Y into the FP64 basis; their sigma is
+      // recomputed below
+      {
+        std::vector<double> Yc;
+        cols_of(m, 0, kw, Yc);
+        L.gemm_nn(n, m, kw, V32.p, mmax, Yc.data(), 0.0, V.p, mmax);
+      }
       k = kw;
       kc = std::min(2 * k, mmax - k > k ? mmax - k : k);
-      collapse(k, false);  // their sigma is recomputed below
+      std::fill(A.begin(), A.end(), 0.0);
+      for (int i = 0; i < k; ++i) A[(size_t)i * mmax + i] = w[i];
+      m = k;
       phase32 = false;
       S.switch_iter = it;
       // the FP64 phase starts from an orthonormal basis to FP64 precision
@@ -385,9 +415,11 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     if (olsen) {
       std::vector<double> Ssel((size_t)k * nsel, 0.0);
       for (int j = 0; j < nsel; ++j) Ssel[(size_t)sel[j] * nsel + j] = 1.0;
-      L.gemm_nn(n, k, nsel, R.p, k, Ssel.data(), 0.0, V.p + m, mmax);
+      on_basis([&](auto* Vb, auto*) { L.gemm_nn(n, k, nsel, R.p, k, Ssel.data(), 0.0, Vb + m, mmax); });
     } else {
-      L.precondition_select(n, k, c.d_diag.p, w.data(), 1.0e-3, R.p, k, sel.data(), nsel, V.p + m, mmax);
+      on_basis([&](auto* Vb, auto*) {
+        L.precondition_select(n, k, c.d_diag.p, w.data(), 1.0e-3, R.p, k, sel.data(), nsel, Vb + m, mmax);
+      });
     }
     int nb = nsel;
     {
@@ -404,7 +436,7 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
         // one projection gives P = V^T x (rows 0..m) and the block's Gram
         // matrix G0 (rows m..m+nb)
         std::vector<double> Q((size_t)(m + nb) * nb);
-        L.gemm_tn_host(n, m + nb, nb, V.p, mmax, V.p + m, mmax, Q.data());
+        on_basis([&](auto* Vb, auto*) { L.gemm_tn_host(n, m + nb, nb, Vb, mmax, Vb + m, mmax, Q.data()); });
         P.assign(Q.begin(), Q.begin() + (size_t)m * nb);
         std::copy(Q.begin() + (size_t)m * nb, Q.end(), G0.begin());
         // V[:, 0:m) is orthonormal, so after x -= V (V^T x) the block's Gram
@@ -444,7 +476,7 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
               }
             for (int i = 0; i < nb; ++i)
               for (int j = 0; j < nk; ++j) Yf[(size_t)(m + i) * nk + j] = M[(size_t)i * nk + j];
-            L.gemm_nn_rows(n, m + nb, nk, V.p, mmax, Yf.data(), V.p + m, mmax);
+            on_basis([&](auto* Vb, auto*) { L.gemm_nn_rows(n, m + nb, nk, Vb, mmax, Yf.data(), Vb + m, mmax); });
           }
           nb = nk;
           fused = true;
@@ -452,7 +484,7 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
           // the first pass's update from the P at hand, then the second pass
           std::vector<double> Pn(P.size());
           for (size_t e = 0; e < P.size(); ++e) Pn[e] = -P[e];
-          L.gemm_nn(n, m, nb, V.p, mmax, Pn.data(), 1.0, V.p + m, mmax);
+          on_basis([&](auto* Vb, auto*) { L.gemm_nn(n, m, nb, Vb, mmax, Pn.data(), 1.0, Vb + m, mmax); });
         }
       } else {
         project_out(m, nb);
@@ -476,7 +508,7 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
   for (int j = 0; j < kw; ++j) evals[j] = w[j];
   if (evecs) {
     cols_of(m, 0, kw, Yk);
-    L.gemm_nn(n, m, kw, V.p, mmax, Yk.data(), 0.0, R.p, kw);
+    on_basis([&](auto* Vb, auto*) { L.gemm_nn(n, m, kw, Vb, mmax, Yk.data(), 0.0, R.p, kw); });
     R.download(evecs, (size_t)n * kw, c.stream);
     c.sync();
   }

@@ -20,67 +20,105 @@ namespace {
 constexpr int kWave = 64;
 constexpr size_t kMaxLds = 48 * 1024;  // dynamic LDS per block used without opt-in
 typedef double double4_t __attribute__((ext_vector_type(4)));
-#ifndef PSOCI_TN_U
-#define PSOCI_TN_U 1
-#endif
 #ifndef PSOCI_NN_FULL
 #define PSOCI_NN_FULL 1
 #endif
 constexpr bool kNnFull = PSOCI_NN_FULL;  // unpredicated loads for whole chunks of the basis update
-constexpr int kTnU = PSOCI_TN_U;  // 16-row steps per iteration of the projection loop
-#ifndef PSOCI_TN_ROUNDS
-#define PSOCI_TN_ROUNDS 1
-#endif
-constexpr int kTnRounds = PSOCI_TN_ROUNDS;  // rounds of resident workgroups per projection
+// Columns of A per lane in the projection: 2 for FP32 columns, so that a
+// wavefront reads whole 128-byte row segments (32 floats, two output tiles)
+template <typename TA>
+constexpr int tn_cols() {
+  return sizeof(TA) == 4 ? 2 : 1;
+}
 
 // C_partial[chunk][m][b] = sum over rows of chunk of A[row][a0 + i] * B[row][b0 + j]
-// for one 16x16 output tile per wavefront, on v_mfma_f64_16x16x4f64.
-// Lane layout of the instruction (measured on gfx950): A operand lane l =
-// (row k = l/16, col i = l%16), B operand lane l = (row k = l/16,
-// col j = l%16), accumulator register t of lane l = (i = 4*t + l/16, j = l%16).
-__global__ __launch_bounds__(256) void k_gemm_tn_mfma(int64_t n, int m, int b, const double* __restrict__ A, int lda,
-                                                      const double* __restrict__ B, int ldb, int64_t rows_per_chunk,
+// for CP 16x16 output tiles per wavefront (CP = tn_cols<TA>()), on
+// v_mfma_f64_16x16x4f64.  Lane layout of the instruction (measured on
+// gfx950): A operand lane l = (row k = l/16, col i = l%16), B operand lane
+// l = (row k = l/16, col j = l%16), accumulator register t of lane l =
+// (i = 4*t + l/16, j = l%16).  With CP = 2 lane i carries A columns
+// 32 tm + 2i and 32 tm + 2i + 1, one per tile.  TA: the element type of A
+// and B (float columns are widened on load).
+template <typename TA, int CP = tn_cols<TA>()>
+__global__ __launch_bounds__(256) void k_gemm_tn_mfma(int64_t n, int m, int b, const TA* __restrict__ A, int lda,
+                                                      const TA* __restrict__ B, int ldb, int64_t rows_per_chunk,
                                                       double* __restrict__ partial) {
+  constexpr int NCH = 4 / CP;  // accumulation chains per tile
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const int tiles_m = (m + 15) / 16, tiles_b = (b + 15) / 16;
+  const int tiles_m = (m + 16 * CP - 1) / (16 * CP), tiles_b = (b + 15) / 16;
   const int tile = blockIdx.x * (blockDim.x / kWave) + wave;
   if (tile >= tiles_m * tiles_b) return;
   const int tm = tile / tiles_b, tb = tile - tm * tiles_b;
   const int64_t chunk = blockIdx.y;
   const int64_t r0 = chunk * rows_per_chunk;
   const int64_t r1 = (r0 + rows_per_chunk < n) ? r0 + rows_per_chunk : n;
-  const int ci = tm * 16 + (lane & 15);
+  const int ci = tm * 16 * CP + CP * (lane & 15);
   const int cj = tb * 16 + (lane & 15);
   const int kk = lane >> 4;
-  const bool vi = ci < m, vj = cj < b;
-  // four independent accumulation chains, 4 * TN_U rows per chain and step:
-  // the loads of a step are all issued before its MFMAs
-  double4_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0, acc2 = acc0, acc3 = acc0;
-  for (int64_t r = r0; r < r1; r += 16 * kTnU) {
-    double a[4 * kTnU], bb[4 * kTnU];
+  const bool vj = cj < b;
+  // 4 / CP independent accumulation chains per tile, 16 rows per step: the
+  // loads of a step are all issued before its MFMAs
+  double4_t acc[CP][NCH];
 #pragma unroll
-    for (int u = 0; u < 4 * kTnU; ++u) {
+  for (int c = 0; c < CP; ++c)
+#pragma unroll
+    for (int h = 0; h < NCH; ++h) acc[c][h] = double4_t{0.0, 0.0, 0.0, 0.0};
+  // raw loads of one 16-row step; the next step's loads are issued before
+  // the current step's MFMAs (software pipelining: each wavefront keeps a
+  // step of loads in flight while its MFMAs run)
+  TA ra[CP][4], rb[4];
+  auto load = [&](int64_t r) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
       const int64_t row = r + 4 * u + kk;
       const bool in = row < r1;
-      a[u] = (in && vi) ? A[row * lda + ci] : 0.0;
-      bb[u] = (in && vj) ? B[row * ldb + cj] : 0.0;
+      if constexpr (CP == 2) {
+        // the lane's two columns in one 8-byte load (dword alignment is
+        // enough for global loads)
+        typedef float f2a4 __attribute__((ext_vector_type(2), aligned(4)));
+        if (in && ci + 1 < m) {
+          const f2a4 v = *reinterpret_cast<const f2a4*>(A + row * lda + ci);
+          ra[0][u] = v.x;
+          ra[1][u] = v.y;
+        } else {
+          ra[0][u] = (in && ci < m) ? A[row * lda + ci] : TA(0);
+          ra[1][u] = TA(0);
+        }
+      } else {
+        ra[0][u] = (in && ci < m) ? A[row * lda + ci] : TA(0);
+      }
+      rb[u] = (in && vj) ? B[row * ldb + cj] : TA(0);
     }
+  };
+  if (r0 < r1) load(r0);
+  for (int64_t r = r0; r < r1; r += 16) {
+    double a[CP][4], bb[4];
 #pragma unroll
-    for (int u = 0; u < kTnU; ++u) {
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[4 * u], bb[4 * u], acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[4 * u + 1], bb[4 * u + 1], acc1, 0, 0, 0);
-      acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[4 * u + 2], bb[4 * u + 2], acc2, 0, 0, 0);
-      acc3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[4 * u + 3], bb[4 * u + 3], acc3, 0, 0, 0);
+    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+      for (int c = 0; c < CP; ++c) a[c][u] = (double)ra[c][u];
+      bb[u] = (double)rb[u];
     }
+    if (r + 16 < r1) load(r + 16);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < CP; ++c)
+        acc[c][u % NCH] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c][u], bb[u], acc[c][u % NCH], 0, 0, 0);
   }
-  const double4_t acc = (acc0 + acc1) + (acc2 + acc3);
   double* out = partial + chunk * (int64_t)m * b;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int i = tm * 16 + 4 * t + (lane >> 4);
-    const int j = tb * 16 + (lane & 15);
-    if (i < m && j < b) out[(int64_t)i * b + j] = acc[t];
+  for (int c = 0; c < CP; ++c) {
+    double4_t sum = acc[c][0];
+#pragma unroll
+    for (int h = 1; h < NCH; ++h) sum += acc[c][h];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int i = tm * 16 * CP + CP * (4 * t + (lane >> 4)) + c;
+      const int j = tb * 16 + (lane & 15);
+      if (i < m && j < b) out[(int64_t)i * b + j] = sum[t];
+    }
   }
 }
 
@@ -115,27 +153,30 @@ __global__ __launch_bounds__(256) void k_reduce_chunks(int64_t nchunks, int mb, 
 // the product (k <= 128: one wavefront reads every input value of its rows
 // before it writes any output of them).  For the same reason A may hold the
 // same rows as Out at other or overlapping columns when k <= 128.
-template <int NCT, bool TWO, bool XOUT, bool INPLACE = false, int RT = 1>
-__global__ __launch_bounds__(256) void k_gemm_nn_mfma(int64_t n, int m, int k, const double* A,
-                                                      int lda, const double* __restrict__ B, int ldb,
+// TA: the element type of A and B, TO of Out (float columns are widened on
+// load and rounded on store; the arithmetic is FP64).
+template <int NCT, bool TWO, bool XOUT, bool INPLACE, int RT, typename TA, typename TO>
+__global__ __launch_bounds__(256) void k_gemm_nn_mfma(int64_t n, int m, int k, const TA* A,
+                                                      int lda, const TA* __restrict__ B, int ldb,
                                                       const double* __restrict__ Y1, const double* __restrict__ Y2,
-                                                      double alpha, double* Out, int ldo, double* __restrict__ Out2,
+                                                      double alpha, TO* Out, int ldo, double* __restrict__ Out2,
                                                       int ldo2) {
+  static_assert(!INPLACE || sizeof(TA) == sizeof(TO), "in place: one element type");
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int64_t r0 = ((int64_t)blockIdx.x * (blockDim.x / kWave) + wave) * 16 * RT;
   if (r0 >= n) return;  // uniform per wavefront
   const int cb = blockIdx.y * 128;
   const int i = lane & 15, q = lane >> 4;
-  const double* a_row[RT];
-  const double* b_row[RT];
+  const TA* a_row[RT];
+  const TA* b_row[RT];
   bool rv[RT];
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt) {
     const int64_t row = r0 + 16 * rt + i;
     rv[rt] = row < n;
     const int64_t rs = rv[rt] ? row : 0;
-    a_row[rt] = INPLACE ? Out + rs * (int64_t)ldo : A + rs * (int64_t)lda;
+    a_row[rt] = INPLACE ? reinterpret_cast<const TA*>(Out) + rs * (int64_t)ldo : A + rs * (int64_t)lda;
     b_row[rt] = TWO ? B + rs * (int64_t)ldb : nullptr;
   }
   double4_t acc[RT][NCT], acc2[RT][NCT];
@@ -156,8 +197,8 @@ __global__ __launch_bounds__(256) void k_gemm_nn_mfma(int64_t n, int m, int k, c
       for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          a[rt][u] = a_row[rt][tq + u];
-          b[rt][u] = TWO ? b_row[rt][tq + u] : 0.0;
+          a[rt][u] = (double)a_row[rt][tq + u];
+          b[rt][u] = TWO ? (double)b_row[rt][tq + u] : 0.0;
         }
     } else {
 #pragma unroll
@@ -165,8 +206,8 @@ __global__ __launch_bounds__(256) void k_gemm_nn_mfma(int64_t n, int m, int k, c
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const bool tv = rv[rt] && tq + u < m;
-          a[rt][u] = tv ? a_row[rt][tq + u] : 0.0;
-          b[rt][u] = (TWO && tv) ? b_row[rt][tq + u] : 0.0;
+          a[rt][u] = tv ? (double)a_row[rt][tq + u] : 0.0;
+          b[rt][u] = (TWO && tv) ? (double)b_row[rt][tq + u] : 0.0;
         }
     }
 #pragma unroll
@@ -197,8 +238,8 @@ __global__ __launch_bounds__(256) void k_gemm_nn_mfma(int64_t n, int m, int k, c
       for (int j = 0; j < 4; ++j) {
         const int64_t rr = r0 + 16 * rt + 4 * j + q;
         if (rr >= n) continue;
-        double* o = Out + rr * ldo + c;
-        *o = (alpha == 0.0) ? acc[rt][ct][j] : __builtin_fma(alpha, *o, acc[rt][ct][j]);
+        TO* o = Out + rr * ldo + c;
+        *o = (TO)((alpha == 0.0) ? acc[rt][ct][j] : __builtin_fma(alpha, (double)*o, acc[rt][ct][j]));
         if (XOUT) Out2[rr * ldo2 + c] = acc2[rt][ct][j];
       }
     }
@@ -269,9 +310,10 @@ __global__ void k_olsen_apply(int64_t n, int k, const double* __restrict__ D, co
 // denominator exceeds dmin in magnitude, else R[row][sel[j]]: the diagonal
 // preconditioner applied to the selected columns on their way into the
 // basis (one pass; the same numbers as preconditioning R and then copying)
+template <typename TO>
 __global__ void k_precondition_select(int64_t n, int ns, const double* __restrict__ D,
                                       const double* __restrict__ theta_sel, const int* __restrict__ sel,
-                                      double dmin, const double* __restrict__ R, int ldr, double* __restrict__ Out,
+                                      double dmin, const double* __restrict__ R, int ldr, TO* __restrict__ Out,
                                       int ldo) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t row = g / ns;
@@ -279,7 +321,7 @@ __global__ void k_precondition_select(int64_t n, int ns, const double* __restric
   if (row >= n) return;
   const double r = R[row * ldr + sel[j]];
   const double den = D[row] - theta_sel[j];
-  Out[row * ldo + j] = fabs(den) > dmin ? r / den : r;
+  Out[row * ldo + j] = (TO)(fabs(den) > dmin ? r / den : r);
 }
 
 __global__ void k_precondition(int64_t n, int k, const double* __restrict__ D, const double* __restrict__ theta,
@@ -310,7 +352,8 @@ __global__ void k_copy_cols_f32_to_f64(int64_t n, int k, const float* __restrict
   B[row * ldb + c] = (double)A[row * lda + c];
 }
 
-__global__ void k_copy_cols(int64_t n, int k, const double* __restrict__ A, int lda, double* __restrict__ B, int ldb) {
+template <typename T>
+__global__ void k_copy_cols(int64_t n, int k, const T* __restrict__ A, int lda, T* __restrict__ B, int ldb) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t row = g / k;
   const int c = (int)(g - row * k);
@@ -318,12 +361,13 @@ __global__ void k_copy_cols(int64_t n, int k, const double* __restrict__ A, int 
   B[row * ldb + c] = A[row * lda + c];
 }
 
-__global__ void k_unit_cols(int64_t n, int k, const int64_t* __restrict__ idx, double* __restrict__ V, int ldv) {
+template <typename T>
+__global__ void k_unit_cols(int64_t n, int k, const int64_t* __restrict__ idx, T* __restrict__ V, int ldv) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t row = g / k;
   const int c = (int)(g - row * k);
   if (row >= n) return;
-  V[row * ldv + c] = (row == idx[c]) ? 1.0 : 0.0;
+  V[row * ldv + c] = (row == idx[c]) ? T(1) : T(0);
 }
 
 // The k lowest entries of D (k <= 128), ordered by (value, index), of each
@@ -407,7 +451,6 @@ inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per
 
 namespace {
 
-// about 4096 wavefronts in flight: chunks of a multiple of 16 rows
 // Wavefronts per workgroup of k_gemm_tn_mfma for a tile count: the tiles
 // spread evenly over the fewest workgroups of at most 4 wavefronts.
 int tn_waves(int tiles) {
@@ -417,29 +460,35 @@ int tn_waves(int tiles) {
 
 // Workgroups of k_gemm_tn_mfma with w wavefronts that the current device
 // holds at once (compute units x resident workgroups per unit), queried once
-// per device and width.
-int64_t tn_resident(int w) {
-  static int64_t cache[64][5] = {};
+// per device, width and columns per lane.
+int64_t tn_resident(int w, int cp) {
+  static int64_t cache[64][5][3] = {};
   int dev = 0;
   PSOCI_HIP(hipGetDevice(&dev));
   if (dev < 0 || dev >= 64) dev = 0;
-  if (cache[dev][w] == 0) {
+  if (cache[dev][w][cp] == 0) {
     int per_cu = 0, cus = 0;
-    PSOCI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gemm_tn_mfma, kWave * w, 0));
+    if (cp == 2)
+      PSOCI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gemm_tn_mfma<float>, kWave * w, 0));
+    else
+      PSOCI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gemm_tn_mfma<double>, kWave * w, 0));
     PSOCI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    cache[dev][w] = std::max<int64_t>(1, (int64_t)per_cu * cus);
+    cache[dev][w][cp] = std::max<int64_t>(1, (int64_t)per_cu * cus);
   }
-  return cache[dev][w];
+  return cache[dev][w][cp];
 }
+
+// Output tiles of the projection's wavefronts (cp columns of A per lane).
+int tn_tiles(int m, int b, int cp) { return ((m + 16 * cp - 1) / (16 * cp)) * ((b + 15) / 16); }
 
 // Row chunks of the projection: one chunk per group of workgroups covering
 // the output tiles, as many groups as the device holds at once (a single
 // round of workgroups, no partly filled second round).
-void tn_chunks(int64_t n, int m, int b, int64_t& rows_per_chunk, int64_t& nchunks) {
-  const int tiles = ((m + 15) / 16) * ((b + 15) / 16);
+void tn_chunks(int64_t n, int m, int b, int cp, int64_t& rows_per_chunk, int64_t& nchunks) {
+  const int tiles = tn_tiles(m, b, cp);
   const int w = tn_waves(tiles);
   const int64_t blocks_per_chunk = (tiles + w - 1) / w;
-  int64_t want = std::max<int64_t>(1, kTnRounds * tn_resident(w) / blocks_per_chunk);
+  int64_t want = std::max<int64_t>(1, tn_resident(w, cp) / blocks_per_chunk);
   want = std::min<int64_t>(want, 65535);
   rows_per_chunk = (n + want - 1) / want;
   rows_per_chunk = std::max<int64_t>(64, (rows_per_chunk + 15) / 16 * 16);
@@ -449,9 +498,11 @@ void tn_chunks(int64_t n, int m, int b, int64_t& rows_per_chunk, int64_t& nchunk
 }  // namespace
 
 size_t Linalg::tn_part_size(int64_t n, int m, int b) {
-  int64_t rpc, nch;
-  tn_chunks(n, m, b, rpc, nch);
-  return (size_t)nch * m * b;
+  // the larger of the FP64 and FP32 kernels' needs
+  int64_t rpc, nch1, nch2;
+  tn_chunks(n, m, b, 1, rpc, nch1);
+  tn_chunks(n, m, b, 2, rpc, nch2);
+  return (size_t)std::max(nch1, nch2) * m * b;
 }
 
 void Linalg::reserve_tn(int64_t n, int mmax, int bmax) {
@@ -461,22 +512,25 @@ void Linalg::reserve_tn(int64_t n, int mmax, int bmax) {
   part.reserve(need);
 }
 
-void Linalg::gemm_tn(int64_t n, int m, int b, const double* A, int lda, const double* B, int ldb, double* out_dev) {
+template <typename TA>
+void Linalg::gemm_tn(int64_t n, int m, int b, const TA* A, int lda, const TA* B, int ldb, double* out_dev) {
   if (m == 0 || b == 0) return;
+  constexpr int cp = tn_cols<TA>();
   int64_t rows_per_chunk, nchunks;
-  tn_chunks(n, m, b, rows_per_chunk, nchunks);
+  tn_chunks(n, m, b, cp, rows_per_chunk, nchunks);
   part.reserve((size_t)nchunks * m * b);
-  const int tiles = ((m + 15) / 16) * ((b + 15) / 16);
+  const int tiles = tn_tiles(m, b, cp);
   const int w = tn_waves(tiles);
   dim3 grid(nblk(tiles, w), (unsigned)nchunks);
-  hipLaunchKernelGGL(k_gemm_tn_mfma, grid, dim3(kWave * w), 0, stream, n, m, b, A, lda, B, ldb, rows_per_chunk,
+  hipLaunchKernelGGL(k_gemm_tn_mfma<TA>, grid, dim3(kWave * w), 0, stream, n, m, b, A, lda, B, ldb, rows_per_chunk,
                      part.p);
   PSOCI_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)(m * b)), dim3(256), 0, stream, nchunks, m * b, part.p, out_dev);
   PSOCI_HIP(hipGetLastError());
 }
 
-void Linalg::gemm_tn_host(int64_t n, int m, int b, const double* A, int lda, const double* B, int ldb, double* out_host) {
+template <typename TA>
+void Linalg::gemm_tn_host(int64_t n, int m, int b, const TA* A, int lda, const TA* B, int ldb, double* out_host) {
   small.reserve((size_t)m * b);
   gemm_tn(n, m, b, A, lda, B, ldb, small.p);
   PSOCI_HIP(hipMemcpyAsync(out_host, small.p, (size_t)m * b * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -493,14 +547,15 @@ namespace {
 #endif
 constexpr int kNnRt1 = PSOCI_NN_RT1, kNnRt2 = PSOCI_NN_RT2;
 
-template <bool TWO, bool XOUT, bool INPLACE = false>
-void launch_nn(hipStream_t st, int64_t n, int m, int k, const double* A, int lda, const double* B, int ldb,
-               const double* Y1, const double* Y2, double alpha, double* Out, int ldo, double* Out2, int ldo2) {
+template <bool TWO, bool XOUT, bool INPLACE, typename TA, typename TO>
+void launch_nn(hipStream_t st, int64_t n, int m, int k, const TA* A, int lda, const TA* B, int ldb,
+               const double* Y1, const double* Y2, double alpha, TO* Out, int ldo, double* Out2, int ldo2) {
   const int nct = std::min(8, (k + 15) / 16);
   // row tiles per wavefront: each Y value read feeds RT tiles of 16 rows
-#define PSOCI_NN(NCT, RT)                                                                                      \
-  hipLaunchKernelGGL((k_gemm_nn_mfma<NCT, TWO, XOUT, INPLACE, RT>), dim3(nblk(n, 64 * RT), (unsigned)((k + 127) / 128)), \
-                     dim3(256), 0, st, n, m, k, A, lda, B, ldb, Y1, Y2, alpha, Out, ldo, Out2, ldo2)
+#define PSOCI_NN(NCT, RT)                                                                            \
+  hipLaunchKernelGGL((k_gemm_nn_mfma<NCT, TWO, XOUT, INPLACE, RT, TA, TO>),                          \
+                     dim3(nblk(n, 64 * RT), (unsigned)((k + 127) / 128)), dim3(256), 0, st, n, m, k, A, lda, B, \
+                     ldb, Y1, Y2, alpha, Out, ldo, Out2, ldo2)
   if (nct <= 1) PSOCI_NN(1, kNnRt1);
   else if (nct <= 2) PSOCI_NN(2, kNnRt2);
   else if (nct <= 4) PSOCI_NN(4, 1);
@@ -511,33 +566,38 @@ void launch_nn(hipStream_t st, int64_t n, int m, int k, const double* A, int lda
 
 }  // namespace
 
-void Linalg::gemm_nn(int64_t n, int m, int k, const double* A, int lda, const double* Y1_host, double alpha, double* Out,
+template <typename TA, typename TO>
+void Linalg::gemm_nn(int64_t n, int m, int k, const TA* A, int lda, const double* Y1_host, double alpha, TO* Out,
                      int ldo) {
   if (k == 0 || n == 0) return;
   ysmall.reserve((size_t)m * k);
   PSOCI_HIP(hipMemcpyAsync(ysmall.p, Y1_host, (size_t)m * k * sizeof(double), hipMemcpyHostToDevice, stream));
-  launch_nn<false, false>(stream, n, m, k, A, lda, nullptr, 0, ysmall.p, nullptr, alpha, Out, ldo, nullptr, 0);
+  launch_nn<false, false, false, TA, TO>(stream, n, m, k, A, lda, nullptr, 0, ysmall.p, nullptr, alpha, Out, ldo,
+                                         nullptr, 0);
 }
 
-void Linalg::gemm_nn_rows(int64_t n, int m, int k, const double* A, int lda, const double* Y1_host, double* Out,
-                          int ldo) {
+template <typename T>
+void Linalg::gemm_nn_rows(int64_t n, int m, int k, const T* A, int lda, const double* Y1_host, T* Out, int ldo) {
   if (k == 0 || n == 0) return;
   PSOCI_CHECK(k <= 128, PSOCI_ERR_ARG, "row-aliased product: at most 128 output columns");
   ysmall.reserve((size_t)m * k);
   PSOCI_HIP(hipMemcpyAsync(ysmall.p, Y1_host, (size_t)m * k * sizeof(double), hipMemcpyHostToDevice, stream));
-  launch_nn<false, false>(stream, n, m, k, A, lda, nullptr, 0, ysmall.p, nullptr, 0.0, Out, ldo, nullptr, 0);
+  launch_nn<false, false, false, T, T>(stream, n, m, k, A, lda, nullptr, 0, ysmall.p, nullptr, 0.0, Out, ldo, nullptr,
+                                       0);
 }
 
-void Linalg::gemm_nn_inplace(int64_t n, int m, int k, const double* Y1_host, double* Out, int ldo) {
+template <typename T>
+void Linalg::gemm_nn_inplace(int64_t n, int m, int k, const double* Y1_host, T* Out, int ldo) {
   if (k == 0 || n == 0) return;
   PSOCI_CHECK(k <= 128 && k <= m, PSOCI_ERR_ARG, "in-place product: at most 128 and at most m output columns");
   ysmall.reserve((size_t)m * k);
   PSOCI_HIP(hipMemcpyAsync(ysmall.p, Y1_host, (size_t)m * k * sizeof(double), hipMemcpyHostToDevice, stream));
-  launch_nn<false, false, true>(stream, n, m, k, nullptr, 0, nullptr, 0, ysmall.p, nullptr, 0.0, Out, ldo, nullptr,
-                                0);
+  launch_nn<false, false, true, T, T>(stream, n, m, k, nullptr, 0, nullptr, 0, ysmall.p, nullptr, 0.0, Out, ldo,
+                                      nullptr, 0);
 }
 
-void Linalg::gemm_nn2(int64_t n, int m, int k, const double* A, int lda, const double* Y1_host, const double* B, int ldb,
+template <typename TA>
+void Linalg::gemm_nn2(int64_t n, int m, int k, const TA* A, int lda, const double* Y1_host, const TA* B, int ldb,
                       const double* Y2_host, double* Out, int ldo, double* X, int ldx) {
   if (k == 0 || n == 0) return;
   ysmall.reserve((size_t)2 * m * k);
@@ -547,10 +607,11 @@ void Linalg::gemm_nn2(int64_t n, int m, int k, const double* A, int lda, const d
   PSOCI_HIP(hipMemcpyAsync(ysmall.p + (size_t)m * k, yneg.data(), (size_t)m * k * sizeof(double),
                            hipMemcpyHostToDevice, stream));
   if (X)
-    launch_nn<true, true>(stream, n, m, k, A, lda, B, ldb, ysmall.p, ysmall.p + (size_t)m * k, 0.0, Out, ldo, X, ldx);
+    launch_nn<true, true, false, TA, double>(stream, n, m, k, A, lda, B, ldb, ysmall.p, ysmall.p + (size_t)m * k, 0.0,
+                                             Out, ldo, X, ldx);
   else
-    launch_nn<true, false>(stream, n, m, k, A, lda, B, ldb, ysmall.p, ysmall.p + (size_t)m * k, 0.0, Out, ldo,
-                           nullptr, 0);
+    launch_nn<true, false, false, TA, double>(stream, n, m, k, A, lda, B, ldb, ysmall.p, ysmall.p + (size_t)m * k,
+                                              0.0, Out, ldo, nullptr, 0);
   // yneg is read by the copy above: keep it until the stream has consumed it
   PSOCI_HIP(hipStreamSynchronize(stream));
 }
@@ -562,8 +623,9 @@ void Linalg::precondition(int64_t n, int k, const double* D, const double* theta
   PSOCI_HIP(hipGetLastError());
 }
 
+template <typename TO>
 void Linalg::precondition_select(int64_t n, int k, const double* D, const double* theta_host, double dmin,
-                                 const double* R, int ldr, const int* sel_host, int ns, double* Out, int ldo) {
+                                 const double* R, int ldr, const int* sel_host, int ns, TO* Out, int ldo) {
   if (ns == 0 || n == 0) return;
   std::vector<double> ts(ns);
   for (int j = 0; j < ns; ++j) ts[j] = theta_host[sel_host[j]];
@@ -572,7 +634,7 @@ void Linalg::precondition_select(int64_t n, int k, const double* D, const double
   isel.reserve(ns);
   PSOCI_HIP(hipMemcpyAsync(theta.p, ts.data(), ns * sizeof(double), hipMemcpyHostToDevice, stream));
   PSOCI_HIP(hipMemcpyAsync(isel.p, sel_host, ns * sizeof(int), hipMemcpyHostToDevice, stream));
-  hipLaunchKernelGGL(k_precondition_select, dim3(nblk(n * ns, 256)), dim3(256), 0, stream, n, ns, D, theta.p,
+  hipLaunchKernelGGL(k_precondition_select<TO>, dim3(nblk(n * ns, 256)), dim3(256), 0, stream, n, ns, D, theta.p,
                      isel.p, dmin, R, ldr, Out, ldo);
   PSOCI_HIP(hipGetLastError());
   // ts is read by the copy above: keep it until the stream has consumed it
@@ -608,8 +670,9 @@ void Linalg::to_f64(int64_t n, int k, const float* A, int lda, double* B, int ld
   PSOCI_HIP(hipGetLastError());
 }
 
-void Linalg::copy_cols(int64_t n, int k, const double* A, int lda, double* B, int ldb) {
-  hipLaunchKernelGGL(k_copy_cols, dim3(nblk(n * k, 256)), dim3(256), 0, stream, n, k, A, lda, B, ldb);
+template <typename T>
+void Linalg::copy_cols(int64_t n, int k, const T* A, int lda, T* B, int ldb) {
+  hipLaunchKernelGGL(k_copy_cols<T>, dim3(nblk(n * k, 256)), dim3(256), 0, stream, n, k, A, lda, B, ldb);
   PSOCI_HIP(hipGetLastError());
 }
 
@@ -636,12 +699,43 @@ void Linalg::lowest(int64_t n, int k, const double* D, int64_t* idx_host) {
   for (int r = 0; r < k; ++r) idx_host[r] = ci[ord[r]];
 }
 
-void Linalg::unit_cols(int64_t n, int k, const int64_t* idx_host, double* V, int ldv) {
+template <typename T>
+void Linalg::unit_cols(int64_t n, int k, const int64_t* idx_host, T* V, int ldv) {
   DevBuf<int64_t> idx;
   idx.upload(idx_host, k, stream);
-  hipLaunchKernelGGL(k_unit_cols, dim3(nblk(n * k, 256)), dim3(256), 0, stream, n, k, idx.p, V, ldv);
+  hipLaunchKernelGGL(k_unit_cols<T>, dim3(nblk(n * k, 256)), dim3(256), 0, stream, n, k, idx.p, V, ldv);
   PSOCI_HIP(hipGetLastError());
   PSOCI_HIP(hipStreamSynchronize(stream));
 }
+
+// the element types the block Davidson uses: FP64 bases, FP32 bases in the
+// FP32 phase of a mixed-precision solve
+template void Linalg::gemm_tn<double>(int64_t, int, int, const double*, int, const double*, int, double*);
+template void Linalg::gemm_tn<float>(int64_t, int, int, const float*, int, const float*, int, double*);
+template void Linalg::gemm_tn_host<double>(int64_t, int, int, const double*, int, const double*, int, double*);
+template void Linalg::gemm_tn_host<float>(int64_t, int, int, const float*, int, const float*, int, double*);
+template void Linalg::gemm_nn<double, double>(int64_t, int, int, const double*, int, const double*, double, double*,
+                                              int);
+template void Linalg::gemm_nn<float, float>(int64_t, int, int, const float*, int, const double*, double, float*, int);
+template void Linalg::gemm_nn<float, double>(int64_t, int, int, const float*, int, const double*, double, double*,
+                                             int);
+template void Linalg::gemm_nn<double, float>(int64_t, int, int, const double*, int, const double*, double, float*,
+                                             int);
+template void Linalg::gemm_nn_rows<double>(int64_t, int, int, const double*, int, const double*, double*, int);
+template void Linalg::gemm_nn_rows<float>(int64_t, int, int, const float*, int, const double*, float*, int);
+template void Linalg::gemm_nn_inplace<double>(int64_t, int, int, const double*, double*, int);
+template void Linalg::gemm_nn_inplace<float>(int64_t, int, int, const double*, float*, int);
+template void Linalg::gemm_nn2<double>(int64_t, int, int, const double*, int, const double*, const double*, int,
+                                       const double*, double*, int, double*, int);
+template void Linalg::gemm_nn2<float>(int64_t, int, int, const float*, int, const double*, const float*, int,
+                                      const double*, double*, int, double*, int);
+template void Linalg::precondition_select<double>(int64_t, int, const double*, const double*, double, const double*,
+                                                  int, const int*, int, double*, int);
+template void Linalg::precondition_select<float>(int64_t, int, const double*, const double*, double, const double*,
+                                                 int, const int*, int, float*, int);
+template void Linalg::copy_cols<double>(int64_t, int, const double*, int, double*, int);
+template void Linalg::copy_cols<float>(int64_t, int, const float*, int, float*, int);
+template void Linalg::unit_cols<double>(int64_t, int, const int64_t*, double*, int);
+template void Linalg::unit_cols<float>(int64_t, int, const int64_t*, float*, int);
 
 }  // namespace psoci
