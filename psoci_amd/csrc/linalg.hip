@@ -187,9 +187,12 @@ __global__ __launch_bounds__(256) void k_gemm_nn_mfma(int64_t n, int m, int k, c
       acc[rt][ct] = double4_t{0.0, 0.0, 0.0, 0.0};
       acc2[rt][ct] = acc[rt][ct];
     }
-  for (int t0 = 0; t0 < m; t0 += 16) {
+  // raw loads of one chunk of 16 values of t; the next chunk's loads are
+  // issued after this chunk's Y values and before its MFMAs (the in-order
+  // load counter then lets the MFMAs start while the next chunk is in flight)
+  TA ra[RT][4], rb[RT][4];
+  auto load = [&](int t0) {
     const int tq = t0 + 4 * q;
-    double a[RT][4], b[RT][4];
     if (kNnFull && t0 + 16 <= m) {
       // a whole chunk: unpredicated loads (rows past n read row 0; their
       // products land in output rows that are not written)
@@ -197,8 +200,8 @@ __global__ __launch_bounds__(256) void k_gemm_nn_mfma(int64_t n, int m, int k, c
       for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          a[rt][u] = (double)a_row[rt][tq + u];
-          b[rt][u] = TWO ? (double)b_row[rt][tq + u] : 0.0;
+          ra[rt][u] = a_row[rt][tq + u];
+          rb[rt][u] = TWO ? b_row[rt][tq + u] : TA(0);
         }
     } else {
 #pragma unroll
@@ -206,24 +209,41 @@ __global__ __launch_bounds__(256) void k_gemm_nn_mfma(int64_t n, int m, int k, c
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const bool tv = rv[rt] && tq + u < m;
-          a[rt][u] = tv ? (double)a_row[rt][tq + u] : 0.0;
-          b[rt][u] = (TWO && tv) ? (double)b_row[rt][tq + u] : 0.0;
+          ra[rt][u] = tv ? a_row[rt][tq + u] : TA(0);
+          rb[rt][u] = (TWO && tv) ? b_row[rt][tq + u] : TA(0);
         }
     }
+  };
+  if (m > 0) load(0);
+  for (int t0 = 0; t0 < m; t0 += 16) {
+    const int tq = t0 + 4 * q;
+    double a[RT][4], b[RT][4], y1[4][NCT], y2[4][NCT];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int t = tq + u;
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[rt][u] = (double)ra[rt][u];
+        b[rt][u] = (double)rb[rt][u];
+      }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
 #pragma unroll
       for (int ct = 0; ct < NCT; ++ct) {
-        const int c = cb + ct * 16 + i;
+        const int t = tq + u, c = cb + ct * 16 + i;
         const bool yv = t < m && c < k;
-        const double y1 = yv ? Y1[(int64_t)t * k + c] : 0.0;
-        const double y2 = (TWO && yv) ? Y2[(int64_t)t * k + c] : 0.0;
+        y1[u][ct] = yv ? Y1[(int64_t)t * k + c] : 0.0;
+        y2[u][ct] = (TWO && yv) ? Y2[(int64_t)t * k + c] : 0.0;
+      }
+    if (t0 + 16 < m) load(t0 + 16);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) {
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt) {
-          acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt][u], y1, acc[rt][ct], 0, 0, 0);
-          if (XOUT) acc2[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[rt][u], y1, acc2[rt][ct], 0, 0, 0);
-          if (TWO) acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[rt][u], y2, acc[rt][ct], 0, 0, 0);
+          acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt][u], y1[u][ct], acc[rt][ct], 0, 0, 0);
+          if (XOUT) acc2[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[rt][u], y1[u][ct], acc2[rt][ct], 0, 0, 0);
+          if (TWO) acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[rt][u], y2[u][ct], acc[rt][ct], 0, 0, 0);
         }
       }
     }
