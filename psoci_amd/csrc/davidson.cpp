@@ -424,15 +424,17 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     int nb = nsel;
     {
       // block Gram-Schmidt against V[:, 0:m) and symmetric orthonormalisation
-      // of the block.  The second pass runs only when the first removed more
-      // than half of a column's squared norm (the Daniel-Gragg-Kaufman-Stewart
-      // criterion) or left the block's columns nearly dependent (the smallest
-      // eigenvalue of their correlation matrix below 1e-2); otherwise one
-      // pass keeps the block orthogonal to the basis to a few units of
-      // rounding of the arithmetic it runs in
+      // of the block, twice.  In the FP32 phase (whose sigma carries 1e-7
+      // relative errors anyway) the second pass runs only when the first
+      // removed more than half of a column's squared norm (the
+      // Daniel-Gragg-Kaufman-Stewart criterion) or left the block's columns
+      // nearly dependent (the smallest eigenvalue of their correlation
+      // matrix below 1e-2).  The FP64 phase always takes both passes: with
+      // one pass under the same test, solves to residual 1e-8 on small
+      // spaces stalled where two passes converge
       std::vector<double> G0((size_t)nb * nb), G1((size_t)nb * nb);
       bool again = m > 0, fused = false;
-      if (m > 0) {
+      if (phase32 && m > 0) {
         // one projection gives P = V^T x (rows 0..m) and the block's Gram
         // matrix G0 (rows m..m+nb)
         std::vector<double> Q((size_t)(m + nb) * nb);
