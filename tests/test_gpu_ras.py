@@ -85,6 +85,35 @@ def test_ras_davidson_fci10(fci10_strings):
     np.testing.assert_allclose(c["evals"], a["evals"], atol=1e-8, rtol=0)
 
 
+@pytest.mark.parametrize("mode", [api.DIRECT, api.STORED])
+def test_davidson_fp32_basis_paths(fci10_strings, mode):
+    """The FP32 phase keeps its basis in FP32: an FP32-only solve (vectors
+    formed from the FP32 basis), mixed solves with caller vectors, Olsen's
+    correction, guard roots and a small subspace cap (collapses of the FP32
+    basis) all land on the FP64 roots."""
+    H = fci10_strings
+    ref = H.davidson(4, prec=api.FP64, mode=mode, tol=1e-8)
+    assert ref["converged"]
+    e64, v64 = ref["evals"], ref["evecs"]
+    f = H.davidson(4, prec=api.FP32, mode=mode, tol=1e-4)
+    assert f["converged"]
+    np.testing.assert_allclose(f["evals"], e64, atol=1e-5, rtol=0)
+    # the returned vectors (formed from the FP32 basis): their Rayleigh
+    # quotients under the FP64 stored H are the roots, their residuals small
+    V = f["evecs"]
+    HV = H.sigma(V, api.STORED, api.FP64)
+    rq = np.sum(V * HV, axis=0) / np.sum(V * V, axis=0)
+    np.testing.assert_allclose(rq, e64, atol=1e-5, rtol=0)
+    res = np.linalg.norm(HV - V * rq, axis=0) / np.linalg.norm(V, axis=0)
+    assert np.all(res < 1e-3), res
+    guess = v64 + 1e-3 * np.random.default_rng(5).standard_normal(v64.shape)
+    for kw in (dict(guess=2, guess_vectors=guess), dict(precond=1), dict(extra_roots=4),
+               dict(max_subspace=12)):
+        c = H.davidson(4, prec=api.MIXED, mode=mode, tol=1e-7, want_vectors=False, **kw)
+        assert c["converged"], kw
+        np.testing.assert_allclose(c["evals"], e64, atol=1e-9, rtol=0, err_msg=str(kw))
+
+
 @pytest.fixture(scope="module")
 def mrci8x24_pair():
     cfg, ints, k = inputs.workload("mrci8x24")
