@@ -179,10 +179,6 @@ int64_t direct_det_pairs(Ctx& c);
 // (false when the space does not qualify; c.ras_why says why), and the sigma
 bool build_ras(Ctx& c);
 void sigma_ras(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy);
-// the FP32 string-driven sigma of FP64 vectors (rounded to FP32 on the way
-// in, the FP32 result widened on the way out: the same numbers as an FP32
-// sigma of the rounded vectors, without the staging copies)
-void sigma_ras_f32_io64(Ctx& c, int k, const double* X, int ldx, double* Y, int ldy);
 // one timed sigma of k columns on internal buffers, averaged over reps: ms[5]
 // per launch group (scatter, products by kind, gather), flops[3] per kind
 void ras_timing(Ctx& c, int prec, int k, int reps, double* ms, double* flops);

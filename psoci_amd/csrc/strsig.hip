@@ -997,9 +997,4 @@ void sigma_ras(Ctx& c, int prec, int k, const void* X, int ldx, void* Y, int ldy
     ras_any_k<double>(c, k, static_cast<const double*>(X), ldx, static_cast<double*>(Y), ldy);
 }
 
-void sigma_ras_f32_io64(Ctx& c, int k, const double* X, int ldx, double* Y, int ldy) {
-  PSOCI_CHECK(c.have_ras, PSOCI_ERR_STATE, "the string-driven direct sigma is not set up");
-  ras_any_k<float, double>(c, k, X, ldx, Y, ldy);
-}
-
 }  // namespace psoci
