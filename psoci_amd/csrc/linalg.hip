@@ -23,6 +23,10 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 #ifndef PSOCI_NN_FULL
 #define PSOCI_NN_FULL 1
 #endif
+#ifndef PSOCI_TN_F32MFMA
+#define PSOCI_TN_F32MFMA 1
+#endif
+constexpr bool kTnF32Mfma = PSOCI_TN_F32MFMA;  // FP32-column projections: FP32 MFMA per 16-row step
 constexpr bool kNnFull = PSOCI_NN_FULL;  // unpredicated loads for whole chunks of the basis update
 // Columns of A per lane in the projection: 2 for FP32 columns, so that a
 // wavefront reads whole 128-byte row segments (32 floats, two output tiles)
@@ -92,6 +96,43 @@ __global__ __launch_bounds__(256) void k_gemm_tn_mfma(int64_t n, int m, int b, c
     }
   };
   if (r0 < r1) load(r0);
+  if constexpr (CP == 2 && kTnF32Mfma) {
+    // FP32 columns: each 16-row step summed on v_mfma_f32_16x16x4f32 (the
+    // products of FP32 values, 16 terms in FP32), the steps added in FP64
+    typedef float float4_t __attribute__((ext_vector_type(4)));
+    for (int64_t r = r0; r < r1; r += 16) {
+      float fa[2][4], fb[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        fa[0][u] = ra[0][u];
+        fa[1][u] = ra[1][u];
+        fb[u] = rb[u];
+      }
+      if (r + 16 < r1) load(r + 16);
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        float4_t p = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) p = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[c][u], fb[u], p, 0, 0, 0);
+        double4_t& q = acc[c][0];
+        q[0] += (double)p[0];
+        q[1] += (double)p[1];
+        q[2] += (double)p[2];
+        q[3] += (double)p[3];
+      }
+    }
+    // accumulator t of lane l holds output row 4 (l / 16) + t (FP32 layout)
+    double* out = partial + chunk * (int64_t)m * b;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int i = tm * 32 + 2 * (4 * (lane >> 4) + t) + c;
+        const int j = tb * 16 + (lane & 15);
+        if (i < m && j < b) out[(int64_t)i * b + j] = acc[c][0][t];
+      }
+    return;
+  }
   for (int64_t r = r0; r < r1; r += 16) {
     double a[CP][4], bb[4];
 #pragma unroll
