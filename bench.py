@@ -436,7 +436,7 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f32 sigma -> f64 (mixed); f64 subspace",
+        "dtype": "f32 sigma and f32 basis -> f64 sigma and basis (mixed); f64 subspace matrices and eigensolves",
         "data": ("synthetic (seeded integrals and configuration list, written to a SIFS-style file and a "
                  "configuration file and read back by the library)"),
         "config": {"workload": desc, "roots": k, "n_sef": n, "n_cfg": cfg.ncfg,
