@@ -404,15 +404,6 @@ __global__ void k_copy_cols_f64_to_f32(int64_t n, int k, const double* __restric
   B[row * ldb + c] = (float)A[row * lda + c];
 }
 
-__global__ void k_copy_cols_f32_to_f64(int64_t n, int k, const float* __restrict__ A, int lda, double* __restrict__ B,
-                                       int ldb) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t row = g / k;
-  const int c = (int)(g - row * k);
-  if (row >= n) return;
-  B[row * ldb + c] = (double)A[row * lda + c];
-}
-
 template <typename T>
 __global__ void k_copy_cols(int64_t n, int k, const T* __restrict__ A, int lda, T* __restrict__ B, int ldb) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -723,11 +714,6 @@ void Linalg::precondition_olsen(int64_t n, int k, const double* D, const double*
 
 void Linalg::to_f32(int64_t n, int k, const double* A, int lda, float* B, int ldb) {
   hipLaunchKernelGGL(k_copy_cols_f64_to_f32, dim3(nblk(n * k, 256)), dim3(256), 0, stream, n, k, A, lda, B, ldb);
-  PSOCI_HIP(hipGetLastError());
-}
-
-void Linalg::to_f64(int64_t n, int k, const float* A, int lda, double* B, int ldb) {
-  hipLaunchKernelGGL(k_copy_cols_f32_to_f64, dim3(nblk(n * k, 256)), dim3(256), 0, stream, n, k, A, lda, B, ldb);
   PSOCI_HIP(hipGetLastError());
 }
 

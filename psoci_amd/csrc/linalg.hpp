@@ -49,7 +49,6 @@ struct Linalg {
   void precondition_olsen(int64_t n, int k, const double* D, const double* theta_host, double dmin, double* R, int ldr,
                           const double* X, int ldx);
   void to_f32(int64_t n, int k, const double* A, int lda, float* B, int ldb);
-  void to_f64(int64_t n, int k, const float* A, int lda, double* B, int ldb);
   template <typename T>
   void copy_cols(int64_t n, int k, const T* A, int lda, T* B, int ldb);
   template <typename T>
