@@ -373,7 +373,10 @@ typedef struct {
  * (driver.PSOCI.splittest.C:895-1380) with PsociGAbasis / PsociGAsubspace
  * (subspace VtV/VtHV, dsygv, residuals, diagonal preconditioner with
  * DAVIDSON_MIN = 1e-3, Ritz collapse).  evals[k] (without ecore), evecs
- * [n][k] row-major host (may be NULL), stats may be NULL. */
+ * [n][k] row-major host (may be NULL), stats may be NULL.  Device memory
+ * for the basis V and HV, n x max_subspace each: FP64 with PSOCI_FP64, FP32
+ * with PSOCI_FP32, both with PSOCI_MIXED (its FP32 phase keeps an FP32
+ * basis); plus FP64 work blocks of n x k and n x 2k. */
 int psoci_davidson(psoci_ctx* ctx, const psoci_dav_params* params, double* evals, double* evecs,
                    psoci_dav_stats* stats);
 
