@@ -80,7 +80,8 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
   // the basis V and its sigma HV: FP64 for the FP64 phase, FP32 (V32,
   // HV32) for the FP32 phase; a mixed solve holds both
   DevBuf<double> V, HV, R, T;
-  DevBuf<float> V32, HV32, X32, Y32;
+  // R32: the FP32 phase's residual block (Olsen's correction keeps FP64 R)
+  DevBuf<float> V32, HV32, X32, Y32, R32;
   const bool want32 = p.prec != PSOCI_FP64;
   if (p.prec != PSOCI_FP32) {
     V.alloc((size_t)n * mmax);
@@ -93,6 +94,7 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     HV32.alloc((size_t)n * mmax);
     X32.alloc((size_t)n * k0);
     Y32.alloc((size_t)n * k0);
+    if (!olsen) R32.alloc((size_t)n * k0);
   }
   psoci_dav_stats S{};
   S.switch_iter = -1;
@@ -327,11 +329,19 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
     // residuals R = HV Yk - V Yk diag(w); with Olsen's correction also the
     // Ritz vectors X = V Yk (into T, free until the next collapse) in the
     // same pass over V and HV
-    on_basis([&](auto* Vb, auto* HVb) {
-      L.gemm_nn2(n, m, k, HVb, mmax, Yk.data(), Vb, mmax, Yk2.data(), R.p, k, olsen ? T.p : nullptr, k);
-    });
+    // in the FP32 phase (without Olsen's correction) the residual block is
+    // FP32: its norms and the preconditioned corrections need no more
+    const bool r32 = phase32 && !olsen;
     G.resize((size_t)k * k);
-    L.gemm_tn_host(n, k, k, R.p, k, R.p, k, G.data());
+    if (r32) {
+      L.gemm_nn2(n, m, k, HV32.p, mmax, Yk.data(), V32.p, mmax, Yk2.data(), R32.p, k);
+      L.gemm_tn_host(n, k, k, R32.p, k, R32.p, k, G.data());
+    } else {
+      on_basis([&](auto* Vb, auto* HVb) {
+        L.gemm_nn2(n, m, k, HVb, mmax, Yk.data(), Vb, mmax, Yk2.data(), R.p, k, olsen ? T.p : nullptr, k);
+      });
+      L.gemm_tn_host(n, k, k, R.p, k, R.p, k, G.data());
+    }
     double maxr = 0.0;
     for (int j = 0; j < k; ++j) {
       rn[j] = std::sqrt(std::max(G[(size_t)j * k + j], 0.0));
@@ -417,9 +427,10 @@ int davidson(Ctx& c, const psoci_dav_params& p, double* evals, double* evecs, ps
       for (int j = 0; j < nsel; ++j) Ssel[(size_t)sel[j] * nsel + j] = 1.0;
       on_basis([&](auto* Vb, auto*) { L.gemm_nn(n, k, nsel, R.p, k, Ssel.data(), 0.0, Vb + m, mmax); });
     } else {
-      on_basis([&](auto* Vb, auto*) {
-        L.precondition_select(n, k, c.d_diag.p, w.data(), 1.0e-3, R.p, k, sel.data(), nsel, Vb + m, mmax);
-      });
+      if (phase32)
+        L.precondition_select(n, k, c.d_diag.p, w.data(), 1.0e-3, R32.p, k, sel.data(), nsel, V32.p + m, mmax);
+      else
+        L.precondition_select(n, k, c.d_diag.p, w.data(), 1.0e-3, R.p, k, sel.data(), nsel, V.p + m, mmax);
     }
     int nb = nsel;
     {

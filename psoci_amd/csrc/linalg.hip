@@ -371,16 +371,16 @@ __global__ void k_olsen_apply(int64_t n, int k, const double* __restrict__ D, co
 // denominator exceeds dmin in magnitude, else R[row][sel[j]]: the diagonal
 // preconditioner applied to the selected columns on their way into the
 // basis (one pass; the same numbers as preconditioning R and then copying)
-template <typename TO>
+template <typename TR, typename TO>
 __global__ void k_precondition_select(int64_t n, int ns, const double* __restrict__ D,
                                       const double* __restrict__ theta_sel, const int* __restrict__ sel,
-                                      double dmin, const double* __restrict__ R, int ldr, TO* __restrict__ Out,
+                                      double dmin, const TR* __restrict__ R, int ldr, TO* __restrict__ Out,
                                       int ldo) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t row = g / ns;
   const int j = (int)(g - row * ns);
   if (row >= n) return;
-  const double r = R[row * ldr + sel[j]];
+  const double r = (double)R[row * ldr + sel[j]];
   const double den = D[row] - theta_sel[j];
   Out[row * ldo + j] = (TO)(fabs(den) > dmin ? r / den : r);
 }
@@ -648,9 +648,9 @@ void Linalg::gemm_nn_inplace(int64_t n, int m, int k, const double* Y1_host, T* 
                                       nullptr, 0);
 }
 
-template <typename TA>
+template <typename TA, typename TO>
 void Linalg::gemm_nn2(int64_t n, int m, int k, const TA* A, int lda, const double* Y1_host, const TA* B, int ldb,
-                      const double* Y2_host, double* Out, int ldo, double* X, int ldx) {
+                      const double* Y2_host, TO* Out, int ldo, double* X, int ldx) {
   if (k == 0 || n == 0) return;
   ysmall.reserve((size_t)2 * m * k);
   yneg.assign(Y2_host, Y2_host + (size_t)m * k);
@@ -659,11 +659,11 @@ void Linalg::gemm_nn2(int64_t n, int m, int k, const TA* A, int lda, const doubl
   PSOCI_HIP(hipMemcpyAsync(ysmall.p + (size_t)m * k, yneg.data(), (size_t)m * k * sizeof(double),
                            hipMemcpyHostToDevice, stream));
   if (X)
-    launch_nn<true, true, false, TA, double>(stream, n, m, k, A, lda, B, ldb, ysmall.p, ysmall.p + (size_t)m * k, 0.0,
-                                             Out, ldo, X, ldx);
+    launch_nn<true, true, false, TA, TO>(stream, n, m, k, A, lda, B, ldb, ysmall.p, ysmall.p + (size_t)m * k, 0.0,
+                                         Out, ldo, X, ldx);
   else
-    launch_nn<true, false, false, TA, double>(stream, n, m, k, A, lda, B, ldb, ysmall.p, ysmall.p + (size_t)m * k,
-                                              0.0, Out, ldo, nullptr, 0);
+    launch_nn<true, false, false, TA, TO>(stream, n, m, k, A, lda, B, ldb, ysmall.p, ysmall.p + (size_t)m * k, 0.0,
+                                          Out, ldo, nullptr, 0);
   // yneg is read by the copy above: keep it until the stream has consumed it
   PSOCI_HIP(hipStreamSynchronize(stream));
 }
@@ -675,9 +675,9 @@ void Linalg::precondition(int64_t n, int k, const double* D, const double* theta
   PSOCI_HIP(hipGetLastError());
 }
 
-template <typename TO>
+template <typename TR, typename TO>
 void Linalg::precondition_select(int64_t n, int k, const double* D, const double* theta_host, double dmin,
-                                 const double* R, int ldr, const int* sel_host, int ns, TO* Out, int ldo) {
+                                 const TR* R, int ldr, const int* sel_host, int ns, TO* Out, int ldo) {
   if (ns == 0 || n == 0) return;
   std::vector<double> ts(ns);
   for (int j = 0; j < ns; ++j) ts[j] = theta_host[sel_host[j]];
@@ -686,7 +686,7 @@ void Linalg::precondition_select(int64_t n, int k, const double* D, const double
   isel.reserve(ns);
   PSOCI_HIP(hipMemcpyAsync(theta.p, ts.data(), ns * sizeof(double), hipMemcpyHostToDevice, stream));
   PSOCI_HIP(hipMemcpyAsync(isel.p, sel_host, ns * sizeof(int), hipMemcpyHostToDevice, stream));
-  hipLaunchKernelGGL(k_precondition_select<TO>, dim3(nblk(n * ns, 256)), dim3(256), 0, stream, n, ns, D, theta.p,
+  hipLaunchKernelGGL((k_precondition_select<TR, TO>), dim3(nblk(n * ns, 256)), dim3(256), 0, stream, n, ns, D, theta.p,
                      isel.p, dmin, R, ldr, Out, ldo);
   PSOCI_HIP(hipGetLastError());
   // ts is read by the copy above: keep it until the stream has consumed it
@@ -772,14 +772,18 @@ template void Linalg::gemm_nn_rows<double>(int64_t, int, int, const double*, int
 template void Linalg::gemm_nn_rows<float>(int64_t, int, int, const float*, int, const double*, float*, int);
 template void Linalg::gemm_nn_inplace<double>(int64_t, int, int, const double*, double*, int);
 template void Linalg::gemm_nn_inplace<float>(int64_t, int, int, const double*, float*, int);
-template void Linalg::gemm_nn2<double>(int64_t, int, int, const double*, int, const double*, const double*, int,
-                                       const double*, double*, int, double*, int);
-template void Linalg::gemm_nn2<float>(int64_t, int, int, const float*, int, const double*, const float*, int,
-                                      const double*, double*, int, double*, int);
-template void Linalg::precondition_select<double>(int64_t, int, const double*, const double*, double, const double*,
-                                                  int, const int*, int, double*, int);
-template void Linalg::precondition_select<float>(int64_t, int, const double*, const double*, double, const double*,
-                                                 int, const int*, int, float*, int);
+template void Linalg::gemm_nn2<double, double>(int64_t, int, int, const double*, int, const double*, const double*,
+                                               int, const double*, double*, int, double*, int);
+template void Linalg::gemm_nn2<float, double>(int64_t, int, int, const float*, int, const double*, const float*, int,
+                                              const double*, double*, int, double*, int);
+template void Linalg::gemm_nn2<float, float>(int64_t, int, int, const float*, int, const double*, const float*, int,
+                                             const double*, float*, int, double*, int);
+template void Linalg::precondition_select<double, double>(int64_t, int, const double*, const double*, double,
+                                                          const double*, int, const int*, int, double*, int);
+template void Linalg::precondition_select<double, float>(int64_t, int, const double*, const double*, double,
+                                                         const double*, int, const int*, int, float*, int);
+template void Linalg::precondition_select<float, float>(int64_t, int, const double*, const double*, double,
+                                                        const float*, int, const int*, int, float*, int);
 template void Linalg::copy_cols<double>(int64_t, int, const double*, int, double*, int);
 template void Linalg::copy_cols<float>(int64_t, int, const float*, int, float*, int);
 template void Linalg::unit_cols<double>(int64_t, int, const int64_t*, double*, int);

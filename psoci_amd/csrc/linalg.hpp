@@ -33,16 +33,16 @@ struct Linalg {
   // buffer at other or overlapping columns (k <= 128)
   template <typename T>
   void gemm_nn_rows(int64_t n, int m, int k, const T* A, int lda, const double* Y1_host, T* Out, int ldo);
-  // Out[:, 0:k) = A Y1 - B Y2 (Y1, Y2 host, m x k; Out FP64); with X set,
-  // also X = B Y1 in the same pass over A and B (synchronises)
-  template <typename TA>
+  // Out[:, 0:k) = A Y1 - B Y2 (Y1, Y2 host, m x k); with X set, also
+  // X = B Y1 (FP64) in the same pass over A and B (synchronises)
+  template <typename TA, typename TO>
   void gemm_nn2(int64_t n, int m, int k, const TA* A, int lda, const double* Y1_host, const TA* B, int ldb,
-                const double* Y2_host, double* Out, int ldo, double* X = nullptr, int ldx = 0);
+                const double* Y2_host, TO* Out, int ldo, double* X = nullptr, int ldx = 0);
   // R[:, c] /= (D - theta_c) where |D - theta_c| > dmin
   void precondition(int64_t n, int k, const double* D, const double* theta_host, double dmin, double* R, int ldr);
   // Out[:, j) = the diagonal preconditioner applied to R[:, sel[j]] (j < ns)
-  template <typename TO>
-  void precondition_select(int64_t n, int k, const double* D, const double* theta_host, double dmin, const double* R,
+  template <typename TR, typename TO>
+  void precondition_select(int64_t n, int k, const double* D, const double* theta_host, double dmin, const TR* R,
                            int ldr, const int* sel_host, int ns, TO* Out, int ldo);
   // the same with Olsen's correction against the Ritz vectors X:
   // R[:, c] = f (R - eps_c X) with eps_c = (x . f r) / (x . f x)
